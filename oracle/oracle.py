@@ -1,0 +1,208 @@
+"""ctypes front-end of the CPU FP64 oracle (flosic_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-baseline leg of bench.py,
+always as the checker.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class ObBasis(C.Structure):
+    _fields_ = [("natom", C.c_int32), ("xyz", _dp), ("bare_off", _ip), ("alpha", _dp),
+                ("ncon", _ip), ("con_off", _ip), ("con", _dp)]
+
+
+class ObSpin(C.Structure):
+    _fields_ = [("nwf", C.c_int32), ("nocc", C.c_int32), ("coef", _dp), ("fod", _dp),
+                ("fmat", _dp), ("over", _dp), ("q", _dp), ("U", _dp), ("tmat", _dp),
+                ("results", _dp), ("sic", _dp), ("vab", _dp), ("debdax", _dp), ("grad", _dp),
+                ("vpot", _dp), ("vh", _dp)]
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "flosic_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_eval.restype = C.c_int
+        L.oracle_eval.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, C.c_int, C.POINTER(ObSpin),
+                                  _dp, C.c_int]
+        L.oracle_boys.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.oracle_basis_eval.argtypes = [C.POINTER(ObBasis), C.c_int, C.c_int, _dp, _dp]
+        L.oracle_pairs.restype = C.c_int
+        L.oracle_pairs.argtypes = [C.POINTER(ObBasis), C.c_int, _ip]
+        L.oracle_poisson_batch.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]
+        L.oracle_xc_points.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.oracle_eigh.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.oracle_fod_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]
+        L.oracle_xc_kernel.argtypes = [C.c_int, _dp, _dp]
+        L.oracle_nbas.restype = C.c_int
+        L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class _BasisHold:
+    def __init__(self, b):
+        self.xyz = np.ascontiguousarray(b.xyz, dtype=np.float64)
+        self.bare_off = np.ascontiguousarray(b.bare_off, dtype=np.int32)
+        self.alpha = np.ascontiguousarray(b.alpha, dtype=np.float64)
+        self.ncon = np.ascontiguousarray(b.ncon, dtype=np.int32).ravel()
+        self.con_off = np.ascontiguousarray(b.con_off, dtype=np.int32).ravel()
+        self.con = np.ascontiguousarray(b.con, dtype=np.float64)
+        self.s = ObBasis(b.natom, _d(self.xyz), _i(self.bare_off), _d(self.alpha), _i(self.ncon),
+                         _i(self.con_off), _d(self.con))
+
+
+def boys(top: int, T) -> np.ndarray:
+    T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+    out = np.zeros((T.size, top + 1))
+    lib().oracle_boys(top, T.size, _d(T), _d(out))
+    return out
+
+
+def basis_eval(basis, pts, nderv: int = 0) -> np.ndarray:
+    h = _BasisHold(basis)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    ng = {0: 1, 1: 4, 2: 10}[nderv]
+    out = np.zeros((ng, pts.shape[0], basis.nbas))
+    lib().oracle_basis_eval(C.byref(h.s), nderv, pts.shape[0], _d(pts), _d(out))
+    return out
+
+
+def pairs(basis) -> np.ndarray:
+    h = _BasisHold(basis)
+    n = lib().oracle_pairs(C.byref(h.s), 0, _i(np.zeros(6, np.int32)))
+    out = np.zeros((max(n, 1), 6), np.int32)
+    lib().oracle_pairs(C.byref(h.s), n, _i(out))
+    return out[:n]
+
+
+def poisson_batch(alpha, A, beta, B, rho, pts) -> np.ndarray:
+    """Potential of pairs with Fortran-ordered RHO(10,10,npair) (element (mu,nu) = rho[ip, nu, mu])."""
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    beta = np.ascontiguousarray(beta, np.float64)
+    A = np.ascontiguousarray(A, np.float64)
+    B = np.ascontiguousarray(B, np.float64)
+    rho = np.ascontiguousarray(rho, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    V = np.zeros(pts.shape[0])
+    lib().oracle_poisson_batch(alpha.size, _d(alpha), _d(A), _d(beta), _d(B), _d(rho), pts.shape[0], _d(pts), _d(V))
+    return V
+
+
+def xc_points(xc: int, rhog) -> tuple:
+    rhog = np.ascontiguousarray(rhog, np.float64).reshape(-1, 10)
+    n = rhog.shape[0]
+    v = np.zeros(n)
+    e = np.zeros((n, 4))
+    lib().oracle_xc_points(xc, n, _d(rhog), _d(v), _d(e))
+    return v, e
+
+
+def xc_kernel(which: int, args) -> np.ndarray:
+    """0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v) 3 PBECOR(rs,zet,t,uu,vv,ww)."""
+    a = np.ascontiguousarray(args, np.float64)
+    out = np.zeros(4)
+    lib().oracle_xc_kernel(which, _d(a), _d(out))
+    return out
+
+
+def eigh(A):
+    A = np.ascontiguousarray(A, np.float64)
+    n = A.shape[0]
+    q = np.zeros(n)
+    U = np.zeros((n, n))
+    lib().oracle_eigh(n, _d(A), _d(q), _d(U))
+    return q, U.reshape(n, n).T  # column-major -> U[:, k] eigenvector k
+
+
+def fod_forces(fmat, U, q, sinvh, tmat, sic, ldsic):
+    n = q.size
+    deb = np.zeros(3 * n)
+    grad = np.zeros(3 * n)
+    f = np.ascontiguousarray(fmat)  # [K][i][j] memory == FMAT(j,i,K)
+    lib().oracle_fod_forces(n, _d(f), _d(np.asfortranarray(U)), _d(np.ascontiguousarray(q)),
+                            _d(np.asfortranarray(sinvh)), _d(np.asfortranarray(tmat)),
+                            _d(np.asfortranarray(sic)), ldsic, _d(deb), _d(grad))
+    return deb.reshape(n, 3), grad.reshape(n, 3)
+
+
+def evaluate(sysm, nthreads: int = 0, want_potentials: bool = False, want_vab: bool = True,
+             fods=None, coefs=None) -> dict:
+    """Full FP64 evaluation of a System: returns per-spin dicts and E_SIC."""
+    h = _BasisHold(sysm.basis)
+    pts = np.ascontiguousarray(sysm.mesh.xyz, np.float64)
+    w = np.ascontiguousarray(sysm.mesh.w, np.float64)
+    M = w.size
+    spins = (ObSpin * 2)()
+    keep = []
+    outs = []
+    for s in range(2):
+        C_ = np.asfortranarray((coefs or sysm.coef)[s], np.float64)
+        fod = np.ascontiguousarray((fods or sysm.fod)[s], np.float64)
+        n = fod.shape[0]
+        nwf = C_.shape[1]
+        o = dict(fmat=np.zeros((4, n, n)), over=np.zeros((n, n)), q=np.zeros(n), U=np.zeros((n, n)),
+                 tmat=np.zeros((n, n)), results=np.zeros((n, 13)), sic=np.zeros((n, nwf)),
+                 vab=np.zeros((n, n, n)) if want_vab else None, debdax=np.zeros((n, 3)), grad=np.zeros((n, 3)),
+                 vpot=np.zeros((n, M)) if want_potentials else None, vh=np.zeros((n, M)) if want_potentials else None)
+        keep += [C_, fod]
+        sp = spins[s]
+        sp.nwf = nwf
+        sp.nocc = n
+        sp.coef = _d(C_)
+        sp.fod = _d(fod)
+        for k in ("fmat", "over", "q", "U", "tmat", "results", "sic", "vab", "debdax", "grad", "vpot", "vh"):
+            setattr(sp, k, _d(o[k]) if o[k] is not None else _dp())
+        outs.append(o)
+    esic = np.zeros(1)
+    st = lib().oracle_eval(C.byref(h.s), M, _d(pts), _d(w), int(sysm.nspn), int(sysm.xc), spins, _d(esic),
+                           int(nthreads))
+    res = dict(status=int(st), esic=float(esic[0]), spins=[])
+    for s in range(2):
+        o = outs[s]
+        n = o["q"].size
+        # raw buffers are column-major (Fortran); present them as natural numpy arrays
+        d = dict(
+            fmat=o["fmat"],                               # fmat[K, i, j] = FMAT(j,i,K)
+            over=o["over"].T.copy(),
+            q=o["q"],
+            U=o["U"].T.copy(),                            # U[:, a] eigenvector a
+            tmat=o["tmat"].T.copy(),                      # tmat[i, j] = TMAT(i,j)
+            results=o["results"],                         # results[i, 0:13]
+            sic=o["sic"].T.copy(),                        # sic[j, i] = SIC(j,i)
+            vab=None if o["vab"] is None else np.transpose(o["vab"], (0, 2, 1)).copy(),  # vab[i, a, b]
+            debdax=o["debdax"], grad=o["grad"], vpot=o["vpot"], vh=o["vh"])
+        res["spins"].append(d)
+    return res

@@ -1,0 +1,293 @@
+"""Pins of the CPU FP64 oracle: against the reference's own routines (golden vectors written by
+tests/golden/make_ref_vectors.py from the reference sources), against closed forms, and against
+invariants the method fixes.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import oracle as O
+from publicrelease_2020_amd import system as S
+
+
+# ---------------------------------------------------------------------------------------------- Boys
+@pytest.mark.parametrize("top", [2, 3, 4])
+def test_boys_matches_reference_fffmt(golden, top):
+    """FFFMT3/FFFMT4/FFFMTC (fffmt3.F:24-88): same table, Taylor and recursion -> bitwise-level agreement."""
+    F = O.boys(top, golden["boys_T"])
+    np.testing.assert_allclose(F, golden[f"boys_F{top}"], rtol=1e-15, atol=0)
+
+
+def test_boys_closed_form():
+    """F_m(T) = gamma(m+1/2, T) / (2 T^(m+1/2)); the reference scheme is accurate to ~5e-10 relative
+    just above its T=16 switch (one-term erfc asymptote)."""
+    T = np.concatenate([[1e-12], np.linspace(1e-3, 40.0, 4000)])
+    F = O.boys(4, T)
+    for m in range(5):
+        exact = special.gammainc(m + 0.5, T) * special.gamma(m + 0.5) / (2.0 * T ** (m + 0.5))
+        np.testing.assert_allclose(F[:, m], exact, rtol=2e-9, atol=0)
+    np.testing.assert_allclose(O.boys(4, [0.0])[0], [1.0 / (2 * m + 1) for m in range(5)], rtol=1e-15)
+
+
+# ------------------------------------------------------------------------------------------ Coulomb
+def test_poisson_matches_reference_exact_mode(golden):
+    """POISSON2/POISSON1 with DEBUG (every point analytic, poisson1.f:1105-1110), random s/p/d RHO(10,10)."""
+    V = O.poisson_batch(golden["poisson_alpha"], golden["poisson_A"], golden["poisson_beta"], golden["poisson_B"],
+                        golden["poisson_rho"], golden["poisson_pts"])
+    ref = golden["poisson_V"]
+    assert np.abs(V - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def _ss_potential(al, A, be, B, pts):
+    p = al + be
+    P = (al * A + be * B) / p
+    K = math.exp(-al * be / p * float(((A - B) ** 2).sum()))
+    d = np.linalg.norm(pts - P, axis=1)
+    return K * (math.pi / p) ** 1.5 * special.erf(math.sqrt(p) * d) / d
+
+
+def _one_pair(al, A, be, B, mu, nu, pts):
+    rho = np.zeros((1, 10, 10))
+    rho[0, nu, mu] = 1.0
+    return O.poisson_batch([al], A[None], [be], B[None], rho, pts)
+
+
+def test_poisson_closed_forms():
+    """s-s: K (pi/p)^1.5 erf(sqrt(p) d)/d.  p_x on A = (1/2a) d/dA_x of s-s; d_xy on A = (1/2a)^2 d2/dA_x dA_y;
+    d_xx on A = (1/4a^2)(d2/dA_x^2 + 2a) s-s.  Central differences of the erf form pin s, p and d."""
+    rng = np.random.default_rng(3)
+    al, be = 1.3, 0.7
+    A = np.array([0.2, -0.4, 0.3])
+    B = np.array([-0.5, 0.6, 0.1])
+    pts = rng.uniform(-3, 3, (50, 3))
+    # rtol 1e-9: the T>16 one-term asymptote of the reference Boys scheme is good to ~5e-10
+    np.testing.assert_allclose(_one_pair(al, A, be, B, 0, 0, pts), _ss_potential(al, A, be, B, pts), rtol=1e-9)
+    h = 1e-4
+    e = np.eye(3)
+    dx = (_ss_potential(al, A + h * e[0], be, B, pts) - _ss_potential(al, A - h * e[0], be, B, pts)) / (2 * h)
+    np.testing.assert_allclose(_one_pair(al, A, be, B, 1, 0, pts), dx / (2 * al), rtol=1e-6, atol=1e-9)
+    dyB = (_ss_potential(al, A, be, B + h * e[1], pts) - _ss_potential(al, A, be, B - h * e[1], pts)) / (2 * h)
+    np.testing.assert_allclose(_one_pair(al, A, be, B, 0, 2, pts), dyB / (2 * be), rtol=1e-6, atol=1e-9)
+    h = 1e-3
+    f = lambda dA: _ss_potential(al, A + dA, be, B, pts)
+    dxy = (f(h * (e[0] + e[1])) - f(h * (e[0] - e[1])) - f(h * (-e[0] + e[1])) + f(-h * (e[0] + e[1]))) / (4 * h * h)
+    np.testing.assert_allclose(_one_pair(al, A, be, B, 7, 0, pts), dxy / (4 * al * al), rtol=2e-5, atol=1e-7)
+    dxx = (f(h * e[0]) - 2 * f(0 * e[0]) + f(-h * e[0])) / (h * h)
+    np.testing.assert_allclose(_one_pair(al, A, be, B, 4, 0, pts), (dxx + 2 * al * f(0 * e[0])) / (4 * al * al),
+                               rtol=2e-5, atol=1e-7)
+
+
+def test_poisson_far_field_is_total_charge():
+    """Far from the pair the potential is Q/r with Q the analytic overlap of the two Gaussians."""
+    al, be = 2.0, 0.9
+    A = np.array([0.1, 0.2, -0.3])
+    B = np.array([-0.2, 0.1, 0.4])
+    far = np.array([[300.0, -100.0, 50.0]])
+    V = _one_pair(al, A, be, B, 0, 0, far)[0]
+    p = al + be
+    Q = math.exp(-al * be / p * float(((A - B) ** 2).sum())) * (math.pi / p) ** 1.5
+    P = (al * A + be * B) / p
+    assert abs(V - Q / np.linalg.norm(far[0] - P)) < 1e-14
+
+
+# --------------------------------------------------------------------------------------------- basis
+def test_basis_matches_reference_gorbdrv(golden):
+    b = S.build_basis(np.zeros((1, 3)), [8])
+    out = O.basis_eval(b, golden["gorbdrv_pts"], nderv=2)
+    ref = golden["gorbdrv_out"]
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_basis_derivatives_finite_difference():
+    b = S.build_basis(np.array([[0.0, 0.0, 0.0], [1.2, -0.4, 0.7]]), [6, 1])
+    pts = np.random.default_rng(4).uniform(-1.0, 1.5, (20, 3))
+    v = O.basis_eval(b, pts, nderv=2)
+    h = 1e-5
+    e = np.eye(3)
+    for d in range(3):
+        fd = (O.basis_eval(b, pts + h * e[d], 0)[0] - O.basis_eval(b, pts - h * e[d], 0)[0]) / (2 * h)
+        np.testing.assert_allclose(v[1 + d], fd, rtol=1e-6, atol=1e-6 * np.abs(v[1 + d]).max())
+    hess_idx = {(0, 0): 4, (1, 1): 5, (2, 2): 6, (0, 1): 7, (0, 2): 8, (1, 2): 9}
+    for (d, f), k in hess_idx.items():
+        fd = (O.basis_eval(b, pts + h * e[f], 1)[1 + d] - O.basis_eval(b, pts - h * e[f], 1)[1 + d]) / (2 * h)
+        np.testing.assert_allclose(v[k], fd, rtol=1e-5, atol=1e-5 * np.abs(v[k]).max())
+
+
+# ------------------------------------------------------------------------------------------------ XC
+def test_xc_kernels_match_reference(golden):
+    xin, cin = golden["xc_xin"], golden["xc_cin"]
+    for i in range(xin.shape[0]):
+        np.testing.assert_allclose(O.xc_kernel(0, xin[i])[:3], golden["xc_pw91ex"][i], rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(O.xc_kernel(2, xin[i])[:3], golden["xc_pbeex"][i], rtol=1e-13, atol=1e-15)
+    for i in range(cin.shape[0]):
+        ref = golden["xc_pw91lc"][i]
+        np.testing.assert_allclose(O.xc_kernel(1, cin[i, :2])[:2], ref[:2], rtol=1e-13, atol=1e-15)
+        ref = golden["xc_pbecor"][i]
+        got = O.xc_kernel(3, cin[i])
+        np.testing.assert_allclose(got[[0, 1, 2, 3]], ref[[0, 1, 3, 4]], rtol=1e-12, atol=1e-14)
+
+
+def test_lsda_potential_is_energy_derivative():
+    """LSDA: v_xc(rho) = d(rho eps_xc)/d rho pointwise (rho_down = 0)."""
+    rho = np.logspace(-8, 3, 60)
+    h = 1e-6 * rho
+    rg = np.zeros((rho.size, 10))
+    rg[:, 0] = rho
+    v, _ = O.xc_points(S.XC_LSDA, rg)
+    rg[:, 0] = rho + h
+    _, ep = O.xc_points(S.XC_LSDA, rg)
+    rg[:, 0] = rho - h
+    _, em = O.xc_points(S.XC_LSDA, rg)
+    fd = (ep.sum(1) - em.sum(1)) / (2 * h)
+    np.testing.assert_allclose(v, fd, rtol=1e-7)
+
+
+def _radial_fields(r, cs, es):
+    """rho(r) = sum c e^{-e r^2} on the ray r*(1,1,1)/sqrt(3): value, gradient, Hessian (10 comps)."""
+    u = np.ones(3) / math.sqrt(3.0)
+    f = sum(c * np.exp(-e * r * r) for c, e in zip(cs, es))
+    f1 = sum(-2 * e * r * c * np.exp(-e * r * r) for c, e in zip(cs, es))
+    f2 = sum((4 * e * e * r * r - 2 * e) * c * np.exp(-e * r * r) for c, e in zip(cs, es))
+    rg = np.zeros((r.size, 10))
+    rg[:, 0] = f
+    rg[:, 1:4] = f1[:, None] * u
+    H = f2[:, None, None] * np.outer(u, u) + (f1 / r)[:, None, None] * (np.eye(3) - np.outer(u, u))
+    rg[:, 4], rg[:, 5], rg[:, 6] = H[:, 0, 0], H[:, 1, 1], H[:, 2, 2]
+    rg[:, 7], rg[:, 8], rg[:, 9] = H[:, 0, 1], H[:, 0, 2], H[:, 1, 2]
+    return rg
+
+
+def test_pbe_potential_is_functional_derivative():
+    """PBE potential in the reference's PW86 (s,u,v) form equals dE_xc/d rho: for a spherical density
+    perturbed by g, dE/d(delta) = int v g (radial quadrature, 40k points)."""
+    r = np.linspace(1e-4, 14.0, 40000)
+    wr = 4 * math.pi * r * r * np.gradient(r)
+    cs, es = [1.0, 0.08], [1.0, 0.25]
+    rg = _radial_fields(r, cs, es)
+    v, _ = O.xc_points(S.XC_PBE, rg)
+    g = _radial_fields(r, [1.0], [0.6])
+    dl = 1e-4
+    _, ep = O.xc_points(S.XC_PBE, rg + dl * g)
+    _, em = O.xc_points(S.XC_PBE, rg - dl * g)
+    dE = ((ep.sum(1) - em.sum(1)) * wr).sum() / (2 * dl)
+    dv = (v * g[:, 0] * wr).sum()
+    assert abs(dE - dv) < 2e-6 * abs(dv)
+
+
+# -------------------------------------------------------------------------------- FO, Lowdin, forces
+def _lowdin_from_fmat(fm):
+    T = fm[0]          # T[i, j] = FMAT(j,i,1)
+    Smat = T @ T.T
+    q, U = O.eigh(Smat)
+    sinvh = (U / np.sqrt(q)) @ U.T
+    return q, U, sinvh, sinvh @ T
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+def test_fod_force_matches_reference_frmorb2(golden, spin):
+    """DEBDAX of FRMORB2(-ISPN,-1) (frmorb2.F:589-729) and SLAT = S^-1/2 T for the same FMAT/SIC."""
+    fm = golden[f"frm_fmat{spin}"]
+    q, U, sinvh, tmat = _lowdin_from_fmat(fm)
+    np.testing.assert_allclose(tmat, golden[f"frm_slat{spin}"], atol=1e-13)
+    sic = golden[f"frm_sic{spin}"]
+    deb, grad = O.fod_forces(np.ascontiguousarray(fm), U, q, sinvh, tmat, sic, sic.shape[0])
+    np.testing.assert_allclose(deb, golden[f"frm_debdax{spin}"], atol=1e-14)
+    # the reference formula is the exact back-propagation of dE/dTMAT = 2 SIC^T TMAT through Lowdin
+    np.testing.assert_allclose(grad, deb, atol=1e-13)
+
+
+def test_exact_gradient_equals_reference_formula(tiny_oracle):
+    for sp in tiny_oracle["spins"]:
+        np.testing.assert_allclose(sp["grad"], sp["debdax"], atol=1e-12)
+
+
+def test_fd_gradient_converges_to_analytic():
+    """Central differences of the discrete E_SIC (h=1e-4) agree with the analytic FOD gradient up to the
+    mesh quadrature error (2.6e-4 on 24x50, 3e-6 on 80x302 for this system)."""
+    sy = S.make_synthetic(3, 3, 7, nrad=80, nang=302, box=4.5)
+    res = O.evaluate(sy, want_vab=False)
+    h = 1e-4
+    for (s, i, x) in [(0, 0, 0), (1, 2, 1)]:
+        fp = [f.copy() for f in sy.fod]
+        fm = [f.copy() for f in sy.fod]
+        fp[s][i, x] += h
+        fm[s][i, x] -= h
+        fd = (O.evaluate(sy, fods=fp, want_vab=False)["esic"] - O.evaluate(sy, fods=fm, want_vab=False)["esic"]) / (2 * h)
+        assert abs(fd - res["spins"][s]["grad"][i, x]) < 1e-5
+
+
+def test_lambda_ii_equals_xdel(tiny_oracle):
+    """SICLAGM's lambda_ii and APOTNL_SIC's RESULTS(7) are one number from two code paths."""
+    for sp in tiny_oracle["spins"]:
+        n = sp["q"].size
+        np.testing.assert_allclose(np.diag(sp["sic"][:n, :n]), sp["results"][:, 6], rtol=1e-12, atol=1e-14)
+        # V^i_ab with b = i is SIC(a,i) up to sign
+        np.testing.assert_allclose(np.array([sp["vab"][i, :, i] for i in range(n)]).T, -sp["sic"][:n, :n],
+                                   rtol=1e-11, atol=1e-13)
+
+
+def test_flo_invariant_under_occupied_rotation(tiny_system, tiny_oracle):
+    """F_i depends on the occupied space only: E_SIC, RESULTS and the FOD gradient are unchanged when the
+    occupied KS orbitals are rotated among themselves."""
+    rng = np.random.default_rng(9)
+    coefs = []
+    for C_, n in zip(tiny_system.coef, tiny_system.nocc):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        C2 = C_.copy()
+        C2[:, :n] = C_[:, :n] @ Q
+        coefs.append(C2)
+    r2 = O.evaluate(tiny_system, coefs=coefs)
+    assert abs(r2["esic"] - tiny_oracle["esic"]) < 1e-12
+    for s in range(2):
+        np.testing.assert_allclose(r2["spins"][s]["results"], tiny_oracle["spins"][s]["results"], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(r2["spins"][s]["debdax"], tiny_oracle["spins"][s]["debdax"], atol=1e-11)
+
+
+def test_single_flo_force_is_zero():
+    """One FOD per spin: the FLO is the occupied orbital itself, so the FOD force vanishes anywhere."""
+    sy = S.make_synthetic(2, 1, 5, nrad=20, nang=38, box=3.0)
+    r = O.evaluate(sy)
+    for sp in r["spins"]:
+        assert np.abs(sp["debdax"]).max() < 1e-15
+        assert np.abs(sp["grad"]).max() < 1e-15
+
+
+def test_coincident_fods_report_lowdin_error(tiny_system):
+    f = [x.copy() for x in tiny_system.fod]
+    f[0][1] = f[0][0]
+    assert O.evaluate(tiny_system, fods=f)["status"] == 2
+
+
+def test_orbital_charge_integrates_to_one(tiny_oracle):
+    for sp in tiny_oracle["spins"]:
+        np.testing.assert_allclose(sp["results"][:, 1], 1.0, atol=2e-2)
+        assert np.all(sp["results"][:, 2] == 0.0)
+
+
+def test_pair_list_follows_screening_rule(tiny_system):
+    """Primitive shells (atom, l, bare) with a nonzero contraction coefficient; pairs ps1<=ps2 kept unless
+    a*b/(a+b)*R^2 > CUTEXP (coupot_sic.F:293-310)."""
+    b = tiny_system.basis
+    ps = []
+    for a in range(b.natom):
+        for l in range(3):
+            if b.ncon[a, l] == 0:
+                continue
+            blk = b.con_block(a, l)
+            for i in range(b.nbare(a)):
+                if np.any(np.abs(blk[:, i]) > 1e-30):
+                    ps.append((a, l, i))
+    want = []
+    for i in range(len(ps)):
+        for j in range(i, len(ps)):
+            x, y = ps[i], ps[j]
+            ai = b.alpha[b.bare_off[x[0]] + x[2]]
+            aj = b.alpha[b.bare_off[y[0]] + y[2]]
+            r2 = float(((b.xyz[x[0]] - b.xyz[y[0]]) ** 2).sum())
+            if ai * aj * (1.0 / (ai + aj)) * r2 > S.CUTEXP:
+                continue
+            want.append(x + y)
+    got = O.pairs(b)
+    assert got.shape[0] == len(want)
+    assert np.array_equal(got, np.array(want, dtype=np.int32))
