@@ -1,0 +1,125 @@
+/*
+ * flosic_hip.h -- C ABI of the MI355X FLO-SIC per-Fermi-orbital evaluation library (libflosic_hip.so).
+ *
+ * The reference (FLOSIC/PublicRelease_2020, Fortran) passes the hot path's data through module variables
+ * and files.  Each entry point below replaces one of its subroutines; the module data they read is handed
+ * over explicitly by the set-up calls.  Arrays are column-major with 1-based meaning in the comments and
+ * 0-based storage, REAL*8 -> double, INTEGER -> int32_t.  Every call returns a status (0 = success) and
+ * never aborts the process; flosic_last_error() gives the message.  Calls on one context are not
+ * re-entrant; distinct contexts are independent (one HIP stream each).
+ *
+ * Typical sequence (mirrors main.F:577-775):
+ *   flosic_create -> flosic_set_basis -> flosic_set_mesh -> flosic_set_options
+ *   per SCF/FOD step: flosic_set_orbitals(1|2) -> flosic_set_fods(1|2) -> flosic_evaluate
+ *   then flosic_get_results / flosic_get_sic / flosic_get_vab / flosic_get_fod_forces / flosic_get_fod_gradient
+ */
+#ifndef FLOSIC_HIP_H
+#define FLOSIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct flosic_ctx flosic_ctx;
+
+/* status codes */
+#define FLOSIC_OK 0
+#define FLOSIC_EINVAL 1   /* bad argument or inconsistent sizes */
+#define FLOSIC_ELOWDIN 2  /* Fermi-orbital overlap eigenvalue < 1e-8: LOWSIC STOPIT, lowden_files.f:71-99 */
+#define FLOSIC_ECOUPOT 3  /* "COUPOT IS ZERO": sum |V_H| < 0.001, apotnl_sic.F:277-283 */
+#define FLOSIC_EDEVICE 4  /* HIP / rocBLAS runtime failure */
+#define FLOSIC_ESTATE 5   /* a required set-up call is missing */
+#define FLOSIC_ENOMEM 6   /* device allocation failed */
+
+/* exchange-correlation: the two functionals of the SIC path (setdftyp.F:5-110) */
+#define FLOSIC_XC_LSDA 0 /* LDA-PW91*LDA-PW91 : PW91EX at s=u=v=0 + PW91LC */
+#define FLOSIC_XC_PBE 1  /* GGA-PBE*GGA-PBE   : PBEEX + PBECOR */
+
+/* arithmetic of the mesh contractions */
+#define FLOSIC_PREC_FP64 0  /* FP64 throughout */
+#define FLOSIC_PREC_MIXED 1 /* FP32 mesh contractions, FP64 Lowdin, Boys, energies and block sums */
+
+/* Context on HIP device `device` (a hipSetDevice index). */
+int32_t flosic_create(int32_t device, flosic_ctx **ctx);
+int32_t flosic_destroy(flosic_ctx *ctx);
+const char *flosic_last_error(const flosic_ctx *ctx);
+
+/* Basis: module COMMON2 (BFALP, BFCON, N_BARE, N_CON, RIDT) as consumed by GORBDRV (gorbdrv.f:18) and
+ * COUPOT_SIC (coupot_sic.F:228-482).  Atom a has bare exponents alpha[bare_off[a] .. bare_off[a+1]); for
+ * l = 0,1,2 it has ncon[3a+l] contractions whose coefficients over its bare exponents are the row-major
+ * block con[con_off[3a+l] ...] of shape ncon x nbare (normalisation folded in).  Basis functions are
+ * ordered atom, l, Cartesian component (s; x,y,z; xx,yy,zz,xy,xz,yz), contraction (frmorb2.F:262-266).
+ * Builds the primitive pair list and the FOD-independent basis planes on the mesh (if a mesh is set). */
+int32_t flosic_set_basis(flosic_ctx *ctx, int32_t natom, const double *xyz, const int32_t *bare_off,
+                         const double *alpha, const int32_t *ncon, const int32_t *con_off, const double *con);
+
+/* Mesh: module MESH1 (RMSH(3,NMSH), WMSH(NMSH)). */
+int32_t flosic_set_mesh(flosic_ctx *ctx, int64_t nmsh, const double *rmsh, const double *wmsh);
+
+/* NSPN, functional and precision. */
+int32_t flosic_set_options(flosic_ctx *ctx, int32_t nspn, int32_t xc, int32_t precision);
+
+/* KS orbitals of spin ispn (1 or 2): COMMON5 PSI_COEF as unravelled by UNRAVEL2 (unravel2.f:90-225);
+ * coef is nbas x nwf column-major; the first nocc columns are the occupied orbitals (occupation 1). */
+int32_t flosic_set_orbitals(flosic_ctx *ctx, int32_t ispn, int32_t nbas, int32_t nwf, int32_t nocc,
+                            const double *coef);
+
+/* FOD positions of spin ispn: module FRM BFRM(3,nfrm); nfrm must equal that spin's nocc (frmiorb.f:443). */
+int32_t flosic_set_fods(flosic_ctx *ctx, int32_t ispn, int32_t nfrm, const double *bfrm);
+
+/* FRMORB2(0,0) + FRMIORB(1,1) (frmorb2.F:140-525, frmiorb.f:336-482): Fermi orbitals at the FODs,
+ * FO overlap, Lowdin S^-1/2 and TMAT for both spins. */
+int32_t flosic_frmorb(flosic_ctx *ctx);
+
+/* Fermi-orbital data of spin ispn after flosic_frmorb: FMAT(j,i,K) nfrm x nfrm x 4, OVER nfrm x nfrm,
+ * EVAL (q ascending) nfrm, FILO (eigenvectors) nfrm x nfrm, TMAT nfrm x nfrm.  Any pointer may be NULL. */
+int32_t flosic_get_fo(flosic_ctx *ctx, int32_t ispn, double *fmat, double *over, double *eval, double *filo,
+                      double *tmat);
+
+/* The whole SIC evaluation for both spins: APOTNL_SIC over every FLO (apotnl_sic.F:31-668, with
+ * COUPOT_SIC/POISSON exact Coulomb), SICLAGM (frmorb_mpi.F:2-178), V^i_ab, and FRMORB2(-ISPN,-1) FOD
+ * forces (frmorb2.F:589-729).  zsic (may be NULL) receives E_SIC = sum of RESULTS(9,:,:) (main.F:756). */
+int32_t flosic_evaluate(flosic_ctx *ctx, double *zsic);
+
+/* APOTNL_SIC for all FLOs: RESULTS(13,nfrm) of spin ispn (apotnl_sic.F:595-623) and ZSIC. */
+int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double *zsic);
+
+/* SICLAGM: SIC(j,i) = -sum_k w_k psi~_j V_i phi_i, j over all nwf orbitals (FLOs first), i over FLOs;
+ * nwf x nfrm column-major (frmorb_mpi.F:456-517). */
+int32_t flosic_siclagm(flosic_ctx *ctx, int32_t ispn, double *sic);
+
+/* V^i_ab = sum_k w_k phi_a V_i phi_b over the FLOs of spin ispn: nfrm^3, element (a,b,i) at
+ * a + nfrm*(b + nfrm*i) (the FLO-space restriction of AHAMMAT, sicmat.F:380-411). */
+int32_t flosic_sic_vab(flosic_ctx *ctx, int32_t ispn, double *vab);
+
+/* FRMORB2(-ISPN,-1): DEBDAX(3,nfrm) = +dE_SIC/da (frmorb2.F:589-729; consumed as the gradient by
+ * FOD_OPT, electronic_geometry_lbfgs.f:143-146). */
+int32_t flosic_fod_forces(flosic_ctx *ctx, int32_t ispn, double *debdax);
+
+/* Adjoint FOD gradient dE_SIC/da(3,nfrm): dE/dTMAT = 2 SIC^T TMAT back-propagated through the Lowdin
+ * step and the Fermi-orbital construction; no counterpart entry point in the reference (it equals the
+ * FRMORB2 formula). */
+int32_t flosic_fod_gradient(flosic_ctx *ctx, int32_t ispn, double *grad);
+
+/* ZPOTnnnn analogue (apotnl_sic.F:497-522): V_i = V_xc + V_H of FLO iorb (1-based) of spin ispn on the
+ * mesh; vh (may be NULL) receives V_H alone. */
+int32_t flosic_get_potential(flosic_ctx *ctx, int32_t ispn, int32_t iorb, double *pot, double *vh);
+
+/* Primitive pair list built by flosic_set_basis (coupot_sic.F:228-310): npair rows of
+ * (atomA, lA, bareA, atomB, lB, bareB), 0-based, in canonical order.  Pass list = NULL to query npair. */
+int32_t flosic_get_pairs(flosic_ctx *ctx, int32_t *npair, int32_t *list);
+
+/* Phase timings of the last flosic_evaluate in milliseconds (HIP events on the context's stream):
+ * [0] FO+Lowdin  [1] orbitals on mesh  [2] Hermite pair coefficients  [3] Coulomb  [4] XC+energies
+ * [5] SIC matrix  [6] V^i_ab  [7] forces  [8] total.  n <= 9 values are written. */
+int32_t flosic_get_timings(flosic_ctx *ctx, int32_t n, double *ms);
+
+/* Counters of the last evaluation: [0] npair [1] Hermite rows [2] nmsh [3] nbas [4] nflo (both spins). */
+int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1137 @@
+// Host side of libflosic_hip.so: context, set-up (basis, mesh, pair list, resident basis planes), the
+// per-evaluation pipeline on one HIP stream, the FP64 Fermi-orbital / Lowdin / force algebra, and the
+// C ABI declared in include/flosic_hip.h.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/flosic_hip.h"
+#include "flosic_internal.h"
+
+using namespace flosic;
+
+namespace {
+
+constexpr int kNcart[3] = {1, 3, 6};
+constexpr int kCpow[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                              {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+constexpr int kLoff[3] = {0, 1, 4};
+constexpr double kZed = 1.0e-30;  // coupot_sic.F:61
+
+struct Status {
+    int code = 0;
+    std::string msg;
+};
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // (re)allocate to at least cnt elements; returns false on failure
+    bool ensure(size_t cnt) {
+        if (cnt <= n && p) return true;
+        release();
+        if (cnt == 0) cnt = 1;
+        if (hipMalloc(&p, cnt * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = cnt;
+        return true;
+    }
+};
+
+struct Spin {
+    int nwf = 0, nocc = 0;
+    std::vector<double> coef;  // nbas x nwf col-major
+    std::vector<double> fod;   // 3 x nocc
+    bool have_coef = false, have_fod = false;
+    // FO / Lowdin (host, FP64)
+    std::vector<double> fmat;  // FMAT(j,i,K): n x n x 4
+    std::vector<double> over, q, U, sinvh, tmat;
+    // results
+    std::vector<double> results;  // 13 x n
+    std::vector<double> sic;      // nwf x n
+    std::vector<double> vab;      // n^3
+    std::vector<double> debdax, grad;  // 3 x n
+    DevBuf<double> d_ctil;        // nbas x nwf: FLOs then virtual KS orbitals
+};
+
+}  // namespace
+
+struct flosic_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    hipEvent_t ev[10] = {};
+    Status st;
+    // options
+    int nspn = 2, xc = FLOSIC_XC_LSDA, prec = FLOSIC_PREC_FP64;
+    // basis (host)
+    int natom = 0, nbas = 0;
+    std::vector<double> xyz, alpha, con;
+    std::vector<int> bare_off, ncon, con_off;
+    std::vector<int> fstart;  // 3*natom: first function of (atom,l)
+    // contracted shells and primitives for the mesh kernels
+    DevBuf<double> d_atom_xyz, d_atom_r2cut, d_prim_alpha, d_prim_coef;
+    DevBuf<int> d_atom_cs, d_cs_l, d_cs_f0, d_cs_stride, d_cs_p0;
+    int ncs = 0;
+    // primitive shells and pairs
+    std::vector<int> ps_atom, ps_l, ps_bare;
+    std::vector<int32_t> pair_list;  // 6 per pair, canonical order
+    int npair = 0, nrows = 0;
+    int segL[6] = {0, 0, 0, 0, 0, 0};
+    DevBuf<int> d_ps_f0, d_ps_stride, d_ps_ncart, d_ps_c0, d_ps_nc, d_pair_row;
+    DevBuf<double> d_ps_coef;
+    DevBuf<PairDev> d_pairs;
+    DevBuf<double4> d_pairP;
+    DevBuf<double> d_btab;
+    // mesh
+    int64_t M = 0;
+    DevBuf<double> d_x, d_y, d_z, d_w;
+    int64_t kmax = 0;
+    double rmax = 0.0;
+    // resident basis planes (FOD independent)
+    DevBuf<double> d_planes;
+    int planes_nd = 0;
+    bool planes_ok = false;
+    // spins
+    Spin sp[2];
+    // evaluation buffers
+    int nF = 0, ldh = 0;
+    DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_vabpart, d_vab, d_lambda,
+        d_asy;
+    std::vector<double> h_energy, h_asy;
+    double zsic = 0.0;
+    bool evaluated = false, fo_done = false;
+    double timings[9] = {};
+    int64_t counts[5] = {};
+};
+
+namespace {
+
+int fail(flosic_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) {
+        c->st.code = code;
+        c->st.msg = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return fail(ctx, FLOSIC_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+#define BLASCHK(ctx, call)                                                                       \
+    do {                                                                                         \
+        rocblas_status s_ = (call);                                                              \
+        if (s_ != rocblas_status_success) return fail(ctx, FLOSIC_EDEVICE, "%s: rocblas status %d", #call, (int)s_); \
+    } while (0)
+#define ALLOC(ctx, buf, cnt)                                                                     \
+    do {                                                                                         \
+        if (!(buf).ensure(cnt)) return fail(ctx, FLOSIC_ENOMEM, "device allocation of %zu elements failed", (size_t)(cnt)); \
+    } while (0)
+
+template <typename T>
+int upload(flosic_ctx *c, DevBuf<T> &b, const std::vector<T> &h) {
+    ALLOC(c, b, h.size());
+    if (!h.empty()) HIPCHK(c, hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// host basis at a few points (FODs): values and gradients, same screening as the mesh kernel
+// ------------------------------------------------------------------------------------------------
+void host_basis_at(const flosic_ctx *c, const double *pt, double *out /* 4 x nbas */) {
+    const double geps = std::exp(-kCutExp);
+    std::fill(out, out + 4 * c->nbas, 0.0);
+    for (int a = 0; a < c->natom; ++a) {
+        const double x = pt[0] - c->xyz[3 * a], y = pt[1] - c->xyz[3 * a + 1], z = pt[2] - c->xyz[3 * a + 2];
+        const double r2 = x * x + y * y + z * z;
+        const double r[3] = {x, y, z};
+        const int nb = c->bare_off[a + 1] - c->bare_off[a];
+        for (int l = 0; l < 3; ++l) {
+            const int nc = c->ncon[3 * a + l];
+            for (int cc = 0; cc < nc; ++cc) {
+                double S0 = 0, S1 = 0;
+                for (int i = 0; i < nb; ++i) {
+                    const double al = c->alpha[c->bare_off[a] + i];
+                    const double arg = al * r2;
+                    if (!(arg < kCutExp)) continue;
+                    const double g = std::exp(-arg);
+                    const double cf = c->con[c->con_off[3 * a + l] + cc * nb + i];
+                    if (!(std::fabs(cf) * g > geps)) continue;
+                    S0 += g * cf;
+                    S1 += g * cf * al;
+                }
+                for (int m = 0; m < kNcart[l]; ++m) {
+                    const int *pw = kCpow[kLoff[l] + m];
+                    auto ip = [](double v, int p) { return p == 0 ? 1.0 : (p == 1 ? v : v * v); };
+                    const double P = ip(x, pw[0]) * ip(y, pw[1]) * ip(z, pw[2]);
+                    const int mu = c->fstart[3 * a + l] + m * nc + cc;
+                    out[mu] = P * S0;
+                    for (int d = 0; d < 3; ++d) {
+                        int q[3] = {pw[0], pw[1], pw[2]};
+                        double dP = 0.0;
+                        if (q[d] > 0) {
+                            const int k = q[d];
+                            q[d] -= 1;
+                            dP = k * ip(x, q[0]) * ip(y, q[1]) * ip(z, q[2]);
+                        }
+                        out[(1 + d) * c->nbas + mu] = dP * S0 - 2.0 * r[d] * P * S1;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// symmetric eigen-decomposition, cyclic Jacobi; q ascending; U column-major, largest |component| positive
+void jacobi(int n, const double *Ain, double *q, double *U) {
+    std::vector<double> A(Ain, Ain + (size_t)n * n), V((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double x = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+                tot += x;
+                if (i != j) off += x;
+            }
+        if (off == 0.0 || off <= 1e-30 * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int r = p + 1; r < n; ++r) {
+                const double apr = A[(size_t)p * n + r];
+                if (std::fabs(apr) < 1e-300) continue;
+                const double theta = (A[(size_t)r * n + r] - A[(size_t)p * n + p]) / (2.0 * apr);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[(size_t)k * n + p], akr = A[(size_t)k * n + r];
+                    A[(size_t)k * n + p] = cs * akp - sn * akr;
+                    A[(size_t)k * n + r] = sn * akp + cs * akr;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[(size_t)p * n + k], ark = A[(size_t)r * n + k];
+                    A[(size_t)p * n + k] = cs * apk - sn * ark;
+                    A[(size_t)r * n + k] = sn * apk + cs * ark;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[(size_t)k * n + p], vkr = V[(size_t)k * n + r];
+                    V[(size_t)k * n + p] = cs * vkp - sn * vkr;
+                    V[(size_t)k * n + r] = sn * vkp + cs * vkr;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+    for (int c = 0; c < n; ++c) {
+        q[c] = A[(size_t)idx[c] * n + idx[c]];
+        int big = 0;
+        for (int k = 0; k < n; ++k)
+            if (std::fabs(V[(size_t)k * n + idx[c]]) > std::fabs(V[(size_t)big * n + idx[c]])) big = k;
+        const double sg = V[(size_t)big * n + idx[c]] < 0 ? -1.0 : 1.0;
+        for (int k = 0; k < n; ++k) U[k + (size_t)n * c] = sg * V[(size_t)k * n + idx[c]];
+    }
+}
+
+// FRMORB2(0,0)+FRMIORB for one spin: FMAT, OVER, Lowdin, TMAT and the FLO AO coefficients
+int frmorb_spin(flosic_ctx *c, Spin &s, double *cflo /* nbas x n col-major */) {
+    const int n = s.nocc, N = c->nbas;
+    s.fmat.assign((size_t)n * n * 4, 0.0);
+    std::vector<double> chi(4 * (size_t)N);
+    for (int i = 0; i < n; ++i) {
+        host_basis_at(c, &s.fod[3 * i], chi.data());
+        std::vector<double> psi(4 * (size_t)n, 0.0);
+        for (int K = 0; K < 4; ++K)
+            for (int j = 0; j < n; ++j) {
+                const double *cj = &s.coef[(size_t)j * N];
+                double acc = 0.0;
+                for (int mu = 0; mu < N; ++mu) acc += chi[(size_t)K * N + mu] * cj[mu];
+                psi[(size_t)K * n + j] = acc;
+            }
+        double rhi[4] = {0, 0, 0, 0};
+        for (int j = 0; j < n; ++j)
+            for (int K = 0; K < 4; ++K) rhi[K] += psi[j] * psi[(size_t)K * n + j];
+        const double srho = std::sqrt(rhi[0]);
+        for (int j = 0; j < n; ++j) s.fmat[j + (size_t)n * i] = psi[j] / srho;
+        for (int K = 1; K < 4; ++K)
+            for (int j = 0; j < n; ++j)
+                s.fmat[j + (size_t)n * (i + (size_t)n * K)] =
+                    psi[(size_t)K * n + j] / srho - s.fmat[j + (size_t)n * i] * rhi[K] / rhi[0];
+    }
+    s.over.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += s.fmat[k + (size_t)n * i] * s.fmat[k + (size_t)n * j];
+            s.over[j + (size_t)n * i] = acc;
+        }
+    s.q.assign(n, 0.0);
+    s.U.assign((size_t)n * n, 0.0);
+    jacobi(n, s.over.data(), s.q.data(), s.U.data());
+    for (int i = 0; i < n; ++i)
+        if (s.q[i] < 1.0e-8)
+            return fail(c, FLOSIC_ELOWDIN, "Lowdin eigenvalue %d = %.3e < 1e-8 (coincident FODs?)", i + 1, s.q[i]);
+    s.sinvh.assign((size_t)n * n, 0.0);
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += s.U[a + (size_t)n * k] * s.U[b + (size_t)n * k] / std::sqrt(s.q[k]);
+            s.sinvh[a + (size_t)n * b] = acc;
+        }
+    s.tmat.assign((size_t)n * n, 0.0);  // TMAT(i,j) = sum_k S^-1/2(i,k) T(k,j), T(k,j) = FMAT(j,k,1)
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += s.sinvh[i + (size_t)n * k] * s.fmat[j + (size_t)n * k];
+            s.tmat[i + (size_t)n * j] = acc;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int mu = 0; mu < N; ++mu) {
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += s.tmat[i + (size_t)n * j] * s.coef[(size_t)j * N + mu];
+            cflo[(size_t)i * N + mu] = acc;
+        }
+    return 0;
+}
+
+// FRMORB2(-ISPN,-1) force formula (frmorb2.F:612-717) from FMAT, U, q and SIC(1:n,1:n)
+void force_formula(const Spin &s, double *deb) {
+    const int n = s.nocc, ld = s.nwf;
+    auto F = [&](int j, int i, int K) { return s.fmat[j + (size_t)n * (i + (size_t)n * K)]; };
+    auto U = [&](int k, int a) { return s.U[k + (size_t)n * a]; };
+    auto L = [&](int k, int l) { return s.sic[k + (size_t)ld * l]; };
+    std::vector<double> H((size_t)n * n), O((size_t)n * n), sc(n);
+    std::fill(deb, deb + 3 * n, 0.0);
+    // H(b,a) = sum_kl (U_ka U_lb - U_la U_kb) SIC(k,l) / sqrt(q_a q_b): antisymmetrised SIC in the eigenbasis
+    std::vector<double> LU((size_t)n * n), ULU((size_t)n * n);
+    for (int k = 0; k < n; ++k)
+        for (int b = 0; b < n; ++b) {
+            double acc = 0.0;
+            for (int l = 0; l < n; ++l) acc += L(k, l) * U(l, b);
+            LU[k + (size_t)n * b] = acc;
+        }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += U(k, a) * LU[k + (size_t)n * b];
+            ULU[a + (size_t)n * b] = acc;  // (U^T SIC U)_ab
+        }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b)
+            H[b + (size_t)n * a] = (ULU[a + (size_t)n * b] - ULU[b + (size_t)n * a]) / (std::sqrt(s.q[b]) * std::sqrt(s.q[a]));
+    // D_x(f,m) = <F_f | dF_m/da_mx>
+    std::vector<double> D(3 * (size_t)n * n);
+    for (int x = 0; x < 3; ++x)
+        for (int f = 0; f < n; ++f)
+            for (int m = 0; m < n; ++m) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += F(k, f, 0) * F(k, m, x + 1);
+                D[(size_t)x * n * n + f + (size_t)n * m] = acc;
+            }
+    // first term: sum_a U_ma sum_f (sum_b H(b,a) U_fb) D(f,m)
+    for (int a = 0; a < n; ++a) {
+        for (int f = 0; f < n; ++f) {
+            double acc = 0.0;
+            for (int b = 0; b < n; ++b) acc += H[b + (size_t)n * a] * U(f, b);
+            sc[f] = acc;
+        }
+        for (int x = 0; x < 3; ++x)
+            for (int m = 0; m < n; ++m) {
+                double acc = 0.0;
+                for (int f = 0; f < n; ++f) acc += sc[f] * D[(size_t)x * n * n + f + (size_t)n * m];
+                deb[x + 3 * m] -= U(m, a) * acc;
+            }
+    }
+    // second term with H(b,a) (sqrt q_b - sqrt q_a)/(sqrt q_b + sqrt q_a) and the doubled overlap derivative
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            const double qa = std::sqrt(s.q[a]), qb = std::sqrt(s.q[b]);
+            H[b + (size_t)n * a] = H[b + (size_t)n * a] * (qb - qa) / (qb + qa);
+        }
+    for (int x = 0; x < 3; ++x)
+        for (int m = 0; m < n; ++m) {
+            double acc = 0.0;
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b) {
+                    const double h = H[b + (size_t)n * a];
+                    if (h == 0.0) continue;
+                    double t = 0.0;
+                    for (int f = 0; f < n; ++f)
+                        t += (U(f, b) * U(m, a) + U(m, b) * U(f, a)) * 2.0 * D[(size_t)x * n * n + f + (size_t)n * m];
+                    acc += -0.5 * t * h;
+                }
+            deb[x + 3 * m] -= 0.5 * acc;
+        }
+}
+
+// adjoint: Gbar = 2 SIC^T TMAT (= dE/dTMAT), back-propagated through TMAT = S^-1/2 T and S = T T^T
+void force_adjoint(const Spin &s, double *grad) {
+    const int n = s.nocc, ld = s.nwf;
+    const size_t nn = (size_t)n * n;
+    auto F = [&](int j, int i, int K) { return s.fmat[j + (size_t)n * (i + (size_t)n * K)]; };
+    std::vector<double> G(nn), B(nn), Bt(nn), W(nn), tmp(nn), Tb(nn);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += s.sic[k + (size_t)ld * i] * s.tmat[k + (size_t)n * j];
+            G[i + (size_t)n * j] = 2.0 * acc;
+        }
+    for (int i = 0; i < n; ++i)  // B = G T^T
+        for (int k = 0; k < n; ++k) {
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += G[i + (size_t)n * j] * F(j, k, 0);
+            B[i + (size_t)n * k] = acc;
+        }
+    for (int a = 0; a < n; ++a)
+        for (int k = 0; k < n; ++k) {
+            double acc = 0.0;
+            for (int i = 0; i < n; ++i) acc += s.U[i + (size_t)n * a] * B[i + (size_t)n * k];
+            tmp[a + (size_t)n * k] = acc;
+        }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += tmp[a + (size_t)n * k] * s.U[k + (size_t)n * b];
+            const double dq = s.q[a] - s.q[b];
+            const double f = std::fabs(dq) > 1e-10 * (s.q[a] + s.q[b])
+                                 ? (1.0 / std::sqrt(s.q[a]) - 1.0 / std::sqrt(s.q[b])) / dq
+                                 : -0.5 / (s.q[a] * std::sqrt(s.q[a]));
+            Bt[a + (size_t)n * b] = acc * f;  // Daleckii-Krein divided difference of q^-1/2
+        }
+    for (int i = 0; i < n; ++i)
+        for (int b = 0; b < n; ++b) {
+            double acc = 0.0;
+            for (int a = 0; a < n; ++a) acc += s.U[i + (size_t)n * a] * Bt[a + (size_t)n * b];
+            tmp[i + (size_t)n * b] = acc;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            double acc = 0.0;
+            for (int b = 0; b < n; ++b) acc += tmp[i + (size_t)n * b] * s.U[k + (size_t)n * b];
+            W[i + (size_t)n * k] = acc;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double a1 = 0.0, a2 = 0.0;
+            for (int k = 0; k < n; ++k) {
+                a1 += (W[i + (size_t)n * k] + W[k + (size_t)n * i]) * F(j, k, 0);
+                a2 += s.sinvh[i + (size_t)n * k] * G[k + (size_t)n * j];
+            }
+            Tb[i + (size_t)n * j] = a1 + a2;
+        }
+    for (int m = 0; m < n; ++m)
+        for (int x = 0; x < 3; ++x) {
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += Tb[m + (size_t)n * j] * F(j, m, x + 1);
+            grad[x + 3 * m] = acc;
+        }
+}
+
+// ------------------------------------------------------------------------------------------------
+// set-up helpers
+// ------------------------------------------------------------------------------------------------
+int build_basis_device(flosic_ctx *c) {
+    std::vector<double> axyz = c->xyz, r2cut(c->natom, 0.0), palpha, pcoef;
+    std::vector<int> atom_cs(c->natom + 1, 0), cs_l, cs_f0, cs_stride, cs_p0(1, 0);
+    for (int a = 0; a < c->natom; ++a) {
+        const int nb = c->bare_off[a + 1] - c->bare_off[a];
+        for (int l = 0; l < 3; ++l) {
+            const int nc = c->ncon[3 * a + l];
+            for (int cc = 0; cc < nc; ++cc) {
+                for (int i = 0; i < nb; ++i) {
+                    const double cf = c->con[c->con_off[3 * a + l] + cc * nb + i];
+                    if (cf == 0.0) continue;
+                    const double al = c->alpha[c->bare_off[a] + i];
+                    palpha.push_back(al);
+                    pcoef.push_back(cf);
+                    r2cut[a] = std::max(r2cut[a], kCutExp / al);
+                }
+                cs_l.push_back(l);
+                cs_f0.push_back(c->fstart[3 * a + l] + cc);
+                cs_stride.push_back(nc);
+                cs_p0.push_back((int)palpha.size());
+            }
+        }
+        atom_cs[a + 1] = (int)cs_l.size();
+    }
+    c->ncs = (int)cs_l.size();
+    int r = 0;
+    if ((r = upload(c, c->d_atom_xyz, axyz)) || (r = upload(c, c->d_atom_r2cut, r2cut)) ||
+        (r = upload(c, c->d_prim_alpha, palpha)) || (r = upload(c, c->d_prim_coef, pcoef)) ||
+        (r = upload(c, c->d_atom_cs, atom_cs)) || (r = upload(c, c->d_cs_l, cs_l)) ||
+        (r = upload(c, c->d_cs_f0, cs_f0)) || (r = upload(c, c->d_cs_stride, cs_stride)) ||
+        (r = upload(c, c->d_cs_p0, cs_p0)))
+        return r;
+    return 0;
+}
+
+int build_pairs(flosic_ctx *c) {
+    // primitive shells (atom, l, bare) carrying a nonzero contraction coefficient (ZED test, coupot_sic.F:384-397)
+    c->ps_atom.clear();
+    c->ps_l.clear();
+    c->ps_bare.clear();
+    std::vector<int> ps_f0, ps_stride, ps_ncart, ps_c0, ps_nc;
+    std::vector<double> ps_coef;
+    for (int a = 0; a < c->natom; ++a) {
+        const int nb = c->bare_off[a + 1] - c->bare_off[a];
+        for (int l = 0; l < 3; ++l) {
+            const int nc = c->ncon[3 * a + l];
+            for (int i = 0; i < nb; ++i) {
+                bool used = false;
+                for (int cc = 0; cc < nc; ++cc)
+                    if (std::fabs(c->con[c->con_off[3 * a + l] + cc * nb + i]) > kZed) used = true;
+                if (!used) continue;
+                c->ps_atom.push_back(a);
+                c->ps_l.push_back(l);
+                c->ps_bare.push_back(i);
+                ps_f0.push_back(c->fstart[3 * a + l]);
+                ps_stride.push_back(nc);
+                ps_ncart.push_back(kNcart[l]);
+                ps_c0.push_back((int)ps_coef.size());
+                ps_nc.push_back(nc);
+                for (int cc = 0; cc < nc; ++cc) ps_coef.push_back(c->con[c->con_off[3 * a + l] + cc * nb + i]);
+            }
+        }
+    }
+    const int nps = (int)c->ps_atom.size();
+    // pairs ps1 <= ps2, screened by a*b/(a+b)*|A-B|^2 > CUTEXP (coupot_sic.F:293-310)
+    std::vector<PairDev> pairs;
+    c->pair_list.clear();
+    for (int i = 0; i < nps; ++i)
+        for (int j = i; j < nps; ++j) {
+            const int A = c->ps_atom[i], B = c->ps_atom[j];
+            const double ai = c->alpha[c->bare_off[A] + c->ps_bare[i]], aj = c->alpha[c->bare_off[B] + c->ps_bare[j]];
+            const double dx = c->xyz[3 * A] - c->xyz[3 * B], dy = c->xyz[3 * A + 1] - c->xyz[3 * B + 1],
+                         dz = c->xyz[3 * A + 2] - c->xyz[3 * B + 2];
+            const double rarg = dx * dx + dy * dy + dz * dz;
+            const double rcalp = 1.0 / (ai + aj);
+            if (ai * aj * rcalp * rarg > kCutExp) continue;
+            PairDev p{};
+            for (int d = 0; d < 3; ++d) {
+                p.A[d] = c->xyz[3 * A + d];
+                p.B[d] = c->xyz[3 * B + d];
+            }
+            p.a = ai;
+            p.b = aj;
+            p.psA = i;
+            p.psB = j;
+            p.lA = c->ps_l[i];
+            p.lB = c->ps_l[j];
+            p.sym = (i == j) ? 1.0 : 2.0;
+            pairs.push_back(p);
+            const int32_t row[6] = {A, c->ps_l[i], c->ps_bare[i], B, c->ps_l[j], c->ps_bare[j]};
+            c->pair_list.insert(c->pair_list.end(), row, row + 6);
+        }
+    // order by L for the Coulomb kernel (stable: canonical order inside each L)
+    std::stable_sort(pairs.begin(), pairs.end(),
+                     [](const PairDev &x, const PairDev &y) { return x.lA + x.lB < y.lA + y.lB; });
+    c->npair = (int)pairs.size();
+    std::vector<double4> pP(c->npair);
+    std::vector<int> prow(c->npair);
+    int row = 0;
+    for (int L = 0; L <= 5; ++L) c->segL[L] = 0;
+    for (int k = 0; k < c->npair; ++k) {
+        PairDev &p = pairs[k];
+        const int L = p.lA + p.lB;
+        p.row = row;
+        prow[k] = row;
+        row += (L + 1) * (L + 2) * (L + 3) / 6;
+        const double pe = p.a + p.b;
+        pP[k] = make_double4((p.a * p.A[0] + p.b * p.B[0]) / pe, (p.a * p.A[1] + p.b * p.B[1]) / pe,
+                             (p.a * p.A[2] + p.b * p.B[2]) / pe, pe);
+    }
+    c->nrows = row;
+    // segment offsets: segL[L] = first pair with lA+lB >= L
+    for (int L = 0; L <= 5; ++L) {
+        int k = 0;
+        while (k < c->npair && pairs[k].lA + pairs[k].lB < L) ++k;
+        c->segL[L] = k;
+    }
+    int r = 0;
+    if ((r = upload(c, c->d_pairs, pairs)) || (r = upload(c, c->d_pairP, pP)) || (r = upload(c, c->d_pair_row, prow)) ||
+        (r = upload(c, c->d_ps_f0, ps_f0)) || (r = upload(c, c->d_ps_stride, ps_stride)) ||
+        (r = upload(c, c->d_ps_ncart, ps_ncart)) || (r = upload(c, c->d_ps_c0, ps_c0)) ||
+        (r = upload(c, c->d_ps_nc, ps_nc)) || (r = upload(c, c->d_ps_coef, ps_coef)))
+        return r;
+    return 0;
+}
+
+BasisDev basis_view(const flosic_ctx *c) {
+    BasisDev B;
+    B.natom = c->natom;
+    B.nbas = c->nbas;
+    B.ncs = c->ncs;
+    B.atom_xyz = c->d_atom_xyz.p;
+    B.atom_r2cut = c->d_atom_r2cut.p;
+    B.atom_cs = c->d_atom_cs.p;
+    B.cs_l = c->d_cs_l.p;
+    B.cs_f0 = c->d_cs_f0.p;
+    B.cs_stride = c->d_cs_stride.p;
+    B.cs_p0 = c->d_cs_p0.p;
+    B.prim_alpha = c->d_prim_alpha.p;
+    B.prim_coef = c->d_prim_coef.p;
+    return B;
+}
+
+// FOD-independent basis planes on the mesh, kept resident
+int ensure_planes(flosic_ctx *c) {
+    const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
+    if (c->planes_ok && c->planes_nd == nd) return 0;
+    const size_t cnt = (size_t)nd * c->nbas * (size_t)c->M;
+    ALLOC(c, c->d_planes, cnt);
+    HIPCHK(c, hipMemsetAsync(c->d_planes.p, 0, cnt * sizeof(double), c->stream));
+    launch_basis_planes<double>(basis_view(c), c->d_x.p, c->d_y.p, c->d_z.p, c->M, nd, c->d_planes.p, c->M, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->planes_nd = nd;
+    c->planes_ok = true;
+    return 0;
+}
+
+int check_ready(flosic_ctx *c) {
+    if (c->nbas <= 0) return fail(c, FLOSIC_ESTATE, "basis not set");
+    if (c->M <= 0) return fail(c, FLOSIC_ESTATE, "mesh not set");
+    for (int s = 0; s < 2; ++s) {
+        if (!c->sp[s].have_coef) return fail(c, FLOSIC_ESTATE, "orbitals of spin %d not set", s + 1);
+        if (!c->sp[s].have_fod) return fail(c, FLOSIC_ESTATE, "FODs of spin %d not set", s + 1);
+        if ((int)c->sp[s].fod.size() != 3 * c->sp[s].nocc)
+            return fail(c, FLOSIC_EINVAL, "spin %d: number of FODs differs from occupied orbitals", s + 1);
+    }
+    if (c->sp[0].nocc + c->sp[1].nocc <= 0) return fail(c, FLOSIC_EINVAL, "no occupied orbitals");
+    if (c->sp[0].nocc > 128 || c->sp[1].nocc > 128) return fail(c, FLOSIC_EINVAL, "at most 128 FODs per spin");
+    return 0;
+}
+
+int run_frmorb(flosic_ctx *c, std::vector<double> &cflo) {
+    const int N = c->nbas;
+    cflo.assign((size_t)N * (c->sp[0].nocc + c->sp[1].nocc), 0.0);
+    int off = 0;
+    for (int s = 0; s < 2; ++s) {
+        if (c->sp[s].nocc > 0) {
+            const int r = frmorb_spin(c, c->sp[s], &cflo[(size_t)off * N]);
+            if (r) return r;
+        }
+        off += c->sp[s].nocc;
+    }
+    c->fo_done = true;
+    return 0;
+}
+
+int evaluate(flosic_ctx *c) {
+    int r = check_ready(c);
+    if (r) return r;
+    c->evaluated = false;
+    const int N = c->nbas;
+    const int n0 = c->sp[0].nocc, n1 = c->sp[1].nocc;
+    const int nF = n0 + n1;
+    const int64_t M = c->M;
+    const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
+    c->nF = nF;
+    c->ldh = (nF + 15) / 16 * 16;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    // 1. Fermi orbitals, Lowdin, FLO coefficients (host FP64)
+    std::vector<double> cflo;
+    if ((r = run_frmorb(c, cflo))) return r;
+    if ((r = upload(c, c->d_cflo, cflo))) return r;
+    for (int s = 0; s < 2; ++s) {
+        Spin &sp = c->sp[s];
+        ALLOC(c, sp.d_ctil, (size_t)N * sp.nwf);
+        const int off = s == 0 ? 0 : n0;
+        if (sp.nocc > 0)
+            HIPCHK(c, hipMemcpyAsync(sp.d_ctil.p, c->d_cflo.p + (size_t)off * N, sizeof(double) * N * sp.nocc,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        if (sp.nwf > sp.nocc)
+            HIPCHK(c, hipMemcpyAsync(sp.d_ctil.p + (size_t)sp.nocc * N, sp.coef.data() + (size_t)sp.nocc * N,
+                                     sizeof(double) * N * (sp.nwf - sp.nocc), hipMemcpyHostToDevice, c->stream));
+    }
+    if ((r = ensure_planes(c))) return r;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    // 2. FLOs (and derivatives) on the mesh: phi_d = Phi_d . cflo
+    ALLOC(c, c->d_phi, (size_t)nd * nF * M);
+    {
+        const double one = 1.0, zero = 0.0;
+        for (int d = 0; d < nd; ++d)
+            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, nF, N,
+                                     &one, c->d_planes.p + (size_t)d * N * M, (rocblas_int)M, c->d_cflo.p, N, &zero,
+                                     c->d_phi.p + (size_t)d * nF * M, (rocblas_int)M));
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    // 3. Hermite pair coefficients
+    const int nps = (int)c->ps_atom.size();
+    ALLOC(c, c->d_dps, (size_t)nps * 6 * nF);
+    launch_prim_coef(nps, c->d_ps_f0.p, c->d_ps_stride.p, c->d_ps_ncart.p, c->d_ps_c0.p, c->d_ps_nc.p, c->d_ps_coef.p,
+                     c->d_cflo.p, nF, N, c->d_dps.p, c->stream);
+    {
+        const size_t hcnt = (size_t)c->nrows * c->ldh;
+        const bool fresh = c->d_H.n < hcnt || !c->d_H.p;
+        ALLOC(c, c->d_H, hcnt);
+        if (fresh) HIPCHK(c, hipMemsetAsync(c->d_H.p, 0, c->d_H.n * sizeof(double), c->stream));
+    }
+    launch_hermite(c->npair, c->d_pairs.p, c->d_dps.p, nF, c->d_H.p, c->ldh, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    // 4. Coulomb potentials of all FLOs
+    ALLOC(c, c->d_VH, (size_t)nF * M);
+    launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
+                   c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    // 5. XC, total potential, energies
+    const int nblk = (int)((M + 255) / 256);
+    ALLOC(c, c->d_Vtot, (size_t)nF * M);
+    ALLOC(c, c->d_U, (size_t)nF * M);
+    ALLOC(c, c->d_part, (size_t)nF * nblk * 9);
+    ALLOC(c, c->d_energy, (size_t)nF * 9);
+    ALLOC(c, c->d_asy, (size_t)nF);
+    launch_xc(c->xc, nd, c->d_phi.p, M, nF, c->d_VH.p, c->d_w.p, c->d_Vtot.p, c->d_U.p, c->d_part.p, nblk, c->stream);
+    HIPCHK(c, hipGetLastError());
+    launch_reduce_partials(c->d_part.p, nF, nblk, 9, c->d_energy.p, c->stream);
+    HIPCHK(c, hipMemcpy2DAsync(c->d_asy.p, sizeof(double), c->d_VH.p + c->kmax, (size_t)M * sizeof(double),
+                               sizeof(double), nF, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    // 6. SIC matrix: G = Phi_0^T U, SIC_s = -Ctil_s^T G_s
+    ALLOC(c, c->d_G, (size_t)N * nF);
+    size_t lam_cnt = 0;
+    for (int s = 0; s < 2; ++s) lam_cnt += (size_t)c->sp[s].nwf * c->sp[s].nocc;
+    ALLOC(c, c->d_lambda, lam_cnt);
+    {
+        const double one = 1.0, zero = 0.0, mone = -1.0;
+        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF, (rocblas_int)M,
+                                 &one, c->d_planes.p, (rocblas_int)M, c->d_U.p, (rocblas_int)M, &zero, c->d_G.p, N));
+        size_t lo = 0;
+        for (int s = 0; s < 2; ++s) {
+            Spin &sp = c->sp[s];
+            const int off = s == 0 ? 0 : n0;
+            if (sp.nocc > 0)
+                BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, sp.nwf, sp.nocc,
+                                         N, &mone, sp.d_ctil.p, N, c->d_G.p + (size_t)off * N, N, &zero,
+                                         c->d_lambda.p + lo, sp.nwf));
+            lo += (size_t)sp.nwf * sp.nocc;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+    // 7. V^i_ab per spin
+    size_t vab_cnt = 0, vpart_cnt = 0;
+    int nchunk[2] = {1, 1};
+    for (int s = 0; s < 2; ++s) {
+        const int n = c->sp[s].nocc;
+        if (n == 0) continue;
+        nchunk[s] = std::max(1, std::min(256, (2048 + n - 1) / n));
+        while (nchunk[s] > 1 && M / nchunk[s] < 256) --nchunk[s];
+        vab_cnt += (size_t)n * n * n;
+        vpart_cnt = std::max(vpart_cnt, (size_t)nchunk[s] * n * n * n);
+    }
+    ALLOC(c, c->d_vab, vab_cnt);
+    ALLOC(c, c->d_vabpart, vpart_cnt);
+    {
+        size_t vo = 0;
+        for (int s = 0; s < 2; ++s) {
+            const int n = c->sp[s].nocc;
+            if (n == 0) continue;
+            const int off = s == 0 ? 0 : n0;
+            launch_vab(c->d_phi.p + (size_t)off * M, c->d_Vtot.p + (size_t)off * M, c->d_w.p, M, n, nchunk[s],
+                       c->d_vabpart.p, c->stream);
+            HIPCHK(c, hipGetLastError());
+            launch_reduce_partials(c->d_vabpart.p, 1, nchunk[s], n * n * n, c->d_vab.p + vo, c->stream);
+            vo += (size_t)n * n * n;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
+    // 8. results to host, forces
+    c->h_energy.resize((size_t)nF * 9);
+    c->h_asy.resize(nF);
+    HIPCHK(c, hipMemcpyAsync(c->h_energy.data(), c->d_energy.p, sizeof(double) * nF * 9, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_asy.data(), c->d_asy.p, sizeof(double) * nF, hipMemcpyDeviceToHost, c->stream));
+    {
+        size_t lo = 0, vo = 0;
+        for (int s = 0; s < 2; ++s) {
+            Spin &sp = c->sp[s];
+            const int n = sp.nocc;
+            sp.sic.resize((size_t)sp.nwf * n);
+            sp.vab.resize((size_t)n * n * n);
+            if (n > 0) {
+                HIPCHK(c, hipMemcpyAsync(sp.sic.data(), c->d_lambda.p + lo, sizeof(double) * sp.nwf * n,
+                                         hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipMemcpyAsync(sp.vab.data(), c->d_vab.p + vo, sizeof(double) * n * n * n, hipMemcpyDeviceToHost,
+                                         c->stream));
+            }
+            lo += (size_t)sp.nwf * n;
+            vo += (size_t)n * n * n;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // RESULTS(1:13) per FLO (apotnl_sic.F:564-623)
+    c->zsic = 0.0;
+    const double e13 = (double)(1.0f / 3.0f);
+    for (int s = 0, f = 0; s < 2; ++s) {
+        Spin &sp = c->sp[s];
+        sp.results.assign((size_t)13 * sp.nocc, 0.0);
+        for (int i = 0; i < sp.nocc; ++i, ++f) {
+            const double *q = &c->h_energy[(size_t)f * 9];
+            if (q[8] < 0.001) return fail(c, FLOSIC_ECOUPOT, "COUPOT IS ZERO for spin %d orbital %d", s + 1, i + 1);
+            const double XCOU = q[1];
+            const double EXC = q[4] + q[5] + q[6] + q[7];
+            const double XENG = -EXC - XCOU / 2.0;
+            double *R = &sp.results[(size_t)13 * i];
+            R[0] = i + 1;
+            R[1] = q[0];
+            R[2] = 0.0;
+            R[3] = c->h_asy[f] * c->rmax;
+            R[4] = q[2] * 0.75 * std::pow(6.0 / M_PI, e13);
+            R[5] = -XCOU / 2.0;
+            R[6] = q[3] - XCOU;
+            R[7] = XENG;
+            R[8] = XENG * (2.0 / c->nspn);
+            R[9] = q[4];
+            R[10] = q[6];
+            R[11] = q[5];
+            R[12] = q[7];
+            c->zsic += R[8];
+        }
+    }
+    for (int s = 0; s < 2; ++s) {
+        Spin &sp = c->sp[s];
+        sp.debdax.assign(3 * (size_t)sp.nocc, 0.0);
+        sp.grad.assign(3 * (size_t)sp.nocc, 0.0);
+        if (sp.nocc > 0) {
+            force_formula(sp, sp.debdax.data());
+            force_adjoint(sp, sp.grad.data());
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[8]));
+    float ms = 0.f;
+    for (int i = 0; i < 8; ++i) {
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        c->timings[i] = ms;
+    }
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[8]));
+    c->timings[8] = ms;
+    c->counts[0] = c->npair;
+    c->counts[1] = c->nrows;
+    c->counts[2] = c->M;
+    c->counts[3] = c->nbas;
+    c->counts[4] = nF;
+    c->evaluated = true;
+    return 0;
+}
+
+int need_eval(flosic_ctx *c) {
+    if (c->evaluated) return 0;
+    return evaluate(c);
+}
+
+int spin_index(flosic_ctx *c, int ispn) {
+    if (ispn != 1 && ispn != 2) return fail(c, FLOSIC_EINVAL, "ispn must be 1 or 2 (got %d)", ispn), -1;
+    return ispn - 1;
+}
+
+}  // namespace
+
+// ====================================================================================================
+// C ABI
+// ====================================================================================================
+extern "C" {
+
+int32_t flosic_create(int32_t device, flosic_ctx **out) {
+    if (!out) return FLOSIC_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FLOSIC_EDEVICE;
+    if (device < 0 || device >= ndev) return FLOSIC_EINVAL;
+    flosic_ctx *c = new (std::nothrow) flosic_ctx();
+    if (!c) return FLOSIC_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FLOSIC_EDEVICE;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return FLOSIC_EDEVICE;
+        }
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
+        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
+        delete c;
+        return FLOSIC_EDEVICE;
+    }
+    std::vector<double> tab((size_t)kBoysRows * kBoysCols);
+    boys_table(tab.data());
+    if (upload(c, c->d_btab, tab) || hipStreamSynchronize(c->stream) != hipSuccess) {
+        delete c;
+        return FLOSIC_EDEVICE;
+    }
+    *out = c;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_destroy(flosic_ctx *c) {
+    if (!c) return FLOSIC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FLOSIC_OK;
+}
+
+const char *flosic_last_error(const flosic_ctx *c) { return c ? c->st.msg.c_str() : "null context"; }
+
+int32_t flosic_set_basis(flosic_ctx *c, int32_t natom, const double *xyz, const int32_t *bare_off, const double *alpha,
+                         const int32_t *ncon, const int32_t *con_off, const double *con) {
+    if (!c) return FLOSIC_EINVAL;
+    if (natom <= 0 || !xyz || !bare_off || !alpha || !ncon || !con_off || !con)
+        return fail(c, FLOSIC_EINVAL, "flosic_set_basis: bad arguments");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
+    c->natom = natom;
+    c->xyz.assign(xyz, xyz + 3 * natom);
+    c->bare_off.assign(bare_off, bare_off + natom + 1);
+    if (c->bare_off[0] != 0) return fail(c, FLOSIC_EINVAL, "bare_off[0] must be 0");
+    c->alpha.assign(alpha, alpha + c->bare_off[natom]);
+    for (double a : c->alpha)
+        if (!(a > 0.0)) return fail(c, FLOSIC_EINVAL, "exponents must be positive");
+    c->ncon.assign(ncon, ncon + 3 * natom);
+    c->con_off.assign(con_off, con_off + 3 * natom);
+    size_t ncoef = 0;
+    c->fstart.assign(3 * natom, 0);
+    int nbas = 0;
+    for (int a = 0; a < natom; ++a)
+        for (int l = 0; l < 3; ++l) {
+            if (c->ncon[3 * a + l] < 0) return fail(c, FLOSIC_EINVAL, "negative ncon");
+            const size_t end = (size_t)c->con_off[3 * a + l] +
+                               (size_t)c->ncon[3 * a + l] * (c->bare_off[a + 1] - c->bare_off[a]);
+            ncoef = std::max(ncoef, end);
+            c->fstart[3 * a + l] = nbas;
+            nbas += kNcart[l] * c->ncon[3 * a + l];
+        }
+    c->con.assign(con, con + ncoef);
+    c->nbas = nbas;
+    int r = 0;
+    if ((r = build_basis_device(c)) || (r = build_pairs(c))) return r;
+    c->planes_ok = false;
+    c->evaluated = false;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_basis sync");
+    return FLOSIC_OK;
+}
+
+int32_t flosic_set_mesh(flosic_ctx *c, int64_t nmsh, const double *rmsh, const double *wmsh) {
+    if (!c) return FLOSIC_EINVAL;
+    if (nmsh <= 0 || !rmsh || !wmsh) return fail(c, FLOSIC_EINVAL, "flosic_set_mesh: bad arguments");
+    if (nmsh > (int64_t)INT32_MAX / 2) return fail(c, FLOSIC_EINVAL, "mesh too large");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
+    std::vector<double> x(nmsh), y(nmsh), z(nmsh), w(wmsh, wmsh + nmsh);
+    c->rmax = 0.0;
+    c->kmax = 0;
+    for (int64_t k = 0; k < nmsh; ++k) {
+        x[k] = rmsh[3 * k];
+        y[k] = rmsh[3 * k + 1];
+        z[k] = rmsh[3 * k + 2];
+        const double r = std::sqrt(x[k] * x[k] + y[k] * y[k] + z[k] * z[k]);
+        if (r > c->rmax) {  // first point of largest radius (apotnl_sic.F:409-418)
+            c->rmax = r;
+            c->kmax = k;
+        }
+    }
+    c->M = nmsh;
+    int r = 0;
+    if ((r = upload(c, c->d_x, x)) || (r = upload(c, c->d_y, y)) || (r = upload(c, c->d_z, z)) ||
+        (r = upload(c, c->d_w, w)))
+        return r;
+    c->planes_ok = false;
+    c->evaluated = false;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_mesh sync");
+    return FLOSIC_OK;
+}
+
+int32_t flosic_set_options(flosic_ctx *c, int32_t nspn, int32_t xc, int32_t precision) {
+    if (!c) return FLOSIC_EINVAL;
+    if (nspn != 1 && nspn != 2) return fail(c, FLOSIC_EINVAL, "nspn must be 1 or 2");
+    if (xc != FLOSIC_XC_LSDA && xc != FLOSIC_XC_PBE) return fail(c, FLOSIC_EINVAL, "unknown functional %d", xc);
+    if (precision != FLOSIC_PREC_FP64 && precision != FLOSIC_PREC_MIXED)
+        return fail(c, FLOSIC_EINVAL, "unknown precision %d", precision);
+    if (xc != c->xc) c->planes_ok = false;
+    c->nspn = nspn;
+    c->xc = xc;
+    c->prec = precision;
+    c->evaluated = false;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_set_orbitals(flosic_ctx *c, int32_t ispn, int32_t nbas, int32_t nwf, int32_t nocc, const double *coef) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    if (nbas != c->nbas) return fail(c, FLOSIC_EINVAL, "nbas %d differs from the basis (%d)", nbas, c->nbas);
+    if (nocc < 0 || nwf < nocc || (nwf > 0 && !coef)) return fail(c, FLOSIC_EINVAL, "bad orbital sizes");
+    Spin &sp = c->sp[s];
+    sp.nwf = nwf;
+    sp.nocc = nocc;
+    sp.coef.assign(coef, coef + (size_t)nbas * nwf);
+    sp.have_coef = true;
+    c->evaluated = false;
+    c->fo_done = false;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_set_fods(flosic_ctx *c, int32_t ispn, int32_t nfrm, const double *bfrm) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    if (nfrm < 0 || (nfrm > 0 && !bfrm)) return fail(c, FLOSIC_EINVAL, "bad FOD arguments");
+    for (int i = 0; i < 3 * nfrm; ++i)
+        if (!std::isfinite(bfrm[i])) return fail(c, FLOSIC_EINVAL, "non-finite FOD coordinate");
+    Spin &sp = c->sp[s];
+    if (sp.have_coef && nfrm != sp.nocc)
+        return fail(c, FLOSIC_EINVAL, "MORB .NE. NFRM: %d FODs for %d occupied orbitals (frmiorb.f:443)", nfrm, sp.nocc);
+    sp.fod.assign(bfrm, bfrm + 3 * nfrm);
+    sp.have_fod = true;
+    c->evaluated = false;
+    c->fo_done = false;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_frmorb(flosic_ctx *c) {
+    if (!c) return FLOSIC_EINVAL;
+    int r = check_ready(c);
+    if (r) return r;
+    std::vector<double> cflo;
+    return run_frmorb(c, cflo);
+}
+
+int32_t flosic_get_fo(flosic_ctx *c, int32_t ispn, double *fmat, double *over, double *eval, double *filo, double *tmat) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    if (!c->fo_done) {
+        int r = flosic_frmorb(c);
+        if (r) return r;
+    }
+    const Spin &sp = c->sp[s];
+    if (fmat) std::copy(sp.fmat.begin(), sp.fmat.end(), fmat);
+    if (over) std::copy(sp.over.begin(), sp.over.end(), over);
+    if (eval) std::copy(sp.q.begin(), sp.q.end(), eval);
+    if (filo) std::copy(sp.U.begin(), sp.U.end(), filo);
+    if (tmat) std::copy(sp.tmat.begin(), sp.tmat.end(), tmat);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_evaluate(flosic_ctx *c, double *zsic) {
+    if (!c) return FLOSIC_EINVAL;
+    c->st = Status();
+    const int r = evaluate(c);
+    if (r) return r;
+    if (zsic) *zsic = c->zsic;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_apotnl_sic(flosic_ctx *c, int32_t ispn, double *results, double *zsic) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (results) std::copy(c->sp[s].results.begin(), c->sp[s].results.end(), results);
+    if (zsic) *zsic = c->zsic;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_siclagm(flosic_ctx *c, int32_t ispn, double *sic) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (sic) std::copy(c->sp[s].sic.begin(), c->sp[s].sic.end(), sic);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_sic_vab(flosic_ctx *c, int32_t ispn, double *vab) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (vab) std::copy(c->sp[s].vab.begin(), c->sp[s].vab.end(), vab);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_fod_forces(flosic_ctx *c, int32_t ispn, double *debdax) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (debdax) std::copy(c->sp[s].debdax.begin(), c->sp[s].debdax.end(), debdax);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_fod_gradient(flosic_ctx *c, int32_t ispn, double *grad) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (grad) std::copy(c->sp[s].grad.begin(), c->sp[s].grad.end(), grad);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_get_potential(flosic_ctx *c, int32_t ispn, int32_t iorb, double *pot, double *vh) {
+    if (!c) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    const int r = need_eval(c);
+    if (r) return r;
+    if (iorb < 1 || iorb > c->sp[s].nocc) return fail(c, FLOSIC_EINVAL, "orbital index %d out of range", iorb);
+    const int f = (s == 0 ? 0 : c->sp[0].nocc) + iorb - 1;
+    if (pot) HIPCHK(c, hipMemcpy(pot, c->d_Vtot.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    if (vh) HIPCHK(c, hipMemcpy(vh, c->d_VH.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    return FLOSIC_OK;
+}
+
+int32_t flosic_get_pairs(flosic_ctx *c, int32_t *npair, int32_t *list) {
+    if (!c || !npair) return FLOSIC_EINVAL;
+    *npair = c->npair;
+    if (list) std::copy(c->pair_list.begin(), c->pair_list.end(), list);
+    return FLOSIC_OK;
+}
+
+int32_t flosic_get_timings(flosic_ctx *c, int32_t n, double *ms) {
+    if (!c || !ms || n < 0) return FLOSIC_EINVAL;
+    for (int i = 0; i < n && i < 9; ++i) ms[i] = c->timings[i];
+    return FLOSIC_OK;
+}
+
+int32_t flosic_get_counts(flosic_ctx *c, int32_t n, int64_t *counts) {
+    if (!c || !counts || n < 0) return FLOSIC_EINVAL;
+    for (int i = 0; i < n && i < 5; ++i) counts[i] = c->counts[i];
+    return FLOSIC_OK;
+}
+
+}  // extern "C"

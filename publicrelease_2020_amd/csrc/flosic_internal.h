@@ -1,0 +1,70 @@
+// Internal interface between the C-ABI host code (flosic_api.hip) and the device kernels
+// (flosic_kernels.hip).  All pointers are device pointers unless named h_*.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flosic {
+
+constexpr double kCutExp = 40.0;     // PARAMA2:31
+constexpr int kBoysRows = 1601;      // fffmt3.F:24 table at dT = 0.01
+constexpr int kBoysCols = 8;         // F_0..F_7 per row (top + 3 <= 7)
+
+// basis as consumed by the mesh kernels: contracted shells (atom, l, contraction) with the bare
+// primitives that carry a nonzero coefficient
+struct BasisDev {
+    int natom;
+    int nbas;
+    int ncs;
+    const double *atom_xyz;   // 3*natom
+    const double *atom_r2cut; // natom: max_i CUTEXP/alpha_i (beyond it no primitive survives)
+    const int *atom_cs;       // natom+1 offsets into the shell arrays
+    const int *cs_l;          // ncs
+    const int *cs_f0;         // ncs: function index of Cartesian component 0
+    const int *cs_stride;     // ncs: distance between Cartesian components (= ncon of the (atom,l))
+    const int *cs_p0;         // ncs+1 offsets into prim arrays
+    const double *prim_alpha;
+    const double *prim_coef;
+};
+
+// primitive pair for the Hermite coefficients (sorted by L = lA + lB)
+struct PairDev {
+    double A[3], B[3];
+    double a, b;
+    int psA, psB;     // primitive shell indices (rows of the prim-shell coefficient matrix)
+    int lA, lB;
+    int row;          // first Hermite row
+    double sym;       // 2 for psA != psB, else 1
+};
+
+// per-phase launch helpers (stream-ordered)
+void boys_table(double *h_tab);  // host: fills kBoysRows x kBoysCols
+
+template <typename T>
+void launch_basis_planes(const BasisDev &B, const double *x, const double *y, const double *z, int64_t M, int nd,
+                         T *phi, int64_t ldphi, hipStream_t s);
+
+void launch_prim_coef(int nps, const int *ps_f0, const int *ps_stride, const int *ps_ncart, const int *ps_c0,
+                      const int *ps_nc, const double *ps_coef, const double *cflo, int nF, int ldc, double *dps,
+                      hipStream_t s);
+
+void launch_hermite(int npair, const PairDev *pairs, const double *dps, int nF, double *H, int ldh,
+                    hipStream_t s);
+
+// V[f*M + k] = sum_pairs sum_tuv R_tuv(P - r_k) H[row+tuv][f]
+void launch_coulomb(const int *segL /*6 offsets*/, const double4 *pairP, const int *pairRow, const double *H,
+                    int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
+                    int nF, double *V, int64_t ldv, hipStream_t s);
+
+// XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)
+void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w,
+               double *Vtot, double *U, double *partial /*[nF][nblk][9]*/, int nblk, hipStream_t s);
+
+void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out /*[nF][nq]*/,
+                            hipStream_t s);
+
+// V^i_ab for one spin: phi0 (n FLOs of this spin, [n][M]), Vtot ([n][M]), w -> out [i][b][a] partial sums
+void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nchunk,
+                double *partial /*[nchunk][n^3]*/, hipStream_t s);
+
+}  // namespace flosic

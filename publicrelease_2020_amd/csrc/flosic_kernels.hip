@@ -1,0 +1,763 @@
+// Device kernels of the FLO-SIC per-Fermi-orbital evaluation, written for gfx950 (64-wide wavefronts).
+//
+//   k_basis_planes   contracted Cartesian Gaussians and their derivatives on the mesh (gorbdrv.f:18-316);
+//                    FOD independent, computed once per system and kept resident in HBM
+//   k_prim_coef      FLO coefficients folded onto primitive shells (the ADD/DMAT contraction of
+//                    coupot_sic.F:398-482)
+//   k_hermite        Hermite expansion of every primitive pair density for every FLO (the COEFICIENT
+//                    polynomial set of poisson1.f:295-496, McMurchie-Davidson form), 2 pi/p folded in
+//   k_coulomb        exact Gaussian-pair potential at every mesh point for all FLOs at once: Boys
+//                    functions by the table/Taylor/asymptote scheme of fffmt3.F/fffmt4.F/fffmtc.F,
+//                    Hermite recursion in registers, contraction with the pair coefficients
+//   k_xc             spin-polarised LSDA/PBE for rho_up = rho_i, rho_dn = 0 (subvlxc_libxc.F90), total SIC
+//                    potential and per-block partial sums of the RESULTS energies (apotnl_sic.F:536-581)
+//   k_vab            V^i_ab = sum_k w phi_a V_i phi_b, split over point chunks
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "flosic_internal.h"
+
+namespace flosic {
+
+// ---------------------------------------------------------------------------------------------------
+// Cartesian component tables: index 0..9 = s, x, y, z, xx, yy, zz, xy, xz, yz (gorbdrv.f:78-98)
+// ---------------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int cpow(int idx, int ax) {
+    return idx == 0 ? 0
+           : idx <= 3 ? (idx - 1 == ax ? 1 : 0)
+           : idx <= 6 ? (idx - 4 == ax ? 2 : 0)
+           : idx == 7 ? (ax < 2 ? 1 : 0)
+           : idx == 8 ? (ax != 1 ? 1 : 0)
+                      : (ax > 0 ? 1 : 0);
+}
+__host__ __device__ constexpr int ncart(int l) { return l == 0 ? 1 : (l == 1 ? 3 : 6); }
+__host__ __device__ constexpr int loff(int l) { return l == 0 ? 0 : (l == 1 ? 1 : 4); }
+
+// value (+ gradient, + Hessian xx,yy,zz,xy,xz,yz) of component IDX times the radial sums
+// S0 = sum c e^{-a r^2}, S1 = sum c a e^{-a r^2}, S2 = sum c a^2 e^{-a r^2}
+template <int IDX, int ND>
+__device__ __forceinline__ void cart_eval(double x, double y, double z, double S0, double S1, double S2,
+                                          double *o) {
+    constexpr int a = cpow(IDX, 0), b = cpow(IDX, 1), c = cpow(IDX, 2);
+    const double px[3] = {1.0, x, x * x}, py[3] = {1.0, y, y * y}, pz[3] = {1.0, z, z * z};
+    const double P = px[a] * py[b] * pz[c];
+    o[0] = P * S0;
+    if constexpr (ND >= 4) {
+        const double dPx = a ? a * px[a - 1] * py[b] * pz[c] : 0.0;
+        const double dPy = b ? b * px[a] * py[b - 1] * pz[c] : 0.0;
+        const double dPz = c ? c * px[a] * py[b] * pz[c - 1] : 0.0;
+        o[1] = dPx * S0 - 2.0 * x * P * S1;
+        o[2] = dPy * S0 - 2.0 * y * P * S1;
+        o[3] = dPz * S0 - 2.0 * z * P * S1;
+        if constexpr (ND == 10) {
+            const double dPxx = a >= 2 ? a * (a - 1) * py[b] * pz[c] : 0.0;
+            const double dPyy = b >= 2 ? b * (b - 1) * px[a] * pz[c] : 0.0;
+            const double dPzz = c >= 2 ? c * (c - 1) * px[a] * py[b] : 0.0;
+            const double dPxy = (a && b) ? a * b * px[a - 1] * py[b - 1] * pz[c] : 0.0;
+            const double dPxz = (a && c) ? a * c * px[a - 1] * py[b] * pz[c - 1] : 0.0;
+            const double dPyz = (b && c) ? b * c * px[a] * py[b - 1] * pz[c - 1] : 0.0;
+            o[4] = dPxx * S0 - 4.0 * x * dPx * S1 + P * (4.0 * x * x * S2 - 2.0 * S1);
+            o[5] = dPyy * S0 - 4.0 * y * dPy * S1 + P * (4.0 * y * y * S2 - 2.0 * S1);
+            o[6] = dPzz * S0 - 4.0 * z * dPz * S1 + P * (4.0 * z * z * S2 - 2.0 * S1);
+            o[7] = dPxy * S0 - 2.0 * x * dPy * S1 - 2.0 * y * dPx * S1 + 4.0 * x * y * P * S2;
+            o[8] = dPxz * S0 - 2.0 * x * dPz * S1 - 2.0 * z * dPx * S1 + 4.0 * x * z * P * S2;
+            o[9] = dPyz * S0 - 2.0 * y * dPz * S1 - 2.0 * z * dPy * S1 + 4.0 * y * z * P * S2;
+        }
+    }
+}
+
+template <typename T, int ND, int L, int M0>
+__device__ __forceinline__ void store_comps(double x, double y, double z, double S0, double S1, double S2, T *phi,
+                                            int64_t ldphi, int64_t plane, int f0, int stride, int64_t k) {
+    if constexpr (M0 < ncart(L)) {
+        double o[ND];
+        cart_eval<loff(L) + M0, ND>(x, y, z, S0, S1, S2, o);
+        const int64_t col = (int64_t)(f0 + M0 * stride) * ldphi + k;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) phi[d * plane + col] = (T)o[d];
+        store_comps<T, ND, L, M0 + 1>(x, y, z, S0, S1, S2, phi, ldphi, plane, f0, stride, k);
+    }
+}
+
+// one thread per mesh point; planes phi[d][mu][k] (leading dimension ldphi >= M, plane = nbas*ldphi);
+// the planes must be zeroed beforehand (screened functions are not written)
+template <typename T, int ND>
+__global__ void __launch_bounds__(256) k_basis_planes(BasisDev B, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, const double *__restrict__ Z,
+                                                      int64_t M, T *__restrict__ phi, int64_t ldphi) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    const double px = X[k], py = Y[k], pz = Z[k];
+    const int64_t plane = (int64_t)B.nbas * ldphi;
+    const double geps = exp(-kCutExp);
+    for (int a = 0; a < B.natom; ++a) {
+        const double x = px - B.atom_xyz[3 * a], y = py - B.atom_xyz[3 * a + 1], z = pz - B.atom_xyz[3 * a + 2];
+        const double r2 = x * x + y * y + z * z;
+        if (!(r2 < B.atom_r2cut[a])) continue;
+        for (int s = B.atom_cs[a]; s < B.atom_cs[a + 1]; ++s) {
+            double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+            for (int p = B.cs_p0[s]; p < B.cs_p0[s + 1]; ++p) {
+                const double al = B.prim_alpha[p];
+                const double arg = al * r2;
+                if (!(arg < kCutExp)) continue;
+                const double g = exp(-arg);
+                const double cc = B.prim_coef[p];
+                if (!(fabs(cc) * g > geps)) continue;
+                const double gc = g * cc;
+                S0 += gc;
+                S1 += gc * al;
+                S2 += gc * al * al;
+            }
+            const int l = B.cs_l[s];
+            if (l == 0) store_comps<T, ND, 0, 0>(x, y, z, S0, S1, S2, phi, ldphi, plane, B.cs_f0[s], B.cs_stride[s], k);
+            else if (l == 1) store_comps<T, ND, 1, 0>(x, y, z, S0, S1, S2, phi, ldphi, plane, B.cs_f0[s], B.cs_stride[s], k);
+            else store_comps<T, ND, 2, 0>(x, y, z, S0, S1, S2, phi, ldphi, plane, B.cs_f0[s], B.cs_stride[s], k);
+        }
+    }
+}
+
+template <typename T>
+void launch_basis_planes(const BasisDev &B, const double *x, const double *y, const double *z, int64_t M, int nd,
+                         T *phi, int64_t ldphi, hipStream_t s) {
+    const dim3 grid((unsigned)((M + 255) / 256)), blk(256);
+    if (nd == 1) hipLaunchKernelGGL((k_basis_planes<T, 1>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
+    else if (nd == 4) hipLaunchKernelGGL((k_basis_planes<T, 4>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
+    else hipLaunchKernelGGL((k_basis_planes<T, 10>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
+}
+template void launch_basis_planes<double>(const BasisDev &, const double *, const double *, const double *, int64_t,
+                                          int, double *, int64_t, hipStream_t);
+template void launch_basis_planes<float>(const BasisDev &, const double *, const double *, const double *, int64_t,
+                                         int, float *, int64_t, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------------
+// FLO coefficients on primitive shells: dps[(ps*6 + m)*nF + f] = sum_c con_c * cflo[f*ldc + f0 + m*stride + c]
+// ---------------------------------------------------------------------------------------------------
+__global__ void k_prim_coef(int nps, const int *__restrict__ ps_f0, const int *__restrict__ ps_stride,
+                            const int *__restrict__ ps_ncart, const int *__restrict__ ps_c0,
+                            const int *__restrict__ ps_nc, const double *__restrict__ ps_coef,
+                            const double *__restrict__ cflo, int nF, int ldc, double *__restrict__ dps) {
+    const int ps = blockIdx.x;
+    const int f0 = ps_f0[ps], st = ps_stride[ps], nc = ps_nc[ps], c0 = ps_c0[ps], nm = ps_ncart[ps];
+    for (int f = threadIdx.x; f < nF; f += blockDim.x) {
+        for (int m = 0; m < 6; ++m) {
+            double s = 0.0;
+            if (m < nm)
+                for (int c = 0; c < nc; ++c) s += ps_coef[c0 + c] * cflo[(int64_t)f * ldc + f0 + m * st + c];
+            dps[((int64_t)ps * 6 + m) * nF + f] = s;
+        }
+    }
+}
+
+void launch_prim_coef(int nps, const int *ps_f0, const int *ps_stride, const int *ps_ncart, const int *ps_c0,
+                      const int *ps_nc, const double *ps_coef, const double *cflo, int nF, int ldc, double *dps,
+                      hipStream_t s) {
+    if (nps <= 0) return;
+    hipLaunchKernelGGL(k_prim_coef, dim3(nps), dim3(128), 0, s, nps, ps_f0, ps_stride, ps_ncart, ps_c0, ps_nc,
+                       ps_coef, cflo, nF, ldc, dps);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Hermite index tables: entries grouped by total order L' = 0..4; within an order t descending, then u
+// descending (t+u+v = L')
+// ---------------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int ntuv(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+__host__ __device__ constexpr int tuv_idx(int t, int u, int v) {
+    // position inside order block: enumerate t from L' down, u from L'-t down
+    int L = t + u + v;
+    int base = L == 0 ? 0 : ntuv(L - 1);
+    int pos = 0;
+    for (int tt = L; tt > t; --tt) pos += L - tt + 1;
+    pos += (L - t) - u;
+    return base + pos;
+}
+__host__ __device__ constexpr int tuv_comp(int i, int which) {
+    for (int L = 0; L <= 4; ++L)
+        for (int t = L; t >= 0; --t)
+            for (int u = L - t; u >= 0; --u) {
+                int v = L - t - u;
+                if (tuv_idx(t, u, v) == i) return which == 0 ? t : (which == 1 ? u : v);
+            }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Hermite coefficients of every pair for every FLO.  E^{ij}_t along one axis by the McMurchie-Davidson
+// recursion; H[row+tuv][f] = sym * 2pi/p * sum_{mu,nu} dA[mu][f] dB[nu][f] Ex Ey Ez
+// ---------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_hermite(const PairDev *__restrict__ pairs, const double *__restrict__ dps,
+                                                  int nF, double *__restrict__ H, int ldh) {
+    __shared__ double E[3][3][3][5];
+    const PairDev pr = pairs[blockIdx.x];
+    const double p = pr.a + pr.b;
+    if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double P = (pr.a * pr.A[d] + pr.b * pr.B[d]) / p;
+        const double xpa = P - pr.A[d], xpb = P - pr.B[d];
+        const double dd = pr.A[d] - pr.B[d];
+        const double K = exp(-pr.a * pr.b / p * dd * dd);
+        const double inv2p = 0.5 / p;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int t = 0; t < 5; ++t) E[d][i][j][t] = 0.0;
+        E[d][0][0][0] = K;
+        for (int i = 0; i <= 2; ++i)
+            for (int j = 0; j <= 2; ++j) {
+                if (i == 0 && j == 0) continue;
+                const int pi = i > 0 ? i - 1 : i, pj = i > 0 ? j : j - 1;
+                const double shift = i > 0 ? xpa : xpb;
+                const int np = i + j - 1;
+                for (int t = 0; t <= i + j; ++t) {
+                    double v = 0.0;
+                    if (t >= 1 && t - 1 <= np) v += inv2p * E[d][pi][pj][t - 1];
+                    if (t <= np) v += shift * E[d][pi][pj][t];
+                    if (t + 1 <= np) v += (t + 1) * E[d][pi][pj][t + 1];
+                    E[d][i][j][t] = v;
+                }
+            }
+    }
+    __syncthreads();
+    const int L = pr.lA + pr.lB;
+    const int nt = ntuv(L);
+    const double fac = pr.sym * 2.0 * M_PI / p;
+    const int nA = ncart(pr.lA), nB = ncart(pr.lB), oA = loff(pr.lA), oB = loff(pr.lB);
+    for (int f = threadIdx.x; f < nF; f += blockDim.x) {
+        double h[35];
+        for (int t = 0; t < 35; ++t) h[t] = 0.0;
+        for (int m = 0; m < nA; ++m) {
+            const double da = dps[((int64_t)pr.psA * 6 + m) * nF + f];
+            const int ax = cpow(oA + m, 0), ay = cpow(oA + m, 1), az = cpow(oA + m, 2);
+            for (int n = 0; n < nB; ++n) {
+                const double dab = da * dps[((int64_t)pr.psB * 6 + n) * nF + f];
+                if (dab == 0.0) continue;
+                const int bx = cpow(oB + n, 0), by = cpow(oB + n, 1), bz = cpow(oB + n, 2);
+                for (int t = 0; t <= ax + bx; ++t) {
+                    const double et = dab * E[0][ax][bx][t];
+                    for (int u = 0; u <= ay + by; ++u) {
+                        const double etu = et * E[1][ay][by][u];
+                        for (int v = 0; v <= az + bz; ++v) h[tuv_idx(t, u, v)] += etu * E[2][az][bz][v];
+                    }
+                }
+            }
+        }
+        for (int t = 0; t < nt; ++t) H[(int64_t)(pr.row + t) * ldh + f] = fac * h[t];
+    }
+}
+
+void launch_hermite(int npair, const PairDev *pairs, const double *dps, int nF, double *H, int ldh, hipStream_t s) {
+    if (npair <= 0) return;
+    hipLaunchKernelGGL(k_hermite, dim3(npair), dim3(128), 0, s, pairs, dps, nF, H, ldh);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Boys functions F_0..F_TOP (fffmt3.F:66-88): T > 16 one-term asymptote + upward recursion; otherwise
+// cubic Taylor of F_TOP about the nearest 0.01 node and downward recursion
+// ---------------------------------------------------------------------------------------------------
+template <int TOP>
+__device__ __forceinline__ void boys(double T, double eT, const double *__restrict__ tab, double *F) {
+    if (T > 16.0) {
+        const double rect1 = 0.5 / T;
+        F[0] = rect1 * (sqrt(T) * 1.77245385090551602728 - eT);
+#pragma unroll
+        for (int m = 1; m <= TOP; ++m) F[m] = rect1 * ((2 * m - 1) * F[m - 1] - eT);
+    } else {
+        const int it = (int)(100.0 * T + 1.5);
+        const double dt = 1.0e-2 * (it - 1) - T;
+        const double *f = tab + (it - 1) * kBoysCols;
+        F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] / 2.0 + dt * (f[TOP + 3] / 6.0)));
+        const double rect1 = 2.0 * T;
+#pragma unroll
+        for (int m = TOP - 1; m >= 0; --m) {
+            const double r = F[m + 1] * rect1 + eT;
+            F[m] = (m == 0) ? r : r * (1.0 / (2 * m + 1));
+        }
+    }
+}
+
+// one Hermite step R^{(n)}_I from R^{(n+1)} in place (entries of lower order still hold level n+1)
+template <int I>
+__device__ __forceinline__ void rstep(double *R, double X, double Y, double Z) {
+    constexpr int t = tuv_comp(I, 0), u = tuv_comp(I, 1), v = tuv_comp(I, 2);
+    if constexpr (t > 0) {
+        double r = X * R[tuv_idx(t - 1, u, v)];
+        if constexpr (t > 1) r += (t - 1) * R[tuv_idx(t - 2, u, v)];
+        R[I] = r;
+    } else if constexpr (u > 0) {
+        double r = Y * R[tuv_idx(t, u - 1, v)];
+        if constexpr (u > 1) r += (u - 1) * R[tuv_idx(t, u - 2, v)];
+        R[I] = r;
+    } else {
+        double r = Z * R[tuv_idx(t, u, v - 1)];
+        if constexpr (v > 1) r += (v - 1) * R[tuv_idx(t, u, v - 2)];
+        R[I] = r;
+    }
+}
+template <int I>
+__device__ __forceinline__ void rdown(double *R, double X, double Y, double Z) {
+    if constexpr (I >= 1) {
+        rstep<I>(R, X, Y, Z);
+        rdown<I - 1>(R, X, Y, Z);
+    }
+}
+template <int L, int N>
+__device__ __forceinline__ void rlevels(double *R, double X, double Y, double Z, const double *F, const double *pw) {
+    if constexpr (N >= 0) {
+        rdown<ntuv(L - N) - 1>(R, X, Y, Z);
+        R[0] = pw[N] * F[N];
+        rlevels<L, N - 1>(R, X, Y, Z, F, pw);
+    }
+}
+
+// R_tuv(p, P - C) for all t+u+v <= L, TUV order
+template <int L>
+__device__ __forceinline__ void hermite_R(double p, double X, double Y, double Z, const double *__restrict__ tab,
+                                          double *R) {
+    constexpr int TOP = L < 2 ? 2 : L;
+    const double T = p * (X * X + Y * Y + Z * Z);
+    double F[TOP + 1];
+    boys<TOP>(T, exp(-T), tab, F);
+    double pw[L + 1];
+    pw[0] = 1.0;
+#pragma unroll
+    for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * (-2.0 * p);
+    R[0] = pw[L] * F[L];
+    rlevels<L, L - 1>(R, X, Y, Z, F, pw);
+}
+
+template <int L, int CH>
+__device__ __forceinline__ void coulomb_segment(int beg, int end, const double4 *__restrict__ pP,
+                                                const int *__restrict__ prow, const double *__restrict__ H, int ldh,
+                                                int f0, double x, double y, double z, const double *__restrict__ tab,
+                                                double *acc) {
+    for (int ip = beg; ip < end; ++ip) {
+        const double4 P = pP[ip];
+        const int row = prow[ip];
+        double R[ntuv(L)];
+        hermite_R<L>(P.w, P.x - x, P.y - y, P.z - z, tab, R);
+        const double *h = H + (int64_t)row * ldh + f0;
+#pragma unroll
+        for (int t = 0; t < ntuv(L); ++t) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) acc[c] = fma(R[t], h[(int64_t)t * ldh + c], acc[c]);
+        }
+    }
+}
+
+// block = 4 wavefronts; wave w of block (bx, by) handles points bx*64 .. +63 and FLO chunk (by*4 + w)
+template <int CH>
+__global__ void __launch_bounds__(256) k_coulomb(int s0, int s1, int s2, int s3, int s4, int s5,
+                                                  const double4 *__restrict__ pP, const int *__restrict__ prow,
+                                                  const double *__restrict__ H, int ldh, const double *__restrict__ X,
+                                                  const double *__restrict__ Y, const double *__restrict__ Z,
+                                                  int64_t M, const double *__restrict__ tab, int nF,
+                                                  double *__restrict__ V, int64_t ldv) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int f0 = (blockIdx.y * 4 + wave) * CH;
+    if (f0 >= nF) return;
+    const int64_t k = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = k < M;
+    const double x = valid ? X[k] : 0.0, y = valid ? Y[k] : 0.0, z = valid ? Z[k] : 0.0;
+    double acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+    coulomb_segment<0, CH>(s0, s1, pP, prow, H, ldh, f0, x, y, z, tab, acc);
+    coulomb_segment<1, CH>(s1, s2, pP, prow, H, ldh, f0, x, y, z, tab, acc);
+    coulomb_segment<2, CH>(s2, s3, pP, prow, H, ldh, f0, x, y, z, tab, acc);
+    coulomb_segment<3, CH>(s3, s4, pP, prow, H, ldh, f0, x, y, z, tab, acc);
+    coulomb_segment<4, CH>(s4, s5, pP, prow, H, ldh, f0, x, y, z, tab, acc);
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (f0 + c < nF) V[(int64_t)(f0 + c) * ldv + k] = acc[c];
+    }
+}
+
+void launch_coulomb(const int *segL, const double4 *pairP, const int *pairRow, const double *H, int ldh,
+                    const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF, double *V,
+                    int64_t ldv, hipStream_t s) {
+    constexpr int CH = 16;
+    const int nchunk = (nF + CH - 1) / CH;
+    const dim3 grid((unsigned)((M + 63) / 64), (unsigned)((nchunk + 3) / 4)), blk(256);
+    hipLaunchKernelGGL((k_coulomb<CH>), grid, blk, 0, s, segL[0], segL[1], segL[2], segL[3], segL[4], segL[5], pairP,
+                       pairRow, H, ldh, x, y, z, M, btab, nF, V, ldv);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// exchange-correlation for rho_up = rho_i, rho_dn = 0, NSPN = 2 (subvlxc_libxc.F90:197-470)
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void gcor(double A, double A1, double B1, double B2, double B3, double B4, double RS,
+                                     double &GG, double &GGRS) {
+    const double Q0 = -2 * A * (1.0 + A1 * RS);
+    const double RS12 = sqrt(RS);
+    const double RS32 = RS12 * RS12 * RS12;
+    const double Q1 = 2 * A * (B1 * RS12 + B2 * RS + B3 * RS32 + B4 * RS * RS);
+    const double Q2 = log(1.0 + 1.0 / Q1);
+    GG = Q0 * Q2;
+    const double Q3 = A * (B1 / RS12 + 2 * B2 + 3 * B3 * RS12 + 4 * B4 * RS);
+    GGRS = -2 * A * A1 * Q2 - Q0 * Q3 / (Q1 * Q1 + Q1);
+}
+__device__ __forceinline__ void gcor2(double A, double A1, double B1, double B2, double B3, double B4, double RTRS,
+                                      double &GG, double &GGRS) {
+    const double Q0 = -2.0 * A * (1.0 + A1 * RTRS * RTRS);
+    const double Q1 = 2.0 * A * RTRS * (B1 + RTRS * (B2 + RTRS * (B3 + B4 * RTRS)));
+    const double Q2 = log(1.0 + 1.0 / Q1);
+    GG = Q0 * Q2;
+    const double Q3 = A * (B1 / RTRS + 2.0 * B2 + RTRS * (3.0 * B3 + 4.0 * B4 * RTRS));
+    GGRS = -2.0 * A * A1 * Q2 - Q0 * Q3 / (Q1 * (1.0 + Q1));
+}
+
+// PW91 exchange (pw91ex.f:7-50)
+__device__ __forceinline__ void pw91ex(double DKF, double S, double U, double V, double &EXL, double &EXN,
+                                       double &VX) {
+    const double A1 = 0.19645, A2 = 0.27430, A3 = 0.15084, A4 = 100.0, AX = -0.238732414, A = 7.7956, B1 = 0.004;
+    const double THRD4 = 1.3333333333333333;
+    const double FAC = AX * DKF;
+    const double S2 = S * S, S3 = S2 * S, S4 = S2 * S2;
+    const double AS = A * S;
+    const double PA = sqrt(1.0 + AS * AS);
+    const double P0 = AS / PA;
+    const double P1 = A1 * log(AS + PA);
+    const double P2 = A3 * exp(-A4 * S2);
+    const double P3 = 1.0 / (1.0 + S * P1 + B1 * S4);
+    const double P4 = 1.0 + S * P1 + (A2 - P2) * S2;
+    const double F = P3 * P4;
+    EXL = FAC;
+    EXN = FAC * (F - 1.0);
+    const double P5 = B1 * S2 - (A2 - P2);
+    const double P6 = S * (P1 + A1 * P0);
+    const double P7 = 2 * ((A2 - P2) + A4 * S2 * P2 - 2 * B1 * S2 * F);
+    const double FS = P3 * (P3 * P5 * P6 + P7);
+    const double P8 = 2 * S * (B1 - A4 * P2);
+    const double P9 = P1 + A1 * P0 * (3.0 - P0 * P0);
+    const double P10 = 4 * (A4 * S * P2 * (2.0 - A4 * S2) - 2 * B1 * S * F - B1 * S3 * FS);
+    const double P11 = P3 * P3 * (P1 + A1 * P0 + 4 * B1 * S3);
+    const double FSS = P3 * P3 * (P5 * P9 + P6 * P8) - 2 * P3 * P5 * P6 * P11 + P3 * P10 - P7 * P11;
+    VX = FAC * (THRD4 * F - (U - THRD4 * S3) * FSS - V * FS);
+}
+
+// PW92 local correlation at zeta = 1 (pw91lc.f:4-43)
+__device__ __forceinline__ void pw91lc_pol(double RS, double &EC, double &VCUP) {
+    const double GAMR = 1.92366105, FZZR = 0.58482234, THRD = 0.333333333333, THRD4 = 1.333333333333;
+    const double ZETPR = pow(2.0, THRD);
+    const double F = (ZETPR * ZETPR * ZETPR * ZETPR - 2.0) * GAMR;
+    double EU, EURS, EP, EPRS, ALFM, ALFRSM;
+    gcor(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, RS, EU, EURS);
+    gcor(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, RS, EP, EPRS);
+    gcor(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, RS, ALFM, ALFRSM);
+    // Z3 = Z4 = 1
+    EC = EU * (1.0 - F) + EP * F - ALFM * F * (1.0 - 1.0) * FZZR;
+    const double ECRS = EURS * (1.0 - F) + EPRS * F - ALFRSM * F * (1.0 - 1.0) * FZZR;
+    const double FZ = THRD4 * ZETPR * GAMR;
+    const double ECZET = 4.0 * F * (EP - EU + ALFM * FZZR) + FZ * (EP - EU - (1.0 - 1.0) * ALFM * FZZR);
+    const double COMM = EC - RS * ECRS * THRD - ECZET;
+    VCUP = COMM + ECZET;
+}
+
+// PBE exchange (pbeex.f, MODE=1)
+__device__ __forceinline__ void pbeex(double RHO, double S, double U, double V, double &EXL, double &EXN,
+                                      double &VX) {
+    const double THRD = 1.0 / 3.0, THRD4 = 4.0 / 3.0;
+    const double AX = -0.738558766382022405884230032680836, UM = 0.2195149727645171, UK = 0.8040, UL = UM / UK;
+    const double EXUNIF = AX * pow(RHO, THRD);
+    const double S2 = S * S;
+    const double P0 = 1.0 + UL * S2;
+    const double FXPBE = 1.0 + UK - UK / P0;
+    EXL = EXUNIF;
+    EXN = EXUNIF * (FXPBE - 1.0);
+    const double FS = 2.0 * UK * UL / (P0 * P0);
+    const double FSS = -4.0 * UL * S * FS / P0;
+    VX = EXUNIF * (THRD4 * FXPBE - (U - THRD4 * S2 * S) * FSS - V * FS);
+}
+
+// PBE correlation at zeta = 1, WW = 0 (pbecor.f)
+__device__ __forceinline__ void pbecor_pol(double RS, double T, double UU, double VV, double &EC, double &VCUP,
+                                           double &H, double &DVCUP) {
+    const double THRD = 1.0 / 3.0, THRD2 = 2.0 * THRD, THRD4 = 4.0 * THRD, SIXTHM = -THRD / 2.0;
+    const double GAM = 0.5198420997897463295344212145565, FZZ = 8.0 / (9.0 * GAM);
+    const double GAMMA = 0.03109069086965489503494086371273, BET = 0.06672455060314922, DELT = BET / GAMMA;
+    const double ETA = 1.0e-12;
+    const double ZET = 1.0;
+    const double RTRS = sqrt(RS);
+    double EU, EURS, EP, EPRS, ALFM, ALFRSM;
+    gcor2(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, RTRS, EU, EURS);
+    gcor2(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, RTRS, EP, EPRS);
+    gcor2(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, RTRS, ALFM, ALFRSM);
+    const double Z4 = 1.0;
+    const double F = (pow(1.0 + ZET, THRD4) + 0.0 - 2.0) / GAM;
+    const double ec = EU * (1.0 - F * Z4) + EP * F * Z4 - ALFM * F * (1.0 - Z4) / FZZ;
+    const double ECRS = EURS * (1.0 - F * Z4) + EPRS * F * Z4 - ALFRSM * F * (1.0 - Z4) / FZZ;
+    const double FZ = THRD4 * (pow(1.0 + ZET, THRD) - 0.0) / GAM;
+    const double ECZET = 4.0 * F * (EP - EU + ALFM / FZZ) + FZ * (Z4 * EP - Z4 * EU - (1.0 - Z4) * ALFM / FZZ);
+    const double COMM0 = ec - RS * ECRS / 3.0 - ZET * ECZET;
+    EC = ec;
+    VCUP = COMM0 + ECZET;
+    const double G = (pow(1.0 + ZET, THRD2) + 0.0) / 2.0;
+    const double G3 = G * G * G;
+    const double PON = -ec / (G3 * GAMMA);
+    const double B = DELT / (exp(PON) - 1.0);
+    const double B2 = B * B, T2 = T * T, T4 = T2 * T2;
+    const double Q4 = 1.0 + B * T2;
+    const double Q5 = 1.0 + B * T2 + B2 * T4;
+    const double h = G3 * (BET / DELT) * log(1.0 + DELT * Q4 * T2 / Q5);
+    const double G4 = G3 * G, T6 = T4 * T2, RSTHRD = RS / 3.0;
+    const double GZ = (pow((1.0 + ZET) * (1.0 + ZET) + ETA, SIXTHM) - pow(ETA, SIXTHM)) / 3.0;
+    const double FAC = DELT / B + 1.0;
+    const double BG = -3.0 * B2 * ec * FAC / (BET * G4);
+    const double BEC = B2 * FAC / (BET * G3);
+    const double Q8 = Q5 * Q5 + DELT * Q4 * Q5 * T2;
+    const double Q9 = 1.0 + 2.0 * B * T2;
+    const double HB = -BET * G3 * B * T6 * (2.0 + B * T2) / Q8;
+    const double HRS = -RSTHRD * HB * BEC * ECRS;
+    const double FACT0 = 2.0 * DELT - 6.0 * B;
+    const double FACT1 = Q5 * Q9 + Q4 * Q9 * Q9;
+    const double HBT = 2.0 * BET * G3 * T4 * ((Q4 * Q5 * FACT0 - DELT * FACT1) / Q8) / Q8;
+    const double HRST = RSTHRD * T2 * HBT * BEC * ECRS;
+    const double HZ = 3.0 * GZ * h / G + HB * (BG * GZ + BEC * ECZET);
+    const double HT = 2.0 * BET * G3 * Q9 / Q8;
+    const double FACT2 = Q4 * Q5 + B * T2 * (Q4 * Q9 + Q5);
+    const double FACT3 = 2.0 * B * Q5 * Q9 + DELT * FACT2;
+    const double HTT = 4.0 * BET * G3 * T * (2.0 * B / Q8 - (Q9 * FACT3 / Q8) / Q8);
+    double comm = h + HRS + HRST + T2 * HT / 6.0 + 7.0 * T2 * T * HTT / 6.0;
+    const double PREF = HZ - GZ * T2 * HT / G;
+    comm = comm - PREF * ZET - UU * HTT - VV * HT;
+    H = h;
+    DVCUP = comm + PREF;
+}
+
+// rg: rho, grad(3), Hessian xx,yy,zz,xy,xz,yz of the orbital density; returns v_xc and EXL,EXN,ECL,ECN
+// densities per unit weight
+template <int GGA>
+__device__ __forceinline__ void xc_point(const double *rg, double &vxc, double *e4) {
+    const double THIRD = 0.3333333333333333, TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627;
+    const double DKF2RS = 1.9191582926775128, EPS = 1.0e-20;
+    const double rho = rg[0] < 0.0 ? 0.0 : rg[0];
+    e4[0] = e4[1] = e4[2] = e4[3] = 0.0;
+    vxc = 0.0;
+    if (rho < EPS) return;
+    double DG2 = 0.0, DGG = 0.0, DLP = 0.0;
+    if (GGA) {
+        DG2 = rg[1] * rg[1] + rg[2] * rg[2] + rg[3] * rg[3];
+        DGG = rg[4] * rg[1] * rg[1] + rg[5] * rg[2] * rg[2] + rg[6] * rg[3] * rg[3] +
+              2 * (rg[7] * rg[1] * rg[2] + rg[8] * rg[1] * rg[3] + rg[9] * rg[2] * rg[3]);
+        DLP = rg[4] + rg[5] + rg[6];
+    }
+    {   // exchange of the up spin at density 2 rho
+        const double D = 2.0 * rho;
+        if (D > EPS) {
+            const double DKF = pow(TRPI2 * D, THIRD);
+            double S = 0.0, U = 0.0, V = 0.0;
+            if (GGA) {
+                const double DGRAD = 2.0 * sqrt(DG2);
+                const double DLAP = 2.0 * DLP;
+                const double REC2 = 0.5 / (DKF * D);
+                const double REC1 = REC2 * D;
+                const double REC22 = REC2 * REC2;
+                S = DGRAD * REC2;
+                V = DLAP * REC1 * REC2;
+                U = DGRAD > EPS ? 8.0 * DGG * REC1 * REC22 / DGRAD : 0.0;
+            }
+            double EX, DEX, VX;
+            if (GGA) pbeex(D, S, U, V, EX, DEX, VX);
+            else pw91ex(DKF, S, U, V, EX, DEX, VX);
+            vxc += VX;
+            e4[0] += 0.5 * EX * D;
+            e4[1] += 0.5 * DEX * D;
+        }
+    }
+    {   // correlation, zeta = 1
+        const double D = rho;
+        const double DKF = pow(TRPI2 * D, THIRD);
+        const double RS = DKF2RS / DKF;
+        double EC, VCUP, DEC = 0.0, DVCUP = 0.0;
+        if (!GGA) {
+            pw91lc_pol(RS, EC, VCUP);
+        } else {
+            const double DKS = sqrt(FRDPI * DKF);
+            const double G = 0.5 * pow(2.0, 0.6666666666666667);
+            const double DGRAD = sqrt(DG2);
+            const double REC3 = 1.0 / D;
+            const double REC1 = 0.5 / (DKS * G);
+            const double REC2 = REC1 * REC3;
+            const double REC22 = REC2 * REC2;
+            const double T = DGRAD * REC2;
+            const double VV = DLP * REC1 * REC2;
+            const double UU = DGRAD > EPS ? DGG * REC1 * REC22 / DGRAD : 0.0;
+            pbecor_pol(RS, T, UU, VV, EC, VCUP, DEC, DVCUP);
+        }
+        e4[2] += EC * D;
+        e4[3] += DEC * D;
+        vxc += VCUP + DVCUP;
+    }
+}
+
+// thread (k, f); per-block partial sums q: 0 XCH1 1 XCOU 2 XEXX 3 -sum w rho vxc 4 EXL 5 EXN 6 ECL 7 ECN
+// 8 sum|V_H|
+template <int GGA>
+__global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int64_t M, int nF,
+                                             const double *__restrict__ VH, const double *__restrict__ w,
+                                             double *__restrict__ Vtot, double *__restrict__ U,
+                                             double *__restrict__ partial, int nblk) {
+    __shared__ double red[9][256];
+    const int f = blockIdx.y;
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double q[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q[i] = 0.0;
+    if (k < M) {
+        const int64_t plane = (int64_t)nF * M;
+        const int64_t off = (int64_t)f * M + k;
+        const double ph = phi[off];
+        double rg[10];
+        rg[0] = ph * ph;
+        if (GGA) {
+            const double gx = phi[plane + off], gy = phi[2 * plane + off], gz = phi[3 * plane + off];
+            rg[1] = 2 * ph * gx;
+            rg[2] = 2 * ph * gy;
+            rg[3] = 2 * ph * gz;
+            rg[4] = 2 * (gx * gx + phi[4 * plane + off] * ph);
+            rg[5] = 2 * (gy * gy + phi[5 * plane + off] * ph);
+            rg[6] = 2 * (gz * gz + phi[6 * plane + off] * ph);
+            rg[7] = 2 * (gx * gy + phi[7 * plane + off] * ph);
+            rg[8] = 2 * (gx * gz + phi[8 * plane + off] * ph);
+            rg[9] = 2 * (gy * gz + phi[9 * plane + off] * ph);
+        }
+        double vxc, e4[4];
+        xc_point<GGA>(rg, vxc, e4);
+        const double wk = w[k];
+        const double vh = VH[off];
+        const double vt = vxc + vh;
+        Vtot[off] = vt;
+        U[off] = wk * vt * ph;
+        q[0] = wk * rg[0];
+        q[1] = wk * rg[0] * vh;
+        q[2] = wk * pow(fabs(rg[0]), (double)(4.0f / 3.0f));
+        q[3] = -wk * rg[0] * vxc;
+        q[4] = e4[0] * wk;
+        q[5] = e4[1] * wk;
+        q[6] = e4[2] * wk;
+        q[7] = e4[3] * wk;
+        q[8] = fabs(vh);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) red[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) partial[((int64_t)f * nblk + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w, double *Vtot,
+               double *U, double *partial, int nblk, hipStream_t s) {
+    const dim3 grid((unsigned)nblk, (unsigned)nF), blk(256);
+    if (xc == 1) hipLaunchKernelGGL((k_xc<1>), grid, blk, 0, s, phi, M, nF, VH, w, Vtot, U, partial, nblk);
+    else hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, w, Vtot, U, partial, nblk);
+    (void)nd;
+}
+
+// deterministic compensated (Neumaier) reduction of per-block partials: thread per (f, q)
+__global__ void k_reduce_partials(const double *__restrict__ partial, int nF, int nblk, int nq,
+                                  double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nF * nq) return;
+    const int f = i / nq, q = i % nq;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        const double x = partial[((int64_t)f * nblk + b) * nq + q];
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    out[i] = s + c;
+}
+
+void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out, hipStream_t s) {
+    const int n = nF * nq;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((n + 127) / 128), dim3(128), 0, s, partial, nF, nblk, nq, out);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// V^i_ab partial sums over point chunks.  Block (i, chunk); tile of 64 points staged in LDS as [p][a]
+// ---------------------------------------------------------------------------------------------------
+constexpr int kVabMaxN = 128;
+constexpr int kVabTile = 32;
+
+__global__ void __launch_bounds__(256) k_vab(const double *__restrict__ phi0, const double *__restrict__ Vtot,
+                                              const double *__restrict__ w, int64_t M, int n, int nchunk,
+                                              double *__restrict__ partial) {
+    __shared__ double sphi[kVabTile][kVabMaxN + 1];
+    __shared__ double su[kVabTile];
+    const int i = blockIdx.x, c = blockIdx.y;
+    const int64_t k0 = M * c / nchunk, k1 = M * (c + 1) / nchunk;
+    const int nout = n * n;
+    constexpr int kMaxPer = kVabMaxN * kVabMaxN / 256;
+    double acc[kMaxPer];
+#pragma unroll
+    for (int j = 0; j < kMaxPer; ++j) acc[j] = 0.0;
+    for (int64_t t0 = k0; t0 < k1; t0 += kVabTile) {
+        const int np = (int)((k1 - t0) < kVabTile ? (k1 - t0) : kVabTile);
+        for (int e = threadIdx.x; e < kVabTile * n; e += 256) {
+            const int p = e % kVabTile, a = e / kVabTile;
+            sphi[p][a] = p < np ? phi0[(int64_t)a * M + t0 + p] : 0.0;
+        }
+        if (threadIdx.x < kVabTile) {
+            const int p = threadIdx.x;
+            su[p] = p < np ? w[t0 + p] * Vtot[(int64_t)i * M + t0 + p] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kMaxPer; ++j) {
+            const int o = threadIdx.x + 256 * j;
+            if (o < nout) {
+                const int a = o % n, b = o / n;
+                double s = acc[j];
+                for (int p = 0; p < kVabTile; ++p) s = fma(sphi[p][a] * su[p], sphi[p][b], s);
+                acc[j] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxPer; ++j) {
+        const int o = threadIdx.x + 256 * j;
+        if (o < nout) partial[((int64_t)c * n + i) * nout + o] = acc[j];
+    }
+}
+
+void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nchunk, double *partial,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_vab, dim3((unsigned)n, (unsigned)nchunk), dim3(256), 0, s, phi0, Vtot, w, M, n, nchunk,
+                       partial);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// host: Boys table (fffmt3.F:24-65): F_19 by its series, downward recursion, F_0..F_7 kept
+// ---------------------------------------------------------------------------------------------------
+void boys_table(double *tab) {
+    for (int i = 0; i < kBoysCols; ++i) tab[i] = 1.0 / (2 * (i + 1) - 1);
+    for (int L = 2; L <= kBoysRows; ++L) {
+        const double T = 1.0e-2 * (L - 1);
+        const double X = 2 * T;
+        double den = 39.0, term = 1.0 / den, sum = term;
+        for (int i = 2; i <= 100; ++i) {
+            den += 2.0;
+            term = term * X / den;
+            sum = sum + term;
+            if (term / sum - 1.0e-12 <= 0) break;
+        }
+        const double ex = exp(-T);
+        double G[21];
+        G[20] = ex * sum;
+        for (int i = 1; i <= 19; ++i) {
+            const int kk = 20 - i;
+            G[kk] = (X * G[kk + 1] + ex) / (2 * kk - 1);
+        }
+        for (int i = 0; i < kBoysCols; ++i) tab[(L - 1) * kBoysCols + i] = G[i + 1];
+    }
+}
+
+}  // namespace flosic

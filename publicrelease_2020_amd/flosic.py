@@ -1,0 +1,236 @@
+"""Python host mirror of the FLO-SIC hot path over the C ABI of libflosic_hip.so.
+
+The methods carry the names of the reference subroutines they replace (FLOSIC/PublicRelease_2020):
+  frmorb()         FRMORB2(0,0) + FRMIORB      Fermi orbitals at the FODs, Lowdin, TMAT   frmorb2.F:140-525
+  apotnl_sic()     APOTNL_SIC over all FLOs    RESULTS(13,:) per spin, E_SIC               apotnl_sic.F:31-668
+  siclagm()        SICLAGM                     SIC(j,i), j over all orbitals               frmorb_mpi.F:2-178
+  sic_vab()        AHAMMAT in FLO space        V^i_ab                                       sicmat.F:380-411
+  fod_forces()     FRMORB2(-ISPN,-1)           DEBDAX = +dE_SIC/da                          frmorb2.F:589-729
+  fod_gradient()   adjoint dE_SIC/da           (no reference entry point; equals DEBDAX)
+  potential()      ZPOTnnnn                    V_i = V_xc + V_H on the mesh                 apotnl_sic.F:497-522
+Errors are raised as FlosicError carrying the library status, the way the reference stops (STOPIT).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflosic_hip.so")
+
+OK, EINVAL, ELOWDIN, ECOUPOT, EDEVICE, ESTATE, ENOMEM = range(7)
+XC_LSDA, XC_PBE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+
+
+class FlosicError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"flosic status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def library():
+    """Load the HIP library; raises if it has not been built (there is no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlosicError(ESTATE, f"{LIB_PATH} is missing: run publicrelease_2020_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "flosic_create": [C.c_int32, C.POINTER(vp)],
+            "flosic_destroy": [vp],
+            "flosic_set_basis": [vp, C.c_int32, _dp, _ip, _dp, _ip, _ip, _dp],
+            "flosic_set_mesh": [vp, C.c_int64, _dp, _dp],
+            "flosic_set_options": [vp, C.c_int32, C.c_int32, C.c_int32],
+            "flosic_set_orbitals": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp],
+            "flosic_set_fods": [vp, C.c_int32, C.c_int32, _dp],
+            "flosic_frmorb": [vp],
+            "flosic_get_fo": [vp, C.c_int32, _dp, _dp, _dp, _dp, _dp],
+            "flosic_evaluate": [vp, _dp],
+            "flosic_apotnl_sic": [vp, C.c_int32, _dp, _dp],
+            "flosic_siclagm": [vp, C.c_int32, _dp],
+            "flosic_sic_vab": [vp, C.c_int32, _dp],
+            "flosic_fod_forces": [vp, C.c_int32, _dp],
+            "flosic_fod_gradient": [vp, C.c_int32, _dp],
+            "flosic_get_potential": [vp, C.c_int32, C.c_int32, _dp, _dp],
+            "flosic_get_pairs": [vp, _ip, _ip],
+            "flosic_get_timings": [vp, C.c_int32, _dp],
+            "flosic_get_counts": [vp, C.c_int32, _lp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int32
+        L.flosic_last_error.argtypes = [vp]
+        L.flosic_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list:
+    return ["flosic_create", "flosic_destroy", "flosic_last_error", "flosic_set_basis", "flosic_set_mesh",
+            "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
+            "flosic_evaluate", "flosic_apotnl_sic", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
+            "flosic_fod_gradient", "flosic_get_potential", "flosic_get_pairs", "flosic_get_timings",
+            "flosic_get_counts"]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Flosic:
+    """One evaluation context on a HIP device, holding basis, mesh, orbitals and FODs."""
+
+    def __init__(self, device: int = 0):
+        self._L = library()
+        h = C.c_void_p()
+        st = self._L.flosic_create(int(device), C.byref(h))
+        if st != OK:
+            raise FlosicError(st, "flosic_create failed (no HIP device?)")
+        self._h = h
+        self._n = [0, 0]
+        self._nwf = [0, 0]
+        self.nbas = 0
+        self.npts = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.flosic_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st: int):
+        if st != OK:
+            raise FlosicError(st, self._L.flosic_last_error(self._h).decode())
+
+    # ---- set-up (module data of the reference) ----
+    def set_basis(self, basis):
+        self._keep_basis = (np.ascontiguousarray(basis.xyz, np.float64), np.ascontiguousarray(basis.bare_off, np.int32),
+                            np.ascontiguousarray(basis.alpha, np.float64),
+                            np.ascontiguousarray(basis.ncon, np.int32).ravel(),
+                            np.ascontiguousarray(basis.con_off, np.int32).ravel(),
+                            np.ascontiguousarray(basis.con, np.float64))
+        x, bo, al, nc, co, cn = self._keep_basis
+        self._chk(self._L.flosic_set_basis(self._h, basis.natom, _d(x), bo.ctypes.data_as(_ip), _d(al),
+                                           nc.ctypes.data_as(_ip), co.ctypes.data_as(_ip), _d(cn)))
+        self.nbas = basis.nbas
+
+    def set_mesh(self, xyz, w):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        self._chk(self._L.flosic_set_mesh(self._h, w.size, _d(xyz), _d(w)))
+        self.npts = w.size
+
+    def set_options(self, nspn: int = 2, xc: int = XC_LSDA, precision: int = PREC_FP64):
+        self._chk(self._L.flosic_set_options(self._h, nspn, xc, precision))
+
+    def set_orbitals(self, ispn: int, coef, nocc: int):
+        c = np.asfortranarray(coef, np.float64)
+        self._chk(self._L.flosic_set_orbitals(self._h, ispn, c.shape[0], c.shape[1], nocc, _d(c)))
+        self._n[ispn - 1] = nocc
+        self._nwf[ispn - 1] = c.shape[1]
+
+    def set_fods(self, ispn: int, fod):
+        f = np.ascontiguousarray(fod, np.float64).reshape(-1, 3)
+        self._chk(self._L.flosic_set_fods(self._h, ispn, f.shape[0], _d(f)))
+
+    def load_system(self, sysm, precision: int = PREC_FP64):
+        self.set_basis(sysm.basis)
+        self.set_mesh(sysm.mesh.xyz, sysm.mesh.w)
+        self.set_options(sysm.nspn, sysm.xc, precision)
+        for s in (1, 2):
+            self.set_orbitals(s, sysm.coef[s - 1], sysm.nocc[s - 1])
+            self.set_fods(s, sysm.fod[s - 1])
+
+    # ---- hot path ----
+    def frmorb(self):
+        self._chk(self._L.flosic_frmorb(self._h))
+
+    def fermi_orbitals(self, ispn: int) -> dict:
+        n = self._n[ispn - 1]
+        fm, ov, q, U, tm = (np.zeros(4 * n * n), np.zeros(n * n), np.zeros(n), np.zeros(n * n), np.zeros(n * n))
+        self._chk(self._L.flosic_get_fo(self._h, ispn, _d(fm), _d(ov), _d(q), _d(U), _d(tm)))
+        return dict(fmat=fm.reshape(4, n, n), over=ov.reshape(n, n).T, q=q, U=U.reshape(n, n).T,
+                    tmat=tm.reshape(n, n).T)
+
+    def evaluate(self) -> float:
+        z = np.zeros(1)
+        self._chk(self._L.flosic_evaluate(self._h, _d(z)))
+        return float(z[0])
+
+    def apotnl_sic(self, ispn: int) -> np.ndarray:
+        n = self._n[ispn - 1]
+        r = np.zeros(13 * n)
+        self._chk(self._L.flosic_apotnl_sic(self._h, ispn, _d(r), None))
+        return r.reshape(n, 13)
+
+    def siclagm(self, ispn: int) -> np.ndarray:
+        n, nwf = self._n[ispn - 1], self._nwf[ispn - 1]
+        s = np.zeros(nwf * n)
+        self._chk(self._L.flosic_siclagm(self._h, ispn, _d(s)))
+        return s.reshape(n, nwf).T  # [j, i] = SIC(j,i)
+
+    def sic_vab(self, ispn: int) -> np.ndarray:
+        n = self._n[ispn - 1]
+        v = np.zeros(n ** 3)
+        self._chk(self._L.flosic_sic_vab(self._h, ispn, _d(v)))
+        return np.transpose(v.reshape(n, n, n), (0, 2, 1)).copy()  # [i, a, b]
+
+    def fod_forces(self, ispn: int) -> np.ndarray:
+        n = self._n[ispn - 1]
+        d = np.zeros(3 * n)
+        self._chk(self._L.flosic_fod_forces(self._h, ispn, _d(d)))
+        return d.reshape(n, 3)
+
+    def fod_gradient(self, ispn: int) -> np.ndarray:
+        n = self._n[ispn - 1]
+        d = np.zeros(3 * n)
+        self._chk(self._L.flosic_fod_gradient(self._h, ispn, _d(d)))
+        return d.reshape(n, 3)
+
+    def potential(self, ispn: int, iorb: int):
+        p = np.zeros(self.npts)
+        vh = np.zeros(self.npts)
+        self._chk(self._L.flosic_get_potential(self._h, ispn, iorb, _d(p), _d(vh)))
+        return p, vh
+
+    def pairs(self) -> np.ndarray:
+        n = C.c_int32()
+        self._chk(self._L.flosic_get_pairs(self._h, C.byref(n), None))
+        out = np.zeros((max(n.value, 1), 6), np.int32)
+        self._chk(self._L.flosic_get_pairs(self._h, C.byref(n), out.ctypes.data_as(_ip)))
+        return out[:n.value]
+
+    def timings(self) -> np.ndarray:
+        t = np.zeros(9)
+        self._chk(self._L.flosic_get_timings(self._h, 9, _d(t)))
+        return t
+
+    def counts(self) -> np.ndarray:
+        c = np.zeros(5, np.int64)
+        self._chk(self._L.flosic_get_counts(self._h, 5, c.ctypes.data_as(_lp)))
+        return c
+
+    def results(self) -> dict:
+        """Everything the evaluation produced, per spin, in natural numpy layouts."""
+        out = dict(esic=None, spins=[])
+        for s in (1, 2):
+            out["spins"].append(dict(results=self.apotnl_sic(s), sic=self.siclagm(s), vab=self.sic_vab(s),
+                                     debdax=self.fod_forces(s), grad=self.fod_gradient(s)))
+        return out

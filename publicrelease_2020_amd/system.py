@@ -21,9 +21,46 @@ import dataclasses
 import math
 from typing import List, Optional, Sequence
 
+import ctypes
+import os
+import subprocess
+
 import numpy as np
 
 CUTEXP = 40.0  # PARAMA2:31
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SETUP_SRC = os.path.join(_HERE, "csrc", "setup_host.cpp")
+_SETUP_LIB = os.path.join(_HERE, "libflosic_setup.so")
+_setup = None
+
+
+def build_setup_lib() -> str:
+    """Compile the OpenMP set-up helpers (Becke weights, AO overlap) in-tree."""
+    tmp = _SETUP_LIB + ".tmp"
+    subprocess.check_call(["g++", "-O3", "-fopenmp", "-fPIC", "-shared", "-std=c++17", _SETUP_SRC, "-o", tmp])
+    os.replace(tmp, _SETUP_LIB)
+    return _SETUP_LIB
+
+
+def _setup_lib():
+    global _setup
+    if _setup is None:
+        if not os.path.exists(_SETUP_LIB) or os.path.getmtime(_SETUP_LIB) < os.path.getmtime(_SETUP_SRC):
+            build_setup_lib()
+        L = ctypes.CDLL(_SETUP_LIB)
+        dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+        L.setup_becke.argtypes = [ctypes.c_int, dp, ctypes.c_int64, dp, ip, dp]
+        L.setup_overlap.argtypes = [ctypes.c_int, dp, ip, dp, ip, ip, dp, ctypes.c_int, dp]
+        _setup = L
+    return _setup
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
 # Cartesian powers in reference order (gorbdrv.f:78-98)
 CART = {
@@ -152,81 +189,17 @@ def build_basis(xyz: np.ndarray, Z: Sequence[int], rules=SHELL_RULES, ladder=EXP
                  con=np.asarray(con, dtype=np.float64))
 
 
-def _md_e0(i: int, j: int, xpa, xpb, p, K):
-    """McMurchie-Davidson E^{ij}_0 (vectorised), the 1-D overlap factor of x_A^i x_B^j."""
-    E = {(0, 0): [K]}
-    inv2p = 0.5 / p
-    def get(ii, jj):
-        if (ii, jj) in E:
-            return E[(ii, jj)]
-        if ii > 0:
-            prev = get(ii - 1, jj)
-            shift = xpa
-        else:
-            prev = get(ii, jj - 1)
-            shift = xpb
-        n = len(prev)
-        cur = []
-        for t in range(n + 1):
-            v = 0.0
-            if t - 1 >= 0:
-                v = v + inv2p * prev[t - 1]
-            if t < n:
-                v = v + shift * prev[t]
-            if t + 1 < n:
-                v = v + (t + 1) * prev[t + 1]
-            cur.append(v)
-        E[(ii, jj)] = cur
-        return cur
-    return get(i, j)[0]
-
-
 def overlap_matrix(b: Basis) -> np.ndarray:
     """Analytic AO overlap (the OVMXSF integrals, ovmxsf.f:3-183, summed over contractions)."""
-    funcs = b.functions()
-    N = len(funcs)
-    # index of first function for each (atom, l): functions ordered l, m, con
-    start = {}
-    k = 0
-    for a in range(b.natom):
-        for l in range(3):
-            start[(a, l)] = k
-            k += NCART[l] * int(b.ncon[a, l])
+    N = b.nbas
     S = np.zeros((N, N))
-    for A in range(b.natom):
-        aA = b.alpha[b.bare_off[A]:b.bare_off[A + 1]]
-        for B in range(b.natom):
-            aB = b.alpha[b.bare_off[B]:b.bare_off[B + 1]]
-            al = aA[:, None]
-            be = aB[None, :]
-            p = al + be
-            mu = al * be / p
-            d = b.xyz[A] - b.xyz[B]
-            P = (al[..., None] * b.xyz[A] + be[..., None] * b.xyz[B]) / p[..., None]
-            xpa = P - b.xyz[A]
-            xpb = P - b.xyz[B]
-            K = np.exp(-mu[..., None] * d ** 2)  # per dimension
-            pref = (math.pi / p) ** 1.5
-            for la in range(3):
-                na = int(b.ncon[A, la])
-                if na == 0:
-                    continue
-                ca = b.con_block(A, la)
-                for lb in range(3):
-                    nbn = int(b.ncon[B, lb])
-                    if nbn == 0:
-                        continue
-                    cb = b.con_block(B, lb)
-                    for ma, (ax, ay, az) in enumerate(CART[la]):
-                        for mb, (bx, by, bz) in enumerate(CART[lb]):
-                            sx = _md_e0(ax, bx, xpa[..., 0], xpb[..., 0], p, K[..., 0])
-                            sy = _md_e0(ay, by, xpa[..., 1], xpb[..., 1], p, K[..., 1])
-                            sz = _md_e0(az, bz, xpa[..., 2], xpb[..., 2], p, K[..., 2])
-                            prim = pref * sx * sy * sz            # (nbareA, nbareB)
-                            blk = ca @ prim @ cb.T                 # (na, nb)
-                            ia = start[(A, la)] + ma * na
-                            ib = start[(B, lb)] + mb * nbn
-                            S[ia:ia + na, ib:ib + nbn] = blk
+    xyz = np.ascontiguousarray(b.xyz, np.float64)
+    bo = np.ascontiguousarray(b.bare_off, np.int32)
+    al = np.ascontiguousarray(b.alpha, np.float64)
+    nc = np.ascontiguousarray(b.ncon, np.int32).ravel()
+    co = np.ascontiguousarray(b.con_off, np.int32).ravel()
+    cn = np.ascontiguousarray(b.con, np.float64)
+    _setup_lib().setup_overlap(b.natom, _dp(xyz), _ip(bo), _dp(al), _ip(nc), _ip(co), _dp(cn), N, _dp(S))
     return 0.5 * (S + S.T)
 
 
@@ -285,7 +258,7 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
 
 
 def becke_mesh(xyz: np.ndarray, rng: np.random.Generator, nrad: int = 60, nang: int = 194,
-               rmin: float = 5.0e-4, rmax: float = 25.0, chunk: int = 16384) -> Mesh:
+               rmin: float = 5.0e-4, rmax: float = 25.0) -> Mesh:
     """Per-atom log radial grid x rotated Fibonacci sphere, Becke (1988) fuzzy-cell weights."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     K = xyz.shape[0]
@@ -309,25 +282,9 @@ def becke_mesh(xyz: np.ndarray, rng: np.random.Generator, nrad: int = 60, nang: 
     wts = np.concatenate(wts)
     owner = np.concatenate(owner)
     if K > 1:
-        rab = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
-        np.fill_diagonal(rab, 1.0)
-        inv_rab = 1.0 / rab
-        part = np.empty_like(wts)
-        for s0 in range(0, pts.shape[0], chunk):
-            pp = pts[s0:s0 + chunk]
-            d = np.linalg.norm(pp[:, None, :] - xyz[None, :, :], axis=2)      # (n, K)
-            mu = (d[:, :, None] - d[:, None, :]) * inv_rab[None, :, :]        # (n, K, K)
-            f = mu
-            for _ in range(3):
-                f = 1.5 * f - 0.5 * f ** 3
-            s = 0.5 * (1.0 - f)
-            idx = np.arange(K)
-            s[:, idx, idx] = 1.0
-            cell = np.prod(s, axis=2)                                         # (n, K)
-            tot = cell.sum(1)
-            own = owner[s0:s0 + chunk]
-            part[s0:s0 + chunk] = cell[np.arange(pp.shape[0]), own] / tot
-        wts = wts * part
+        pts = np.ascontiguousarray(pts)
+        owner32 = np.ascontiguousarray(owner, np.int32)
+        _setup_lib().setup_becke(K, _dp(np.ascontiguousarray(xyz)), pts.shape[0], _dp(pts), _ip(owner32), _dp(wts))
     keep = wts > 0.0
     return Mesh(xyz=np.ascontiguousarray(pts[keep]), w=np.ascontiguousarray(wts[keep]))
 
