@@ -1,0 +1,128 @@
+! flosic_hip.f90 -- ISO_C_BINDING interfaces to libflosic_hip.so (see flosic_hip.h for the contract of each
+! entry point).  A Fortran host such as the FLOSIC SCF driver USEs this module; arrays are passed as plain
+! column-major REAL*8 / INTEGER*4 buffers, the context as an opaque TYPE(C_PTR), and every function returns
+! the library status (0 = success).
+module flosic_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: FLOSIC_OK = 0, FLOSIC_EINVAL = 1, FLOSIC_ELOWDIN = 2, FLOSIC_ECOUPOT = 3, &
+                                   FLOSIC_EDEVICE = 4, FLOSIC_ESTATE = 5, FLOSIC_ENOMEM = 6
+  integer(c_int32_t), parameter :: FLOSIC_XC_LSDA = 0, FLOSIC_XC_PBE = 1
+  integer(c_int32_t), parameter :: FLOSIC_PREC_FP64 = 0, FLOSIC_PREC_MIXED = 1
+
+  interface
+    integer(c_int32_t) function flosic_create(device, ctx) bind(C, name="flosic_create")
+      import :: c_int32_t, c_ptr
+      integer(c_int32_t), value :: device
+      type(c_ptr) :: ctx
+    end function
+    integer(c_int32_t) function flosic_destroy(ctx) bind(C, name="flosic_destroy")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function flosic_last_error(ctx) bind(C, name="flosic_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function flosic_set_basis(ctx, natom, xyz, bare_off, alpha, ncon, con_off, con) &
+        bind(C, name="flosic_set_basis")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: natom
+      real(c_double) :: xyz(*), alpha(*), con(*)
+      integer(c_int32_t) :: bare_off(*), ncon(*), con_off(*)
+    end function
+    integer(c_int32_t) function flosic_set_mesh(ctx, nmsh, rmsh, wmsh) bind(C, name="flosic_set_mesh")
+      import :: c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nmsh
+      real(c_double) :: rmsh(3, *), wmsh(*)
+    end function
+    integer(c_int32_t) function flosic_set_options(ctx, nspn, xc, precision) bind(C, name="flosic_set_options")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nspn, xc, precision
+    end function
+    integer(c_int32_t) function flosic_set_orbitals(ctx, ispn, nbas, nwf, nocc, coef) bind(C, name="flosic_set_orbitals")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn, nbas, nwf, nocc
+      real(c_double) :: coef(nbas, *)
+    end function
+    integer(c_int32_t) function flosic_set_fods(ctx, ispn, nfrm, bfrm) bind(C, name="flosic_set_fods")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn, nfrm
+      real(c_double) :: bfrm(3, *)
+    end function
+    integer(c_int32_t) function flosic_frmorb(ctx) bind(C, name="flosic_frmorb")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function flosic_get_fo(ctx, ispn, fmat, over, eval, filo, tmat) bind(C, name="flosic_get_fo")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: fmat(*), over(*), eval(*), filo(*), tmat(*)
+    end function
+    integer(c_int32_t) function flosic_evaluate(ctx, zsic) bind(C, name="flosic_evaluate")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: zsic
+    end function
+    integer(c_int32_t) function flosic_apotnl_sic(ctx, ispn, results, zsic) bind(C, name="flosic_apotnl_sic")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: results(13, *), zsic
+    end function
+    integer(c_int32_t) function flosic_siclagm(ctx, ispn, sic) bind(C, name="flosic_siclagm")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: sic(*)
+    end function
+    integer(c_int32_t) function flosic_sic_vab(ctx, ispn, vab) bind(C, name="flosic_sic_vab")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: vab(*)
+    end function
+    integer(c_int32_t) function flosic_fod_forces(ctx, ispn, debdax) bind(C, name="flosic_fod_forces")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: debdax(3, *)
+    end function
+    integer(c_int32_t) function flosic_fod_gradient(ctx, ispn, grad) bind(C, name="flosic_fod_gradient")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: grad(3, *)
+    end function
+    integer(c_int32_t) function flosic_get_potential(ctx, ispn, iorb, pot, vh) bind(C, name="flosic_get_potential")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn, iorb
+      real(c_double) :: pot(*), vh(*)
+    end function
+    integer(c_int32_t) function flosic_get_pairs(ctx, npair, list) bind(C, name="flosic_get_pairs")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: npair, list(*)
+    end function
+    integer(c_int32_t) function flosic_get_timings(ctx, n, ms) bind(C, name="flosic_get_timings")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      real(c_double) :: ms(*)
+    end function
+    integer(c_int32_t) function flosic_get_counts(ctx, n, counts) bind(C, name="flosic_get_counts")
+      import :: c_int32_t, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      integer(c_int64_t) :: counts(*)
+    end function
+  end interface
+end module flosic_hip

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -98,6 +99,9 @@ struct flosic_ctx {
     DevBuf<int> d_ps_f0, d_ps_stride, d_ps_ncart, d_ps_c0, d_ps_nc, d_pair_row;
     DevBuf<double> d_ps_coef;
     DevBuf<PairDev> d_pairs;
+    DevBuf<ChunkDev> d_chunks;
+    int nchunks = 0;
+    DevBuf<float> d_H32;
     DevBuf<double4> d_pairP;
     DevBuf<double> d_btab;
     // mesh
@@ -552,14 +556,27 @@ int build_pairs(flosic_ctx *c) {
     c->npair = (int)pairs.size();
     std::vector<double4> pP(c->npair);
     std::vector<int> prow(c->npair);
+    // chunks of consecutive pairs of one L, rows contiguous and zero-padded to a multiple of 16
+    std::vector<ChunkDev> chunks;
     int row = 0;
-    for (int L = 0; L <= 5; ++L) c->segL[L] = 0;
+    for (int k = 0; k < c->npair;) {
+        const int L = pairs[k].lA + pairs[k].lB;
+        const int nt = (L + 1) * (L + 2) * (L + 3) / 6;
+        const int cap = std::max(1, kChunkRows / nt);
+        int e = k;
+        while (e < c->npair && e - k < cap && pairs[e].lA + pairs[e].lB == L) ++e;
+        const int kr = ((e - k) * nt + 15) / 16 * 16;
+        chunks.push_back(ChunkDev{k, e, L, kr, row});
+        for (int i = k; i < e; ++i) {
+            pairs[i].row = row + (i - k) * nt;
+            prow[i] = pairs[i].row;
+        }
+        row += kr;
+        k = e;
+    }
+    c->nchunks = (int)chunks.size();
     for (int k = 0; k < c->npair; ++k) {
-        PairDev &p = pairs[k];
-        const int L = p.lA + p.lB;
-        p.row = row;
-        prow[k] = row;
-        row += (L + 1) * (L + 2) * (L + 3) / 6;
+        const PairDev &p = pairs[k];
         const double pe = p.a + p.b;
         pP[k] = make_double4((p.a * p.A[0] + p.b * p.B[0]) / pe, (p.a * p.A[1] + p.b * p.B[1]) / pe,
                              (p.a * p.A[2] + p.b * p.B[2]) / pe, pe);
@@ -573,6 +590,7 @@ int build_pairs(flosic_ctx *c) {
     }
     int r = 0;
     if ((r = upload(c, c->d_pairs, pairs)) || (r = upload(c, c->d_pairP, pP)) || (r = upload(c, c->d_pair_row, prow)) ||
+        (r = upload(c, c->d_chunks, chunks)) ||
         (r = upload(c, c->d_ps_f0, ps_f0)) || (r = upload(c, c->d_ps_stride, ps_stride)) ||
         (r = upload(c, c->d_ps_ncart, ps_ncart)) || (r = upload(c, c->d_ps_c0, ps_c0)) ||
         (r = upload(c, c->d_ps_nc, ps_nc)) || (r = upload(c, c->d_ps_coef, ps_coef)))
@@ -650,7 +668,7 @@ int evaluate(flosic_ctx *c) {
     const int64_t M = c->M;
     const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
     c->nF = nF;
-    c->ldh = (nF + 15) / 16 * 16;
+    c->ldh = (nF + 63) / 64 * 64;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     // 1. Fermi orbitals, Lowdin, FLO coefficients (host FP64)
@@ -685,20 +703,32 @@ int evaluate(flosic_ctx *c) {
     ALLOC(c, c->d_dps, (size_t)nps * 6 * nF);
     launch_prim_coef(nps, c->d_ps_f0.p, c->d_ps_stride.p, c->d_ps_ncart.p, c->d_ps_c0.p, c->d_ps_nc.p, c->d_ps_coef.p,
                      c->d_cflo.p, nF, N, c->d_dps.p, c->stream);
-    {
+    {   // padding rows between chunks and padding columns must read as zero
         const size_t hcnt = (size_t)c->nrows * c->ldh;
-        const bool fresh = c->d_H.n < hcnt || !c->d_H.p;
         ALLOC(c, c->d_H, hcnt);
-        if (fresh) HIPCHK(c, hipMemsetAsync(c->d_H.p, 0, c->d_H.n * sizeof(double), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_H.p, 0, hcnt * sizeof(double), c->stream));
     }
     launch_hermite(c->npair, c->d_pairs.p, c->d_dps.p, nF, c->d_H.p, c->ldh, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     // 4. Coulomb potentials of all FLOs
     ALLOC(c, c->d_VH, (size_t)nF * M);
-    launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
-                   c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-    HIPCHK(c, hipGetLastError());
+    {
+        const char *alg = getenv("FLOSIC_COULOMB");
+        if (alg && !strcmp(alg, "valu")) {  // per-thread VALU form, FP64 (kept for A/B measurements)
+            launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
+                           c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+        } else if (c->prec == FLOSIC_PREC_MIXED) {
+            const size_t hcnt = (size_t)c->nrows * c->ldh;
+            ALLOC(c, c->d_H32, hcnt);
+            launch_convert_f32(c->d_H.p, c->d_H32.p, (int64_t)hcnt, c->stream);
+            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H32.p, c->ldh, c->d_x.p, c->d_y.p,
+                                       c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+        } else {
+            launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p,
+                                        c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+        }
+    }
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     // 5. XC, total potential, energies
     const int nblk = (int)((M + 255) / 256);

@@ -37,6 +37,13 @@ struct PairDev {
     double sym;       // 2 for psA != psB, else 1
 };
 
+// chunk of consecutive pairs of one order L whose Hermite rows [hrow0, hrow0 + kr) are contiguous and
+// zero-padded to kr (a multiple of 16, at most kChunkRows)
+constexpr int kChunkRows = 48;
+struct ChunkDev {
+    int pb, pe, L, kr, hrow0;
+};
+
 // per-phase launch helpers (stream-ordered)
 void boys_table(double *h_tab);  // host: fills kBoysRows x kBoysCols
 
@@ -55,6 +62,15 @@ void launch_hermite(int npair, const PairDev *pairs, const double *dps, int nF, 
 void launch_coulomb(const int *segL /*6 offsets*/, const double4 *pairP, const int *pairRow, const double *H,
                     int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
                     int nF, double *V, int64_t ldv, hipStream_t s);
+
+// MFMA form: R_tuv of a chunk staged in LDS once per (point, pair) and contracted with the chunk's H rows
+// for 64 FLO columns per block.  T = double (FP64 MFMA) or float (FP32 MFMA, FP64 sums per chunk).
+template <typename T>
+void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
+                         const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
+                         double *V, int64_t ldv, hipStream_t s);
+
+void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s);
 
 // XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)
 void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w,

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "flosic_internal.h"
 
 namespace flosic {
@@ -380,6 +382,164 @@ void launch_coulomb(const int *segL, const double4 *pairP, const int *pairRow, c
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)((nchunk + 3) / 4)), blk(256);
     hipLaunchKernelGGL((k_coulomb<CH>), grid, blk, 0, s, segL[0], segL[1], segL[2], segL[3], segL[4], segL[5], pairP,
                        pairRow, H, ldh, x, y, z, M, btab, nF, V, ldv);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// MFMA Coulomb.  Block = 4 wavefronts, tile of TP points x 64 FLO columns.  Per chunk of pairs:
+//   1. R_tuv(P - r_k) for every (pair, point) of the chunk into LDS (computed once, FP64 Boys/recursion)
+//   2. the chunk's H rows into LDS
+//   3. V_tile += R^T H with 16x16x4 MFMA (FP64, or FP32 with the chunk partial added to FP64 sums)
+// Fragment layout of v_mfma_*_16x16x4: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// D[4*(lane/16) + r][lane%16] in accumulator register r.
+// ---------------------------------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaTraits;
+template <> struct MfmaTraits<double> {
+    using acc = v4d;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaTraits<float> {
+    using acc = v4f;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <int L, typename T, int TP, int LDR>
+__device__ __forceinline__ void chunk_R(const ChunkDev &ch, const double4 *__restrict__ pP, const double *__restrict__ X,
+                                        const double *__restrict__ Y, const double *__restrict__ Z, int64_t k0,
+                                        int64_t M, const double *__restrict__ tab, T *Rl) {
+    constexpr int nt = ntuv(L);
+    const int np = ch.pe - ch.pb;
+    for (int task = threadIdx.x; task < np * TP; task += 256) {
+        const int pl = task / TP, pt = task - pl * TP;
+        const int64_t k = k0 + pt;
+        double R[nt];
+        if (k < M) {
+            const double4 P = pP[ch.pb + pl];
+            hermite_R<L>(P.w, P.x - X[k], P.y - Y[k], P.z - Z[k], tab, R);
+        } else {
+#pragma unroll
+            for (int t = 0; t < nt; ++t) R[t] = 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < nt; ++t) Rl[(pl * nt + t) * LDR + pt] = (T)R[t];
+    }
+    for (int e = threadIdx.x; e < (ch.kr - np * nt) * TP; e += 256) {
+        const int r = np * nt + e / TP, pt = e % TP;
+        Rl[r * LDR + pt] = (T)0;
+    }
+}
+
+template <typename T, int TP>
+__global__ void __launch_bounds__(256) k_coulomb_mfma(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                       const double4 *__restrict__ pP, const T *__restrict__ H, int ldh,
+                                                       const double *__restrict__ X, const double *__restrict__ Y,
+                                                       const double *__restrict__ Z, int64_t M,
+                                                       const double *__restrict__ tab, int nF, double *__restrict__ V,
+                                                       int64_t ldv) {
+    constexpr int NFB = 64, NCB = NFB / 16, RBW = TP / 64;  // row blocks of 16 points per wave
+    constexpr int LDR = TP + 4, LDH = NFB + 4;
+    __shared__ T Rl[kChunkRows * LDR];
+    __shared__ T Hl[kChunkRows * LDH];
+    using Tr = MfmaTraits<T>;
+    using acc_t = typename Tr::acc;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t k0 = (int64_t)blockIdx.x * TP;
+    const int f0 = blockIdx.y * NFB;
+    acc_t acc[RBW][NCB];
+    double sum[RBW][NCB][4];
+#pragma unroll
+    for (int i = 0; i < RBW; ++i)
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[i][j][r] = (T)0;
+                sum[i][j][r] = 0.0;
+            }
+        }
+    const int lr = lane >> 4, lc = lane & 15;
+    for (int c = 0; c < nchunks; ++c) {
+        const ChunkDev ch = chunks[c];
+        switch (ch.L) {
+            case 0: chunk_R<0, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+            case 1: chunk_R<1, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+            case 2: chunk_R<2, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+            case 3: chunk_R<3, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+            default: chunk_R<4, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+        }
+        for (int e = threadIdx.x; e < ch.kr * NFB; e += 256) {
+            const int r = e / NFB, f = e - r * NFB;
+            Hl[r * LDH + f] = H[(int64_t)(ch.hrow0 + r) * ldh + f0 + f];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < ch.kr; ks += 4) {
+            T a[RBW], b[NCB];
+#pragma unroll
+            for (int i = 0; i < RBW; ++i) a[i] = Rl[(ks + lr) * LDR + (wave * RBW + i) * 16 + lc];
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) b[j] = Hl[(ks + lr) * LDH + j * 16 + lc];
+#pragma unroll
+            for (int i = 0; i < RBW; ++i)
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
+        }
+        if constexpr (sizeof(T) == 4) {  // FP64 sum of each chunk's FP32 partial
+#pragma unroll
+            for (int i = 0; i < RBW; ++i)
+#pragma unroll
+                for (int j = 0; j < NCB; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        sum[i][j][r] += (double)acc[i][j][r];
+                        acc[i][j][r] = 0.0f;
+                    }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < RBW; ++i)
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) {
+            const int f = f0 + j * 16 + lc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t k = k0 + (wave * RBW + i) * 16 + 4 * lr + r;
+                const double v = sizeof(T) == 4 ? sum[i][j][r] : (double)acc[i][j][r];
+                if (f < nF && k < M) V[(int64_t)f * ldv + k] = v;
+            }
+        }
+}
+
+template <typename T>
+void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
+                         const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
+                         double *V, int64_t ldv, hipStream_t s) {
+    constexpr int TP = 128;
+    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)((nF + 63) / 64)), blk(256);
+    hipLaunchKernelGGL((k_coulomb_mfma<T, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab, nF,
+                       V, ldv);
+}
+template void launch_coulomb_mfma<double>(const ChunkDev *, int, const double4 *, const double *, int, const double *,
+                                          const double *, const double *, int64_t, const double *, int, double *,
+                                          int64_t, hipStream_t);
+template void launch_coulomb_mfma<float>(const ChunkDev *, int, const double4 *, const float *, int, const double *,
+                                         const double *, const double *, int64_t, const double *, int, double *,
+                                         int64_t, hipStream_t);
+
+__global__ void k_convert_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
 }
 
 // ---------------------------------------------------------------------------------------------------

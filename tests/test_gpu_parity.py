@@ -186,3 +186,30 @@ def test_k16_repeat_is_deterministic(k16):
     sysm, ctx, got = k16
     e2 = ctx.evaluate()
     assert e2 == got["esic"]
+
+
+def test_mixed_precision_within_tolerance(small_case):
+    """FP32-MFMA Coulomb with FP64 chunk sums: |dE_SIC| < 1e-6 Ha, forces within 1e-5 Ha/bohr of the FP64 oracle."""
+    sysm, ref, ctx, got = small_case
+    F = _flosic()
+    m = F.Flosic(0)
+    m.load_system(sysm, F.PREC_MIXED)
+    e = m.evaluate()
+    assert abs(e - ref["esic"]) < 1e-6
+    for s in (1, 2):
+        assert np.abs(m.fod_forces(s) - ref["spins"][s - 1]["debdax"]).max() < 1e-5
+        _close(m.sic_vab(s), ref["spins"][s - 1]["vab"], 1e-5, 1e-8)
+
+
+def test_valu_coulomb_form_matches(small_case, monkeypatch):
+    """The per-thread VALU Coulomb form gives the same potentials as the MFMA form (both FP64)."""
+    sysm, ref, ctx, got = small_case
+    F = _flosic()
+    monkeypatch.setenv("FLOSIC_COULOMB", "valu")
+    v = F.Flosic(0)
+    v.load_system(sysm)
+    e = v.evaluate()
+    assert abs(e - ref["esic"]) < 1e-10
+    for s in (1, 2):
+        pot, vh = v.potential(s, 1)
+        _close(vh, ref["spins"][s - 1]["vh"][0], 1e-11)
