@@ -21,6 +21,34 @@ def command(out: str = LIB) -> list:
             "-L" + os.path.join(ROCM, "lib"), "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", out]
 
 
+FORTRAN_EXE = os.path.join(HERE, "flosic_sic_drive")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran host driver (include/flosic_hip.f90 + fortran/*.f90) linked against libflosic_hip.so."""
+    fc = os.path.join(ROCM, "bin", "amdflang")
+    mod = os.path.join(HERE, "_build", "fmod")
+    os.makedirs(mod, exist_ok=True)
+    inc = os.path.join(os.path.dirname(HERE), "include", "flosic_hip.f90")
+    steps = [
+        [fc, "-O2", "-J" + mod, "-c", inc, "-o", os.path.join(mod, "flosic_hip.o")],
+        [fc, "-O2", "-J" + mod, "-I" + mod, "-c", os.path.join(HERE, "fortran", "flosic_sic_block.f90"), "-o",
+         os.path.join(mod, "flosic_sic_block.o")],
+        [fc, "-O2", "-I" + mod, os.path.join(HERE, "fortran", "flosic_sic_drive.f90"),
+         os.path.join(mod, "flosic_sic_block.o"), os.path.join(mod, "flosic_hip.o"), "-L" + HERE, "-lflosic_hip",
+         "-Wl,-rpath,$ORIGIN", "-o", FORTRAN_EXE + ".tmp"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd))
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("amdflang failed building the Fortran host driver")
+    os.replace(FORTRAN_EXE + ".tmp", FORTRAN_EXE)
+    return FORTRAN_EXE
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = command(tmp)
@@ -36,3 +64,5 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    if os.path.exists(os.path.join(ROCM, "bin", "amdflang")):
+        print(build_fortran(verbose=True))

@@ -1,0 +1,69 @@
+"""The Fortran host (ISO_C_BINDING module include/flosic_hip.f90 + the SIC-block shim) drives the library
+end to end; results match the oracle."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from publicrelease_2020_amd import system as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "publicrelease_2020_amd", "flosic_sic_drive")
+
+
+def write_input(path, sysm, precision=0):
+    b = sysm.basis
+    nwf = [c.shape[1] for c in sysm.coef]
+    with open(path, "wb") as f:
+        np.array([b.natom, b.alpha.size, b.con.size], np.int32).tofile(f)
+        np.array([sysm.mesh.npts], np.int64).tofile(f)
+        np.array([sysm.nspn, sysm.xc, precision, b.nbas, nwf[0], nwf[1], sysm.nocc[0], sysm.nocc[1]], np.int32).tofile(f)
+        np.ascontiguousarray(b.xyz, np.float64).tofile(f)
+        np.ascontiguousarray(b.bare_off, np.int32).tofile(f)
+        np.ascontiguousarray(b.alpha, np.float64).tofile(f)
+        np.ascontiguousarray(b.ncon, np.int32).ravel().tofile(f)
+        np.ascontiguousarray(b.con_off, np.int32).ravel().tofile(f)
+        np.ascontiguousarray(b.con, np.float64).tofile(f)
+        np.ascontiguousarray(sysm.mesh.xyz, np.float64).tofile(f)
+        np.ascontiguousarray(sysm.mesh.w, np.float64).tofile(f)
+        for s in range(2):
+            np.asfortranarray(sysm.coef[s], np.float64).ravel(order="F").tofile(f)
+            np.ascontiguousarray(sysm.fod[s], np.float64).tofile(f)
+
+
+def test_fortran_driver_built():
+    if not os.path.exists("/opt/rocm/bin/amdflang"):
+        pytest.skip("amdflang not present")
+    assert os.path.exists(EXE)
+
+
+@pytest.mark.gpu
+def test_fortran_host_matches_oracle():
+    from oracle import oracle as O
+    sysm = S.make_synthetic(3, 3, 7, nrad=24, nang=50, box=5.0)
+    ref = O.evaluate(sysm)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_input(fin, sysm)
+        subprocess.run([EXE, fin, fout], check=True, timeout=120)
+        raw = np.fromfile(fout, np.float64)
+    n = sysm.nocc
+    nwf = [c.shape[1] for c in sysm.coef]
+    zsic = raw[0]
+    o = 1
+    res = []
+    for s in range(2):
+        res.append(raw[o:o + 13 * n[s]].reshape(n[s], 13)); o += 13 * n[s]
+    sic = []
+    for s in range(2):
+        sic.append(raw[o:o + nwf[s] * n[s]].reshape(n[s], nwf[s]).T); o += nwf[s] * n[s]
+    deb = []
+    for s in range(2):
+        deb.append(raw[o:o + 3 * n[s]].reshape(n[s], 3)); o += 3 * n[s]
+    assert abs(zsic - ref["esic"]) < 1e-10
+    for s in range(2):
+        assert np.abs(res[s] - ref["spins"][s]["results"]).max() < 1e-10 * np.abs(ref["spins"][s]["results"]).max() + 1e-12
+        assert np.abs(sic[s] - ref["spins"][s]["sic"]).max() < 1e-10 * np.abs(ref["spins"][s]["sic"]).max() + 1e-12
+        assert np.abs(deb[s] - ref["spins"][s]["debdax"]).max() < 1e-9
