@@ -9,7 +9,8 @@ module flosic_hip
   integer(c_int32_t), parameter :: FLOSIC_OK = 0, FLOSIC_EINVAL = 1, FLOSIC_ELOWDIN = 2, FLOSIC_ECOUPOT = 3, &
                                    FLOSIC_EDEVICE = 4, FLOSIC_ESTATE = 5, FLOSIC_ENOMEM = 6
   integer(c_int32_t), parameter :: FLOSIC_XC_LSDA = 0, FLOSIC_XC_PBE = 1
-  integer(c_int32_t), parameter :: FLOSIC_PREC_FP64 = 0, FLOSIC_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: FLOSIC_PREC_FP64 = 0, FLOSIC_PREC_MIXED = 1, FLOSIC_PREC_BF16X3 = 2, &
+                                   FLOSIC_PREC_BF16X2 = 3
 
   interface
     integer(c_int32_t) function flosic_create(device, ctx) bind(C, name="flosic_create")

@@ -11,7 +11,7 @@
  * Typical sequence (mirrors main.F:577-775):
  *   flosic_create -> flosic_set_basis -> flosic_set_mesh -> flosic_set_options
  *   per SCF/FOD step: flosic_set_orbitals(1|2) -> flosic_set_fods(1|2) -> flosic_evaluate
- *   then flosic_get_results / flosic_get_sic / flosic_get_vab / flosic_get_fod_forces / flosic_get_fod_gradient
+ *   then flosic_apotnl_sic / flosic_siclagm / flosic_sic_vab / flosic_fod_forces / flosic_fod_gradient
  */
 #ifndef FLOSIC_HIP_H
 #define FLOSIC_HIP_H
@@ -37,9 +37,12 @@ typedef struct flosic_ctx flosic_ctx;
 #define FLOSIC_XC_LSDA 0 /* LDA-PW91*LDA-PW91 : PW91EX at s=u=v=0 + PW91LC */
 #define FLOSIC_XC_PBE 1  /* GGA-PBE*GGA-PBE   : PBEEX + PBECOR */
 
-/* arithmetic of the mesh contractions */
-#define FLOSIC_PREC_FP64 0  /* FP64 throughout */
-#define FLOSIC_PREC_MIXED 1 /* FP32 mesh contractions, FP64 Lowdin, Boys, energies and block sums */
+/* arithmetic of the Coulomb contraction (the dominant mesh contraction); Lowdin, Boys functions, the force
+ * algebra and every energy reduction stay FP64 in all modes */
+#define FLOSIC_PREC_FP64 0   /* FP64 MFMA */
+#define FLOSIC_PREC_MIXED 1  /* FP32 MFMA, FP32 accumulation per chunk of <=64 Hermite rows, FP64 sums */
+#define FLOSIC_PREC_BF16X3 2 /* operands split into 3 bf16 pieces, 6 cross products on bf16 MFMA, FP64 sums */
+#define FLOSIC_PREC_BF16X2 3 /* operands split into 2 bf16 pieces, 3 cross products ("3-term BF16 split") */
 
 /* Context on HIP device `device` (a hipSetDevice index). */
 int32_t flosic_create(int32_t device, flosic_ctx **ctx);

@@ -565,7 +565,7 @@ int build_pairs(flosic_ctx *c) {
         const int cap = std::max(1, kChunkRows / nt);
         int e = k;
         while (e < c->npair && e - k < cap && pairs[e].lA + pairs[e].lB == L) ++e;
-        const int kr = ((e - k) * nt + 15) / 16 * 16;
+        const int kr = ((e - k) * nt + 31) / 32 * 32;
         chunks.push_back(ChunkDev{k, e, L, kr, row});
         for (int i = k; i < e; ++i) {
             pairs[i].row = row + (i - k) * nt;
@@ -718,6 +718,9 @@ int evaluate(flosic_ctx *c) {
         if (alg && !strcmp(alg, "valu")) {  // per-thread VALU form, FP64 (kept for A/B measurements)
             launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
                            c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+        } else if (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) {
+            launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H.p,
+                                c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else if (c->prec == FLOSIC_PREC_MIXED) {
             const size_t hcnt = (size_t)c->nrows * c->ldh;
             ALLOC(c, c->d_H32, hcnt);
@@ -1004,7 +1007,7 @@ int32_t flosic_set_options(flosic_ctx *c, int32_t nspn, int32_t xc, int32_t prec
     if (!c) return FLOSIC_EINVAL;
     if (nspn != 1 && nspn != 2) return fail(c, FLOSIC_EINVAL, "nspn must be 1 or 2");
     if (xc != FLOSIC_XC_LSDA && xc != FLOSIC_XC_PBE) return fail(c, FLOSIC_EINVAL, "unknown functional %d", xc);
-    if (precision != FLOSIC_PREC_FP64 && precision != FLOSIC_PREC_MIXED)
+    if (precision < FLOSIC_PREC_FP64 || precision > FLOSIC_PREC_BF16X2)
         return fail(c, FLOSIC_EINVAL, "unknown precision %d", precision);
     if (xc != c->xc) c->planes_ok = false;
     c->nspn = nspn;

@@ -38,8 +38,8 @@ struct PairDev {
 };
 
 // chunk of consecutive pairs of one order L whose Hermite rows [hrow0, hrow0 + kr) are contiguous and
-// zero-padded to kr (a multiple of 16, at most kChunkRows)
-constexpr int kChunkRows = 48;
+// zero-padded to kr (a multiple of 32, at most kChunkRows)
+constexpr int kChunkRows = 64;
 struct ChunkDev {
     int pb, pe, L, kr, hrow0;
 };
@@ -71,6 +71,12 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pai
                          double *V, int64_t ldv, hipStream_t s);
 
 void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s);
+
+// split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,
+// FP32 accumulation inside a chunk, FP64 sums across chunks; H is read in FP64 and split while staged
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const double *H,
+                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
+                         int nF, double *V, int64_t ldv, hipStream_t s);
 
 // XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)
 void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w,

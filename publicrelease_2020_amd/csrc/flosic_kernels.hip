@@ -443,6 +443,7 @@ __global__ void __launch_bounds__(256) k_coulomb_mfma(const ChunkDev *__restrict
                                                        const double *__restrict__ tab, int nF, double *__restrict__ V,
                                                        int64_t ldv) {
     constexpr int NFB = 64, NCB = NFB / 16, RBW = TP / 64;  // row blocks of 16 points per wave
+    static_assert(RBW >= 1, "tile must hold 64 points per wavefront group");
     constexpr int LDR = TP + 4, LDH = NFB + 4;
     __shared__ T Rl[kChunkRows * LDR];
     __shared__ T Hl[kChunkRows * LDH];
@@ -521,7 +522,7 @@ template <typename T>
 void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
                          const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
                          double *V, int64_t ldv, hipStream_t s) {
-    constexpr int TP = 128;
+    constexpr int TP = sizeof(T) == 8 ? 64 : 128;  // LDS: 70 KB (FP64) / 51 KB (FP32) per block
     const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)((nF + 63) / 64)), blk(256);
     hipLaunchKernelGGL((k_coulomb_mfma<T, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab, nF,
                        V, ldv);
@@ -532,6 +533,160 @@ template void launch_coulomb_mfma<double>(const ChunkDev *, int, const double4 *
 template void launch_coulomb_mfma<float>(const ChunkDev *, int, const double4 *, const float *, int, const double *,
                                          const double *, const double *, int64_t, const double *, int, double *,
                                          int64_t, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------------
+// Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces (each the rounded remainder of the previous), so the
+// product x*y is carried by x1y1 + x1y2 + x2y1 (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error,
+// every term on the 16x16x32 bf16 MFMA.  Operand fragments are 8 consecutive Hermite rows of one point /
+// one FLO column, so both tiles are stored row-fastest ([point][row], [column][row]) for 16-byte reads.
+// ---------------------------------------------------------------------------------------------------
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int NP>
+__device__ __forceinline__ void split_bf16(double x, __bf16 *out, int stride) {
+    double r = x;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const __bf16 b = (__bf16)(float)r;
+        out[p * stride] = b;
+        r -= (double)(float)b;
+    }
+}
+
+template <int L, int NP, int TP, int LDK>
+__device__ __forceinline__ void chunk_R_bf16(const ChunkDev &ch, const double4 *__restrict__ pP,
+                                             const double *__restrict__ X, const double *__restrict__ Y,
+                                             const double *__restrict__ Z, int64_t k0, int64_t M,
+                                             const double *__restrict__ tab, __bf16 *Rb) {
+    constexpr int nt = ntuv(L);
+    constexpr int PS = TP * LDK;  // piece stride
+    const int np = ch.pe - ch.pb;
+    for (int task = threadIdx.x; task < np * TP; task += 256) {
+        const int pl = task / TP, pt = task - pl * TP;
+        const int64_t k = k0 + pt;
+        double R[nt];
+        if (k < M) {
+            const double4 P = pP[ch.pb + pl];
+            hermite_R<L>(P.w, P.x - X[k], P.y - Y[k], P.z - Z[k], tab, R);
+        } else {
+#pragma unroll
+            for (int t = 0; t < nt; ++t) R[t] = 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < nt; ++t) split_bf16<NP>(R[t], Rb + pt * LDK + pl * nt + t, PS);
+    }
+    const int r0 = np * nt;
+    for (int e = threadIdx.x; e < (ch.kr - r0) * TP; e += 256) {
+        const int pt = e / (ch.kr - r0), r = r0 + e % (ch.kr - r0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) Rb[p * PS + pt * LDK + r] = (__bf16)0.0f;
+    }
+}
+
+template <int NP, int TP>
+__global__ void __launch_bounds__(256) k_coulomb_bf16(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                       const double4 *__restrict__ pP, const double *__restrict__ H,
+                                                       int ldh, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+                                                       int64_t M, const double *__restrict__ tab, int nF,
+                                                       double *__restrict__ V, int64_t ldv) {
+    constexpr int NFB = 64, NCB = NFB / 16, RBW = TP / 64;
+    constexpr int LDK = kChunkRows + 8;  // 144-byte rows: conflict-free 16-byte fragment reads
+    __shared__ __attribute__((aligned(16))) __bf16 Rb[NP * TP * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Hb[NP * NFB * LDK];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int64_t k0 = (int64_t)blockIdx.x * TP;
+    const int f0 = blockIdx.y * NFB;
+    v4f acc[RBW][NCB];
+    double sum[RBW][NCB][4];
+#pragma unroll
+    for (int i = 0; i < RBW; ++i)
+#pragma unroll
+        for (int j = 0; j < NCB; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[i][j][r] = 0.0f;
+                sum[i][j][r] = 0.0;
+            }
+    for (int c = 0; c < nchunks; ++c) {
+        const ChunkDev ch = chunks[c];
+        switch (ch.L) {
+            case 0: chunk_R_bf16<0, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+            case 1: chunk_R_bf16<1, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+            case 2: chunk_R_bf16<2, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+            case 3: chunk_R_bf16<3, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+            default: chunk_R_bf16<4, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+        }
+        for (int e = threadIdx.x; e < ch.kr * NFB; e += 256) {
+            const int r = e / NFB, f = e - r * NFB;
+            split_bf16<NP>(H[(int64_t)(ch.hrow0 + r) * ldh + f0 + f], Hb + f * LDK + r, NFB * LDK);
+        }
+        __syncthreads();
+        for (int ks = 0; ks < ch.kr; ks += 32) {
+            v8bf a[NP][RBW], b[NP][NCB];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                for (int i = 0; i < RBW; ++i)
+                    a[p][i] = *(const v8bf *)(Rb + p * TP * LDK + ((wave * RBW + i) * 16 + lc) * LDK + ks + 8 * lr);
+#pragma unroll
+                for (int j = 0; j < NCB; ++j)
+                    b[p][j] = *(const v8bf *)(Hb + p * NFB * LDK + (j * 16 + lc) * LDK + ks + 8 * lr);
+            }
+#pragma unroll
+            for (int i = 0; i < RBW; ++i)
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) {
+                    v4f cc = acc[i][j];
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], cc, 0, 0, 0);
+                    if constexpr (NP == 3) {
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], cc, 0, 0, 0);
+                    }
+                    acc[i][j] = cc;
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < RBW; ++i)
+#pragma unroll
+            for (int j = 0; j < NCB; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sum[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j][r] = 0.0f;
+                }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < RBW; ++i)
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) {
+            const int f = f0 + j * 16 + lc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t k = k0 + (wave * RBW + i) * 16 + 4 * lr + r;
+                if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[i][j][r];
+            }
+        }
+}
+
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const double *H,
+                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
+                         int nF, double *V, int64_t ldv, hipStream_t s) {
+    constexpr int TP = 64;
+    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)((nF + 63) / 64)), blk(256);
+    if (npieces == 2)
+        hipLaunchKernelGGL((k_coulomb_bf16<2, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab,
+                           nF, V, ldv);
+    else
+        hipLaunchKernelGGL((k_coulomb_bf16<3, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab,
+                           nF, V, ldv);
+}
 
 __global__ void k_convert_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libflosic_hip.so")
 
 OK, EINVAL, ELOWDIN, ECOUPOT, EDEVICE, ESTATE, ENOMEM = range(7)
 XC_LSDA, XC_PBE = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_BF16X3, PREC_BF16X2 = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -188,12 +188,14 @@ def test_k16_repeat_is_deterministic(k16):
     assert e2 == got["esic"]
 
 
-def test_mixed_precision_within_tolerance(small_case):
-    """FP32-MFMA Coulomb with FP64 chunk sums: |dE_SIC| < 1e-6 Ha, forces within 1e-5 Ha/bohr of the FP64 oracle."""
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_precision_within_tolerance(small_case, prec):
+    """FP32-MFMA (1) and 3-piece split-BF16 (2) Coulomb with FP64 chunk sums: |dE_SIC| < 1e-6 Ha and FOD forces
+    within 1e-5 Ha/bohr of the FP64 oracle."""
     sysm, ref, ctx, got = small_case
     F = _flosic()
     m = F.Flosic(0)
-    m.load_system(sysm, F.PREC_MIXED)
+    m.load_system(sysm, prec)
     e = m.evaluate()
     assert abs(e - ref["esic"]) < 1e-6
     for s in (1, 2):
@@ -213,3 +215,13 @@ def test_valu_coulomb_form_matches(small_case, monkeypatch):
     for s in (1, 2):
         pot, vh = v.potential(s, 1)
         _close(vh, ref["spins"][s - 1]["vh"][0], 1e-11)
+
+
+def test_bf16x2_error_is_bounded(small_case):
+    """The 2-piece split (3 products, ~2^-16 per product) stays within 1e-4 Ha of the FP64 oracle."""
+    sysm, ref, ctx, got = small_case
+    F = _flosic()
+    m = F.Flosic(0)
+    m.load_system(sysm, F.PREC_BF16X2)
+    e = m.evaluate()
+    assert abs(e - ref["esic"]) < 1e-4
