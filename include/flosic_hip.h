@@ -116,10 +116,14 @@ int32_t flosic_get_pairs(flosic_ctx *ctx, int32_t *npair, int32_t *list);
 
 /* Phase timings of the last flosic_evaluate in milliseconds (HIP events on the context's stream):
  * [0] FO+Lowdin  [1] orbitals on mesh  [2] Hermite pair coefficients  [3] Coulomb  [4] XC+energies
- * [5] SIC matrix  [6] V^i_ab  [7] forces  [8] total.  n <= 9 values are written. */
+ * [5] SIC matrix  [6] V^i_ab  [7] forces  [8] total  [9] the Coulomb kernel alone (inside [3]).
+ * n <= 10 values are written. */
 int32_t flosic_get_timings(flosic_ctx *ctx, int32_t n, double *ms);
 
-/* Counters of the last evaluation: [0] npair [1] Hermite rows [2] nmsh [3] nbas [4] nflo (both spins). */
+/* Counters of the last evaluation: [0] screened primitive shell pairs (the flosic_get_pairs list) [1] Hermite
+ * rows (shell pairs that share both primitive exponents and centres are one group with one row block)
+ * [2] nmsh [3] nbas [4] nflo (both spins) [5] exponent-pair groups [6] Hermite rows as the Coulomb kernel of
+ * the current precision streams them (chunks zero-extended to its MFMA K step).  n <= 7 values are written. */
 int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
 
 #ifdef __cplusplus

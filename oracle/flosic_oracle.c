@@ -31,6 +31,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -926,11 +927,28 @@ void oracle_fod_forces(int n, const double *fmat, const double *U, const double 
 /* full evaluation                                                                                   */
 /* ------------------------------------------------------------------------------------------------ */
 /* status: 0 ok, 1 bad input, 2 Lowdin eigenvalue < 1e-8, 3 Coulomb potential zero */
+/* wall time of the last oracle_eval split into work that does not grow with the mesh (Fermi orbitals and
+ * Lowdin, the Hermite expansion of every pair for every FLO, the force algebra) and work proportional to the
+ * number of mesh points (everything else); used to time the oracle on a sample of the mesh */
+static double g_stage_fixed, g_stage_mesh;
+static double wall_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+void oracle_stage_times(double *out) {
+    out[0] = g_stage_fixed;
+    out[1] = g_stage_mesh;
+}
+
 int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double *w, int nspn, int xc,
                 ob_spin *spins, double *esic, int nthreads) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
+    const double t_start = wall_s();
+    double t_mark = t_start, t_fixed = 0.0;
+    g_stage_fixed = g_stage_mesh = 0.0;
     boys_init(); tuv_init();
     int nbas = basis_nbas(b);
     int ntot = 0;
@@ -948,6 +966,7 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
                         spins[s].tmat, cflo + (size_t)off[s] * nbas);
         if (st) { free(cflo); free(sinvh[0]); free(sinvh[1]); return st; }
     }
+    t_fixed += wall_s() - t_mark;
     /* stage B: orbital fields on the mesh (value, gradient, Hessian) */
     int nd = xc == 1 ? 10 : 1;
     double *chi = malloc(sizeof(double) * (size_t)nd * npts * nbas);
@@ -970,6 +989,7 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
     double *H = calloc((size_t)npair * 35 * ntot, sizeof(double));
     int *PL = malloc(sizeof(int) * npair);
     double *PP = malloc(sizeof(double) * 4 * npair);
+    t_mark = wall_s();
 #pragma omp parallel for schedule(dynamic)
     for (int k = 0; k < npair; ++k) {
         const pshell *x = ps + pr[2 * k], *y = ps + pr[2 * k + 1];
@@ -1003,6 +1023,7 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
             pair_hermite(ax, A, ay, B, x->l, y->l, D, H + ((size_t)k * ntot + f) * 35, L);
         }
     }
+    t_fixed += wall_s() - t_mark;
     double *VH = calloc((size_t)ntot * npts, sizeof(double));
 #pragma omp parallel
     {
@@ -1152,12 +1173,16 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
         }
         free(cj);
         /* stage D: FOD forces from SIC(1:n,1:n) */
+        t_mark = wall_s();
         if (spins[s].sic && (spins[s].debdax || spins[s].grad))
             oracle_fod_forces(n, spins[s].fmat, spins[s].U, spins[s].q, sinvh[s], spins[s].tmat, spins[s].sic, nwf,
                               spins[s].debdax, spins[s].grad);
+        t_fixed += wall_s() - t_mark;
     }
     free(chi); free(phi); free(VH); free(VX); free(cflo);
     free(sinvh[0]); free(sinvh[1]);
+    g_stage_fixed = t_fixed;
+    g_stage_mesh = wall_s() - t_start - t_fixed;
     return status;
 }
 

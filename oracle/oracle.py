@@ -58,6 +58,7 @@ def lib():
         L.oracle_eigh.argtypes = [C.c_int, _dp, _dp, _dp]
         L.oracle_fod_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]
         L.oracle_xc_kernel.argtypes = [C.c_int, _dp, _dp]
+        L.oracle_stage_times.argtypes = [_dp]
         L.oracle_nbas.restype = C.c_int
         L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
         _lib = L
@@ -189,7 +190,9 @@ def evaluate(sysm, nthreads: int = 0, want_potentials: bool = False, want_vab: b
     esic = np.zeros(1)
     st = lib().oracle_eval(C.byref(h.s), M, _d(pts), _d(w), int(sysm.nspn), int(sysm.xc), spins, _d(esic),
                            int(nthreads))
-    res = dict(status=int(st), esic=float(esic[0]), spins=[])
+    st_t = np.zeros(2)
+    lib().oracle_stage_times(_d(st_t))
+    res = dict(status=int(st), esic=float(esic[0]), spins=[], stage_s=dict(fixed=float(st_t[0]), mesh=float(st_t[1])))
     for s in range(2):
         o = outs[s]
         n = o["q"].size

@@ -5,6 +5,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -70,6 +71,8 @@ struct Spin {
     std::vector<double> vab;      // n^3
     std::vector<double> debdax, grad;  // 3 x n
     DevBuf<double> d_ctil;        // nbas x nwf: FLOs then virtual KS orbitals
+    DevBuf<double> d_cocc;        // nbas x nocc: occupied KS orbitals
+    DevBuf<double> d_xfod, d_psi, d_tmat;  // basis at the FODs (4 x nbas x n), orbitals at the FODs (4 x n x n)
 };
 
 }  // namespace
@@ -93,15 +96,18 @@ struct flosic_ctx {
     int ncs = 0;
     // primitive shells and pairs
     std::vector<int> ps_atom, ps_l, ps_bare;
-    std::vector<int32_t> pair_list;  // 6 per pair, canonical order
-    int npair = 0, nrows = 0;
+    std::vector<int32_t> pair_list;  // 6 per shell pair, canonical order
+    int npair = 0;                   // screened primitive shell pairs
+    int ngroup = 0, nrows = 0;       // exponent-pair groups and their Hermite rows
     int segL[6] = {0, 0, 0, 0, 0, 0};
     DevBuf<int> d_ps_f0, d_ps_stride, d_ps_ncart, d_ps_c0, d_ps_nc, d_pair_row;
     DevBuf<double> d_ps_coef;
     DevBuf<PairDev> d_pairs;
-    DevBuf<ChunkDev> d_chunks;
-    int nchunks = 0;
+    DevBuf<ChunkDev> d_chunks, d_chunks_bf16;
+    int nchunks = 0, nchunks_bf16 = 0;
+    int64_t rows_padded = 0, rows_padded_bf16 = 0;
     DevBuf<float> d_H32;
+    DevBuf<__bf16> d_Hs;
     DevBuf<double4> d_pairP;
     DevBuf<double> d_btab;
     // mesh
@@ -117,13 +123,13 @@ struct flosic_ctx {
     Spin sp[2];
     // evaluation buffers
     int nF = 0, ldh = 0;
-    DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_vabpart, d_vab, d_lambda,
-        d_asy;
+    DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_Gpart, d_vabpart, d_vab,
+        d_lambda, d_asy;
     std::vector<double> h_energy, h_asy;
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
-    double timings[9] = {};
-    int64_t counts[5] = {};
+    double timings[10] = {};
+    int64_t counts[8] = {};
 };
 
 namespace {
@@ -157,10 +163,18 @@ int fail(flosic_ctx *c, int code, const char *fmt, ...) {
     } while (0)
 
 template <typename T>
-int upload(flosic_ctx *c, DevBuf<T> &b, const std::vector<T> &h) {
-    ALLOC(c, b, h.size());
-    if (!h.empty()) HIPCHK(c, hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+int upload(flosic_ctx *c, DevBuf<T> &b, const T *h, size_t cnt) {
+    // synchronous: the host buffer may be released as soon as this returns
+    ALLOC(c, b, cnt);
+    if (cnt) {
+        HIPCHK(c, hipMemcpyAsync(b.p, h, cnt * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return 0;
+}
+template <typename T>
+int upload(flosic_ctx *c, DevBuf<T> &b, const std::vector<T> &h) {
+    return upload(c, b, h.data(), h.size());
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -210,83 +224,170 @@ void host_basis_at(const flosic_ctx *c, const double *pt, double *out /* 4 x nba
     }
 }
 
-// symmetric eigen-decomposition, cyclic Jacobi; q ascending; U column-major, largest |component| positive
-void jacobi(int n, const double *Ain, double *q, double *U) {
-    std::vector<double> A(Ain, Ain + (size_t)n * n), V((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j) {
-                const double x = A[(size_t)i * n + j] * A[(size_t)i * n + j];
-                tot += x;
-                if (i != j) off += x;
+// symmetric eigen-decomposition: Householder reduction to tridiagonal form, then implicit QL with Wilkinson
+// shifts (Wilkinson & Reinsch); q ascending, U column-major with the largest |component| of each vector positive
+void eigh_sym(int n, const double *Ain, double *q, double *U) {
+    // z: row-major working copy, z[i*n + j]
+    std::vector<double> z(Ain, Ain + (size_t)n * n), d(n), e(n);
+    auto Z = [&](int i, int j) -> double & { return z[(size_t)i * n + j]; };
+    for (int i = n - 1; i > 0; --i) {
+        const int l = i - 1;
+        double h = 0.0;
+        if (l > 0) {
+            double scale = 0.0;
+            for (int k = 0; k <= l; ++k) scale += std::fabs(Z(i, k));
+            if (scale == 0.0) {
+                e[i] = Z(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) {
+                    Z(i, k) /= scale;
+                    h += Z(i, k) * Z(i, k);
+                }
+                double f = Z(i, l);
+                const double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+                e[i] = scale * g;
+                h -= f * g;
+                Z(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    Z(j, i) = Z(i, j) / h;
+                    double gg = 0.0;
+                    for (int k = 0; k <= j; ++k) gg += Z(j, k) * Z(i, k);
+                    for (int k = j + 1; k <= l; ++k) gg += Z(k, j) * Z(i, k);
+                    e[j] = gg / h;
+                    f += e[j] * Z(i, j);
+                }
+                const double hh = f / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    const double ff = Z(i, j);
+                    const double gg = e[j] - hh * ff;
+                    e[j] = gg;
+                    for (int k = 0; k <= j; ++k) Z(j, k) -= ff * e[k] + gg * Z(i, k);
+                }
             }
-        if (off == 0.0 || off <= 1e-30 * tot) break;
-        for (int p = 0; p < n - 1; ++p)
-            for (int r = p + 1; r < n; ++r) {
-                const double apr = A[(size_t)p * n + r];
-                if (std::fabs(apr) < 1e-300) continue;
-                const double theta = (A[(size_t)r * n + r] - A[(size_t)p * n + p]) / (2.0 * apr);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
-                for (int k = 0; k < n; ++k) {
-                    const double akp = A[(size_t)k * n + p], akr = A[(size_t)k * n + r];
-                    A[(size_t)k * n + p] = cs * akp - sn * akr;
-                    A[(size_t)k * n + r] = sn * akp + cs * akr;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double apk = A[(size_t)p * n + k], ark = A[(size_t)r * n + k];
-                    A[(size_t)p * n + k] = cs * apk - sn * ark;
-                    A[(size_t)r * n + k] = sn * apk + cs * ark;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double vkp = V[(size_t)k * n + p], vkr = V[(size_t)k * n + r];
-                    V[(size_t)k * n + p] = cs * vkp - sn * vkr;
-                    V[(size_t)k * n + r] = sn * vkp + cs * vkr;
-                }
+        } else {
+            e[i] = Z(i, l);
+        }
+        d[i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; ++i) {  // accumulate the transformations
+        if (d[i] != 0.0) {
+            for (int j = 0; j < i; ++j) {
+                double g = 0.0;
+                for (int k = 0; k < i; ++k) g += Z(i, k) * Z(k, j);
+                for (int k = 0; k < i; ++k) Z(k, j) -= g * Z(k, i);
             }
+        }
+        d[i] = Z(i, i);
+        Z(i, i) = 1.0;
+        for (int j = 0; j < i; ++j) Z(j, i) = Z(i, j) = 0.0;
+    }
+    // implicit QL on the tridiagonal (d, e), rotating the rows of v = z^T (row j = eigenvector j)
+    std::vector<double> v((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) v[(size_t)j * n + i] = Z(i, j);
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-300 || std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 200) break;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i];
+                    const double b = c * e[i];
+                    e[i + 1] = r = std::hypot(f, g);
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    double *vi = &v[(size_t)i * n], *vi1 = &v[(size_t)(i + 1) * n];
+                    for (int k = 0; k < n; ++k) {
+                        const double t = vi1[k];
+                        vi1[k] = s * vi[k] + c * t;
+                        vi[k] = c * vi[k] - s * t;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
     }
     std::vector<int> idx(n);
     for (int i = 0; i < n; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
     for (int c = 0; c < n; ++c) {
-        q[c] = A[(size_t)idx[c] * n + idx[c]];
+        const int j = idx[c];
+        q[c] = d[j];
+        const double *vj = &v[(size_t)j * n];
         int big = 0;
         for (int k = 0; k < n; ++k)
-            if (std::fabs(V[(size_t)k * n + idx[c]]) > std::fabs(V[(size_t)big * n + idx[c]])) big = k;
-        const double sg = V[(size_t)big * n + idx[c]] < 0 ? -1.0 : 1.0;
-        for (int k = 0; k < n; ++k) U[k + (size_t)n * c] = sg * V[(size_t)k * n + idx[c]];
+            if (std::fabs(vj[k]) > std::fabs(vj[big])) big = k;
+        const double sg = vj[big] < 0 ? -1.0 : 1.0;
+        for (int k = 0; k < n; ++k) U[k + (size_t)n * c] = sg * vj[k];
     }
 }
 
-// FRMORB2(0,0)+FRMIORB for one spin: FMAT, OVER, Lowdin, TMAT and the FLO AO coefficients
-int frmorb_spin(flosic_ctx *c, Spin &s, double *cflo /* nbas x n col-major */) {
+// FRMORB2(0,0)+FRMIORB for one spin: FMAT, OVER, Lowdin, TMAT and the FLO AO coefficients.  The orbitals at
+// the FODs (psi_j(a_i) and gradients: 4 products of nocc x nbas x nocc) and the FLO coefficients
+// cflo = C_occ TMAT^T run as rocBLAS products on the context stream; the n x n algebra stays on the host.
+int frmorb_spin(flosic_ctx *c, Spin &s, double *d_cflo /* device, nbas x n col-major */) {
     const int n = s.nocc, N = c->nbas;
-    s.fmat.assign((size_t)n * n * 4, 0.0);
-    std::vector<double> chi(4 * (size_t)N);
+    const size_t nn = (size_t)n * n;
+    std::vector<double> xf(4 * (size_t)N * n), chi(4 * (size_t)N);
     for (int i = 0; i < n; ++i) {
         host_basis_at(c, &s.fod[3 * i], chi.data());
-        std::vector<double> psi(4 * (size_t)n, 0.0);
         for (int K = 0; K < 4; ++K)
-            for (int j = 0; j < n; ++j) {
-                const double *cj = &s.coef[(size_t)j * N];
-                double acc = 0.0;
-                for (int mu = 0; mu < N; ++mu) acc += chi[(size_t)K * N + mu] * cj[mu];
-                psi[(size_t)K * n + j] = acc;
-            }
+            std::copy(chi.begin() + (size_t)K * N, chi.begin() + (size_t)(K + 1) * N,
+                      xf.begin() + ((size_t)K * n + i) * N);
+    }
+    int r = 0;
+    if ((r = upload(c, s.d_cocc, s.coef.data(), (size_t)N * n)) || (r = upload(c, s.d_xfod, xf))) return r;
+    ALLOC(c, s.d_psi, 4 * nn);
+    {   // Psi_K(j, i) = sum_mu C(mu, j) chi_K(mu, a_i)
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, N,
+                                                 &one, s.d_cocc.p, N, 0, s.d_xfod.p, N, (rocblas_stride)N * n, &zero,
+                                                 s.d_psi.p, n, (rocblas_stride)nn, 4));
+    }
+    std::vector<double> psiK(4 * nn);
+    HIPCHK(c, hipMemcpyAsync(psiK.data(), s.d_psi.p, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s.fmat.assign(nn * 4, 0.0);
+    for (int i = 0; i < n; ++i) {
+        auto psi = [&](int K, int j) { return psiK[(size_t)K * nn + j + (size_t)n * i]; };
         double rhi[4] = {0, 0, 0, 0};
         for (int j = 0; j < n; ++j)
-            for (int K = 0; K < 4; ++K) rhi[K] += psi[j] * psi[(size_t)K * n + j];
+            for (int K = 0; K < 4; ++K) rhi[K] += psi(0, j) * psi(K, j);
         const double srho = std::sqrt(rhi[0]);
-        for (int j = 0; j < n; ++j) s.fmat[j + (size_t)n * i] = psi[j] / srho;
+        for (int j = 0; j < n; ++j) s.fmat[j + (size_t)n * i] = psi(0, j) / srho;
         for (int K = 1; K < 4; ++K)
             for (int j = 0; j < n; ++j)
                 s.fmat[j + (size_t)n * (i + (size_t)n * K)] =
-                    psi[(size_t)K * n + j] / srho - s.fmat[j + (size_t)n * i] * rhi[K] / rhi[0];
+                    psi(K, j) / srho - s.fmat[j + (size_t)n * i] * rhi[K] / rhi[0];
     }
-    s.over.assign((size_t)n * n, 0.0);
+    s.over.assign(nn, 0.0);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             double acc = 0.0;
@@ -294,31 +395,38 @@ int frmorb_spin(flosic_ctx *c, Spin &s, double *cflo /* nbas x n col-major */) {
             s.over[j + (size_t)n * i] = acc;
         }
     s.q.assign(n, 0.0);
-    s.U.assign((size_t)n * n, 0.0);
-    jacobi(n, s.over.data(), s.q.data(), s.U.data());
+    s.U.assign(nn, 0.0);
+    eigh_sym(n, s.over.data(), s.q.data(), s.U.data());
     for (int i = 0; i < n; ++i)
         if (s.q[i] < 1.0e-8)
             return fail(c, FLOSIC_ELOWDIN, "Lowdin eigenvalue %d = %.3e < 1e-8 (coincident FODs?)", i + 1, s.q[i]);
-    s.sinvh.assign((size_t)n * n, 0.0);
-    for (int a = 0; a < n; ++a)
-        for (int b = 0; b < n; ++b) {
-            double acc = 0.0;
-            for (int k = 0; k < n; ++k) acc += s.U[a + (size_t)n * k] * s.U[b + (size_t)n * k] / std::sqrt(s.q[k]);
-            s.sinvh[a + (size_t)n * b] = acc;
+    s.sinvh.assign(nn, 0.0);
+    {
+        std::vector<double> Uq(nn);  // U_ak / sqrt(q_k)
+        for (int k = 0; k < n; ++k) {
+            const double f = 1.0 / std::sqrt(s.q[k]);
+            for (int a = 0; a < n; ++a) Uq[a + (size_t)n * k] = s.U[a + (size_t)n * k] * f;
         }
-    s.tmat.assign((size_t)n * n, 0.0);  // TMAT(i,j) = sum_k S^-1/2(i,k) T(k,j), T(k,j) = FMAT(j,k,1)
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
+        for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += Uq[a + (size_t)n * k] * s.U[b + (size_t)n * k];
+                s.sinvh[a + (size_t)n * b] = acc;
+            }
+    }
+    s.tmat.assign(nn, 0.0);  // TMAT(i,j) = sum_k S^-1/2(i,k) T(k,j), T(k,j) = FMAT(j,k,1)
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
             double acc = 0.0;
             for (int k = 0; k < n; ++k) acc += s.sinvh[i + (size_t)n * k] * s.fmat[j + (size_t)n * k];
             s.tmat[i + (size_t)n * j] = acc;
         }
-    for (int i = 0; i < n; ++i)
-        for (int mu = 0; mu < N; ++mu) {
-            double acc = 0.0;
-            for (int j = 0; j < n; ++j) acc += s.tmat[i + (size_t)n * j] * s.coef[(size_t)j * N + mu];
-            cflo[(size_t)i * N + mu] = acc;
-        }
+    if ((r = upload(c, s.d_tmat, s.tmat))) return r;
+    {   // cflo(mu, i) = sum_j C(mu, j) TMAT(i, j)
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, N, n, n, &one,
+                                 s.d_cocc.p, N, s.d_tmat.p, n, &zero, d_cflo, N));
+    }
     return 0;
 }
 
@@ -376,20 +484,30 @@ void force_formula(const Spin &s, double *deb) {
             const double qa = std::sqrt(s.q[a]), qb = std::sqrt(s.q[b]);
             H[b + (size_t)n * a] = H[b + (size_t)n * a] * (qb - qa) / (qb + qa);
         }
-    for (int x = 0; x < 3; ++x)
+    // -0.5 sum_ab (-0.5 h_ba) 2 sum_f (U_fb U_ma + U_mb U_fa) D(f,m) = 0.5 sum_a U_ma [(H^T Y) + (H Y)]_am with
+    // Y = U^T D: three n^3 products per direction instead of the n^4 double loop
+    std::vector<double> Y((size_t)n * n), Z((size_t)n * n);
+    for (int x = 0; x < 3; ++x) {
+        const double *Dx = D.data() + (size_t)x * n * n;
+        for (int m = 0; m < n; ++m)
+            for (int b = 0; b < n; ++b) {
+                double acc = 0.0;
+                for (int f = 0; f < n; ++f) acc += U(f, b) * Dx[f + (size_t)n * m];
+                Y[b + (size_t)n * m] = acc;
+            }
+        for (int m = 0; m < n; ++m)
+            for (int a = 0; a < n; ++a) {
+                double acc = 0.0;
+                for (int b = 0; b < n; ++b)
+                    acc += H[b + (size_t)n * a] * Y[b + (size_t)n * m] + H[a + (size_t)n * b] * Y[b + (size_t)n * m];
+                Z[a + (size_t)n * m] = acc;
+            }
         for (int m = 0; m < n; ++m) {
             double acc = 0.0;
-            for (int a = 0; a < n; ++a)
-                for (int b = 0; b < n; ++b) {
-                    const double h = H[b + (size_t)n * a];
-                    if (h == 0.0) continue;
-                    double t = 0.0;
-                    for (int f = 0; f < n; ++f)
-                        t += (U(f, b) * U(m, a) + U(m, b) * U(f, a)) * 2.0 * D[(size_t)x * n * n + f + (size_t)n * m];
-                    acc += -0.5 * t * h;
-                }
-            deb[x + 3 * m] -= 0.5 * acc;
+            for (int a = 0; a < n; ++a) acc += U(m, a) * Z[a + (size_t)n * m];
+            deb[x + 3 * m] += 0.5 * acc;
         }
+    }
 }
 
 // adjoint: Gbar = 2 SIC^T TMAT (= dE/dTMAT), back-propagated through TMAT = S^-1/2 T and S = T T^T
@@ -522,18 +640,53 @@ int build_pairs(flosic_ctx *c) {
         }
     }
     const int nps = (int)c->ps_atom.size();
-    // pairs ps1 <= ps2, screened by a*b/(a+b)*|A-B|^2 > CUTEXP (coupot_sic.F:293-310)
-    std::vector<PairDev> pairs;
+    // canonical shell pairs ps1 <= ps2, screened by a*b/(a+b)*|A-B|^2 > CUTEXP (coupot_sic.F:293-310)
     c->pair_list.clear();
+    auto screened = [&](int A, int B, double ai, double aj) {
+        const double dx = c->xyz[3 * A] - c->xyz[3 * B], dy = c->xyz[3 * A + 1] - c->xyz[3 * B + 1],
+                     dz = c->xyz[3 * A + 2] - c->xyz[3 * B + 2];
+        const double rarg = dx * dx + dy * dy + dz * dz;
+        const double rcalp = 1.0 / (ai + aj);
+        return ai * aj * rcalp * rarg > kCutExp;
+    };
     for (int i = 0; i < nps; ++i)
         for (int j = i; j < nps; ++j) {
             const int A = c->ps_atom[i], B = c->ps_atom[j];
-            const double ai = c->alpha[c->bare_off[A] + c->ps_bare[i]], aj = c->alpha[c->bare_off[B] + c->ps_bare[j]];
-            const double dx = c->xyz[3 * A] - c->xyz[3 * B], dy = c->xyz[3 * A + 1] - c->xyz[3 * B + 1],
-                         dz = c->xyz[3 * A + 2] - c->xyz[3 * B + 2];
-            const double rarg = dx * dx + dy * dy + dz * dz;
-            const double rcalp = 1.0 / (ai + aj);
-            if (ai * aj * rcalp * rarg > kCutExp) continue;
+            if (screened(A, B, c->alpha[c->bare_off[A] + c->ps_bare[i]], c->alpha[c->bare_off[B] + c->ps_bare[j]]))
+                continue;
+            const int32_t row[6] = {A, c->ps_l[i], c->ps_bare[i], B, c->ps_l[j], c->ps_bare[j]};
+            c->pair_list.insert(c->pair_list.end(), row, row + 6);
+        }
+    c->npair = (int)(c->pair_list.size() / 6);
+    // primitive centres (atom, bare) in atom-major order with their shells; the screening test depends on the
+    // exponents and centres only, so a group survives exactly when each of its shell pairs does
+    std::vector<int> pc_atom, pc_bare;
+    std::vector<std::array<int, 3>> pc_ps;
+    {
+        for (int a = 0; a < c->natom; ++a) {
+            const int nb = c->bare_off[a + 1] - c->bare_off[a];
+            for (int i = 0; i < nb; ++i) {
+                std::array<int, 3> ls = {-1, -1, -1};
+                bool any = false;
+                for (int k = 0; k < nps; ++k)
+                    if (c->ps_atom[k] == a && c->ps_bare[k] == i) {
+                        ls[c->ps_l[k]] = k;
+                        any = true;
+                    }
+                if (!any) continue;
+                pc_atom.push_back(a);
+                pc_bare.push_back(i);
+                pc_ps.push_back(ls);
+            }
+        }
+    }
+    const int npc = (int)pc_atom.size();
+    std::vector<PairDev> pairs;
+    for (int i = 0; i < npc; ++i)
+        for (int j = i; j < npc; ++j) {
+            const int A = pc_atom[i], B = pc_atom[j];
+            const double ai = c->alpha[c->bare_off[A] + pc_bare[i]], aj = c->alpha[c->bare_off[B] + pc_bare[j]];
+            if (screened(A, B, ai, aj)) continue;
             PairDev p{};
             for (int d = 0; d < 3; ++d) {
                 p.A[d] = c->xyz[3 * A + d];
@@ -541,56 +694,63 @@ int build_pairs(flosic_ctx *c) {
             }
             p.a = ai;
             p.b = aj;
-            p.psA = i;
-            p.psB = j;
-            p.lA = c->ps_l[i];
-            p.lB = c->ps_l[j];
+            p.lA = p.lB = 0;
+            for (int l = 0; l < 3; ++l) {
+                p.psA[l] = pc_ps[i][l];
+                p.psB[l] = pc_ps[j][l];
+                if (p.psA[l] >= 0) p.lA = l;
+                if (p.psB[l] >= 0) p.lB = l;
+            }
             p.sym = (i == j) ? 1.0 : 2.0;
             pairs.push_back(p);
-            const int32_t row[6] = {A, c->ps_l[i], c->ps_bare[i], B, c->ps_l[j], c->ps_bare[j]};
-            c->pair_list.insert(c->pair_list.end(), row, row + 6);
         }
-    // order by L for the Coulomb kernel (stable: canonical order inside each L)
+    // order by L for the Coulomb kernels (stable: canonical order inside each L); rows contiguous, unpadded
     std::stable_sort(pairs.begin(), pairs.end(),
                      [](const PairDev &x, const PairDev &y) { return x.lA + x.lB < y.lA + y.lB; });
-    c->npair = (int)pairs.size();
-    std::vector<double4> pP(c->npair);
-    std::vector<int> prow(c->npair);
-    // chunks of consecutive pairs of one L, rows contiguous and zero-padded to a multiple of 16
-    std::vector<ChunkDev> chunks;
+    c->ngroup = (int)pairs.size();
+    std::vector<double4> pP(c->ngroup);
+    std::vector<int> prow(c->ngroup);
     int row = 0;
-    for (int k = 0; k < c->npair;) {
-        const int L = pairs[k].lA + pairs[k].lB;
-        const int nt = (L + 1) * (L + 2) * (L + 3) / 6;
-        const int cap = std::max(1, kChunkRows / nt);
-        int e = k;
-        while (e < c->npair && e - k < cap && pairs[e].lA + pairs[e].lB == L) ++e;
-        const int kr = ((e - k) * nt + 31) / 32 * 32;
-        chunks.push_back(ChunkDev{k, e, L, kr, row});
-        for (int i = k; i < e; ++i) {
-            pairs[i].row = row + (i - k) * nt;
-            prow[i] = pairs[i].row;
-        }
-        row += kr;
-        k = e;
-    }
-    c->nchunks = (int)chunks.size();
-    for (int k = 0; k < c->npair; ++k) {
-        const PairDev &p = pairs[k];
+    for (int k = 0; k < c->ngroup; ++k) {
+        PairDev &p = pairs[k];
+        const int L = p.lA + p.lB;
+        p.row = prow[k] = row;
+        row += (L + 1) * (L + 2) * (L + 3) / 6;
         const double pe = p.a + p.b;
         pP[k] = make_double4((p.a * p.A[0] + p.b * p.B[0]) / pe, (p.a * p.A[1] + p.b * p.B[1]) / pe,
                              (p.a * p.A[2] + p.b * p.B[2]) / pe, pe);
     }
     c->nrows = row;
-    // segment offsets: segL[L] = first pair with lA+lB >= L
+    // chunk lists: consecutive groups of one L, at most `cap_rows` rows, zero-extended to a multiple of `quant`
+    auto make_chunks = [&](int cap_rows, int quant, int64_t &padded) {
+        std::vector<ChunkDev> out;
+        padded = 0;
+        for (int k = 0; k < c->ngroup;) {
+            const int L = pairs[k].lA + pairs[k].lB;
+            const int nt = (L + 1) * (L + 2) * (L + 3) / 6;
+            const int cap = std::max(1, cap_rows / nt);
+            int e = k;
+            while (e < c->ngroup && e - k < cap && pairs[e].lA + pairs[e].lB == L) ++e;
+            const int kr = ((e - k) * nt + quant - 1) / quant * quant;
+            out.push_back(ChunkDev{k, e, L, kr, pairs[k].row, (int)padded});
+            padded += kr;
+            k = e;
+        }
+        return out;
+    };
+    const std::vector<ChunkDev> chunks = make_chunks(kChunkRows, 4, c->rows_padded);
+    const std::vector<ChunkDev> chunks16 = make_chunks(kChunkRowsBf16, 32, c->rows_padded_bf16);
+    c->nchunks = (int)chunks.size();
+    c->nchunks_bf16 = (int)chunks16.size();
+    // segment offsets: segL[L] = first group with lA+lB >= L
     for (int L = 0; L <= 5; ++L) {
         int k = 0;
-        while (k < c->npair && pairs[k].lA + pairs[k].lB < L) ++k;
+        while (k < c->ngroup && pairs[k].lA + pairs[k].lB < L) ++k;
         c->segL[L] = k;
     }
     int r = 0;
     if ((r = upload(c, c->d_pairs, pairs)) || (r = upload(c, c->d_pairP, pP)) || (r = upload(c, c->d_pair_row, prow)) ||
-        (r = upload(c, c->d_chunks, chunks)) ||
+        (r = upload(c, c->d_chunks, chunks)) || (r = upload(c, c->d_chunks_bf16, chunks16)) ||
         (r = upload(c, c->d_ps_f0, ps_f0)) || (r = upload(c, c->d_ps_stride, ps_stride)) ||
         (r = upload(c, c->d_ps_ncart, ps_ncart)) || (r = upload(c, c->d_ps_c0, ps_c0)) ||
         (r = upload(c, c->d_ps_nc, ps_nc)) || (r = upload(c, c->d_ps_coef, ps_coef)))
@@ -643,13 +803,14 @@ int check_ready(flosic_ctx *c) {
     return 0;
 }
 
-int run_frmorb(flosic_ctx *c, std::vector<double> &cflo) {
+// Fermi orbitals of both spins; the FLO coefficients of all FLOs land in d_cflo (nbas x nF, spin 1 first)
+int run_frmorb(flosic_ctx *c) {
     const int N = c->nbas;
-    cflo.assign((size_t)N * (c->sp[0].nocc + c->sp[1].nocc), 0.0);
+    ALLOC(c, c->d_cflo, (size_t)N * (c->sp[0].nocc + c->sp[1].nocc));
     int off = 0;
     for (int s = 0; s < 2; ++s) {
         if (c->sp[s].nocc > 0) {
-            const int r = frmorb_spin(c, c->sp[s], &cflo[(size_t)off * N]);
+            const int r = frmorb_spin(c, c->sp[s], c->d_cflo.p + (size_t)off * N);
             if (r) return r;
         }
         off += c->sp[s].nocc;
@@ -668,13 +829,14 @@ int evaluate(flosic_ctx *c) {
     const int64_t M = c->M;
     const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
     c->nF = nF;
-    c->ldh = (nF + 63) / 64 * 64;
+    {
+        const int nfb = coulomb_cols(nF);
+        c->ldh = (nF + nfb - 1) / nfb * nfb;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    // 1. Fermi orbitals, Lowdin, FLO coefficients (host FP64)
-    std::vector<double> cflo;
-    if ((r = run_frmorb(c, cflo))) return r;
-    if ((r = upload(c, c->d_cflo, cflo))) return r;
+    // 1. Fermi orbitals, Lowdin, FLO coefficients (FP64; n x n algebra on the host)
+    if ((r = run_frmorb(c))) return r;
     for (int s = 0; s < 2; ++s) {
         Spin &sp = c->sp[s];
         ALLOC(c, sp.d_ctil, (size_t)N * sp.nwf);
@@ -703,12 +865,8 @@ int evaluate(flosic_ctx *c) {
     ALLOC(c, c->d_dps, (size_t)nps * 6 * nF);
     launch_prim_coef(nps, c->d_ps_f0.p, c->d_ps_stride.p, c->d_ps_ncart.p, c->d_ps_c0.p, c->d_ps_nc.p, c->d_ps_coef.p,
                      c->d_cflo.p, nF, N, c->d_dps.p, c->stream);
-    {   // padding rows between chunks and padding columns must read as zero
-        const size_t hcnt = (size_t)c->nrows * c->ldh;
-        ALLOC(c, c->d_H, hcnt);
-        HIPCHK(c, hipMemsetAsync(c->d_H.p, 0, hcnt * sizeof(double), c->stream));
-    }
-    launch_hermite(c->npair, c->d_pairs.p, c->d_dps.p, nF, c->d_H.p, c->ldh, c->stream);
+    ALLOC(c, c->d_H, (size_t)c->nrows * c->ldh);
+    launch_hermite(c->ngroup, c->d_pairs.p, c->d_dps.p, nF, c->d_H.p, c->ldh, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     // 4. Coulomb potentials of all FLOs
@@ -716,18 +874,26 @@ int evaluate(flosic_ctx *c) {
     {
         const char *alg = getenv("FLOSIC_COULOMB");
         if (alg && !strcmp(alg, "valu")) {  // per-thread VALU form, FP64 (kept for A/B measurements)
+            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
             launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
                            c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else if (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) {
-            launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H.p,
+            const int np = c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2;
+            ALLOC(c, c->d_Hs, (size_t)np * c->ldh * c->rows_padded_bf16);
+            launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, c->rows_padded_bf16, c->d_Hs.p,
+                           c->stream);
+            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
+            launch_coulomb_bf16(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_pairP.p, c->d_Hs.p, c->rows_padded_bf16,
                                 c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else if (c->prec == FLOSIC_PREC_MIXED) {
             const size_t hcnt = (size_t)c->nrows * c->ldh;
             ALLOC(c, c->d_H32, hcnt);
             launch_convert_f32(c->d_H.p, c->d_H32.p, (int64_t)hcnt, c->stream);
+            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
             launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H32.p, c->ldh, c->d_x.p, c->d_y.p,
                                        c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else {
+            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
             launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p,
                                         c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         }
@@ -746,15 +912,32 @@ int evaluate(flosic_ctx *c) {
     HIPCHK(c, hipMemcpy2DAsync(c->d_asy.p, sizeof(double), c->d_VH.p + c->kmax, (size_t)M * sizeof(double),
                                sizeof(double), nF, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
-    // 6. SIC matrix: G = Phi_0^T U, SIC_s = -Ctil_s^T G_s
+    // 6. SIC matrix: G = Phi_0^T U over point slices (split K: the product is N x nF with K = M), slice partials
+    //    summed in a fixed order; SIC_s = -Ctil_s^T G_s
     ALLOC(c, c->d_G, (size_t)N * nF);
     size_t lam_cnt = 0;
     for (int s = 0; s < 2; ++s) lam_cnt += (size_t)c->sp[s].nwf * c->sp[s].nocc;
     ALLOC(c, c->d_lambda, lam_cnt);
     {
         const double one = 1.0, zero = 0.0, mone = -1.0;
-        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF, (rocblas_int)M,
-                                 &one, c->d_planes.p, (rocblas_int)M, c->d_U.p, (rocblas_int)M, &zero, c->d_G.p, N));
+        int nsl = 64;
+        while (nsl > 1 && M / nsl < 4096) --nsl;
+        const int64_t ms = M / nsl, rest = M - ms * nsl;  // nsl slices of ms points, then the remainder
+        ALLOC(c, c->d_Gpart, (size_t)(nsl + 1) * N * nF);
+        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF,
+                                                 (rocblas_int)ms, &one, c->d_planes.p, (rocblas_int)M, ms, c->d_U.p,
+                                                 (rocblas_int)M, ms, &zero, c->d_Gpart.p, N, (rocblas_stride)N * nF,
+                                                 nsl));
+        int nparts = nsl;
+        if (rest > 0) {
+            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF,
+                                     (rocblas_int)rest, &one, c->d_planes.p + ms * nsl, (rocblas_int)M,
+                                     c->d_U.p + ms * nsl, (rocblas_int)M, &zero, c->d_Gpart.p + (size_t)nsl * N * nF,
+                                     N));
+            nparts = nsl + 1;
+        }
+        launch_reduce_partials(c->d_Gpart.p, 1, nparts, N * nF, c->d_G.p, c->stream);
+        HIPCHK(c, hipGetLastError());
         size_t lo = 0;
         for (int s = 0; s < 2; ++s) {
             Spin &sp = c->sp[s];
@@ -773,8 +956,7 @@ int evaluate(flosic_ctx *c) {
     for (int s = 0; s < 2; ++s) {
         const int n = c->sp[s].nocc;
         if (n == 0) continue;
-        nchunk[s] = std::max(1, std::min(256, (2048 + n - 1) / n));
-        while (nchunk[s] > 1 && M / nchunk[s] < 256) --nchunk[s];
+        nchunk[s] = vab_slices(M);
         vab_cnt += (size_t)n * n * n;
         vpart_cnt = std::max(vpart_cnt, (size_t)nchunk[s] * n * n * n);
     }
@@ -864,11 +1046,15 @@ int evaluate(flosic_ctx *c) {
     }
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[8]));
     c->timings[8] = ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[9], c->ev[4]));
+    c->timings[9] = ms;
     c->counts[0] = c->npair;
     c->counts[1] = c->nrows;
     c->counts[2] = c->M;
     c->counts[3] = c->nbas;
     c->counts[4] = nF;
+    c->counts[5] = c->ngroup;
+    c->counts[6] = (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) ? c->rows_padded_bf16 : c->rows_padded;
     c->evaluated = true;
     return 0;
 }
@@ -1054,8 +1240,7 @@ int32_t flosic_frmorb(flosic_ctx *c) {
     if (!c) return FLOSIC_EINVAL;
     int r = check_ready(c);
     if (r) return r;
-    std::vector<double> cflo;
-    return run_frmorb(c, cflo);
+    return run_frmorb(c);
 }
 
 int32_t flosic_get_fo(flosic_ctx *c, int32_t ispn, double *fmat, double *over, double *eval, double *filo, double *tmat) {
@@ -1157,13 +1342,13 @@ int32_t flosic_get_pairs(flosic_ctx *c, int32_t *npair, int32_t *list) {
 
 int32_t flosic_get_timings(flosic_ctx *c, int32_t n, double *ms) {
     if (!c || !ms || n < 0) return FLOSIC_EINVAL;
-    for (int i = 0; i < n && i < 9; ++i) ms[i] = c->timings[i];
+    for (int i = 0; i < n && i < 10; ++i) ms[i] = c->timings[i];
     return FLOSIC_OK;
 }
 
 int32_t flosic_get_counts(flosic_ctx *c, int32_t n, int64_t *counts) {
     if (!c || !counts || n < 0) return FLOSIC_EINVAL;
-    for (int i = 0; i < n && i < 5; ++i) counts[i] = c->counts[i];
+    for (int i = 0; i < n && i < 7; ++i) counts[i] = c->counts[i];
     return FLOSIC_OK;
 }
 

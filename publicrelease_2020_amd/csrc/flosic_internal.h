@@ -27,22 +27,31 @@ struct BasisDev {
     const double *prim_coef;
 };
 
-// primitive pair for the Hermite coefficients (sorted by L = lA + lB)
+// Exponent-pair group: every primitive shell pair whose two primitives sit on primitive centres (atom, bare
+// exponent) cA <= cB.  All of its shell pairs share p = a + b and P = (aA + bB)/p, hence one set of Hermite
+// integrals R_tuv(p, P - r); their Hermite coefficients add into one row block of ntuv(lA + lB) rows, lA and
+// lB the highest l present on each centre.  Groups are sorted by L = lA + lB.
 struct PairDev {
     double A[3], B[3];
     double a, b;
-    int psA, psB;     // primitive shell indices (rows of the prim-shell coefficient matrix)
+    int psA[3], psB[3];  // primitive shell index of l = 0,1,2 on each centre, -1 where absent
     int lA, lB;
-    int row;          // first Hermite row
-    double sym;       // 2 for psA != psB, else 1
+    int row;             // first Hermite row (rows of consecutive groups are contiguous, no padding)
+    double sym;          // 2 for cA != cB, else 1
 };
 
-// chunk of consecutive pairs of one order L whose Hermite rows [hrow0, hrow0 + kr) are contiguous and
-// zero-padded to kr (a multiple of 32, at most kChunkRows)
-constexpr int kChunkRows = 64;
+// chunk of consecutive groups of one order L: Hermite rows [hrow0, hrow0 + (pe-pb)*ntuv(L)) staged in LDS and
+// zero-extended to kr rows (a multiple of the MFMA K step of the kernel that uses the chunk list)
+constexpr int kChunkRows = 64;      // FP64 / FP32 MFMA kernels: K step 4
+constexpr int kChunkRowsBf16 = 64;  // split-BF16 kernel: K step 32
 struct ChunkDev {
     int pb, pe, L, kr, hrow0;
+    int prow0;  // first row of the chunk in the zero-extended row space of its list (split-BF16 H layout)
 };
+
+// FLO columns handled by one Coulomb block: all of them up to 256; the H leading dimension ldh is nF rounded
+// up to a multiple of this
+int coulomb_cols(int nF);
 
 // per-phase launch helpers (stream-ordered)
 void boys_table(double *h_tab);  // host: fills kBoysRows x kBoysCols
@@ -55,7 +64,8 @@ void launch_prim_coef(int nps, const int *ps_f0, const int *ps_stride, const int
                       const int *ps_nc, const double *ps_coef, const double *cflo, int nF, int ldc, double *dps,
                       hipStream_t s);
 
-void launch_hermite(int npair, const PairDev *pairs, const double *dps, int nF, double *H, int ldh,
+// H[row][f] for every group; columns nF <= f < ldh are written as zero
+void launch_hermite(int ngroup, const PairDev *pairs, const double *dps, int nF, double *H, int ldh,
                     hipStream_t s);
 
 // V[f*M + k] = sum_pairs sum_tuv R_tuv(P - r_k) H[row+tuv][f]
@@ -73,10 +83,13 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pai
 void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s);
 
 // split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,
-// FP32 accumulation inside a chunk, FP64 sums across chunks; H is read in FP64 and split while staged
-void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const double *H,
-                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
-                         int nF, double *V, int64_t ldv, hipStream_t s);
+// FP32 accumulation inside a chunk, FP64 sums across chunks.  H is split once per evaluation into
+// Hs[piece][ldh columns][ldp padded rows] (ldp = total zero-extended rows of the bf16 chunk list).
+void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, int64_t ldp,
+                    __bf16 *Hs, hipStream_t s);
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const __bf16 *Hs,
+                         int64_t ldp, int ldh, const double *x, const double *y, const double *z, int64_t M,
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s);
 
 // XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)
 void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w,
@@ -85,8 +98,9 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
 void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out /*[nF][nq]*/,
                             hipStream_t s);
 
-// V^i_ab for one spin: phi0 (n FLOs of this spin, [n][M]), Vtot ([n][M]), w -> out [i][b][a] partial sums
-void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nchunk,
-                double *partial /*[nchunk][n^3]*/, hipStream_t s);
+// V^i_ab for one spin: phi0 (n FLOs of this spin, [n][M]), Vtot ([n][M]), w -> partial sums [slice][i][b][a]
+int vab_slices(int64_t M);
+void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nslice,
+                double *partial /*[nslice][n^3]*/, hipStream_t s);
 
 }  // namespace flosic

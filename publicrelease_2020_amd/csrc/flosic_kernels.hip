@@ -11,7 +11,7 @@
 //                    Hermite recursion in registers, contraction with the pair coefficients
 //   k_xc             spin-polarised LSDA/PBE for rho_up = rho_i, rho_dn = 0 (subvlxc_libxc.F90), total SIC
 //                    potential and per-block partial sums of the RESULTS energies (apotnl_sic.F:536-581)
-//   k_vab            V^i_ab = sum_k w phi_a V_i phi_b, split over point chunks
+//   k_vab_mfma       V^i_ab = sum_k w phi_a V_i phi_b as an FP64 MFMA GEMM over point slices
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -221,22 +221,32 @@ __global__ void __launch_bounds__(128) k_hermite(const PairDev *__restrict__ pai
     const int L = pr.lA + pr.lB;
     const int nt = ntuv(L);
     const double fac = pr.sym * 2.0 * M_PI / p;
-    const int nA = ncart(pr.lA), nB = ncart(pr.lB), oA = loff(pr.lA), oB = loff(pr.lB);
-    for (int f = threadIdx.x; f < nF; f += blockDim.x) {
+    for (int f = threadIdx.x; f < ldh; f += blockDim.x) {
+        if (f >= nF) {
+            for (int t = 0; t < nt; ++t) H[(int64_t)(pr.row + t) * ldh + f] = 0.0;
+            continue;
+        }
         double h[35];
         for (int t = 0; t < 35; ++t) h[t] = 0.0;
-        for (int m = 0; m < nA; ++m) {
-            const double da = dps[((int64_t)pr.psA * 6 + m) * nF + f];
-            const int ax = cpow(oA + m, 0), ay = cpow(oA + m, 1), az = cpow(oA + m, 2);
-            for (int n = 0; n < nB; ++n) {
-                const double dab = da * dps[((int64_t)pr.psB * 6 + n) * nF + f];
-                if (dab == 0.0) continue;
-                const int bx = cpow(oB + n, 0), by = cpow(oB + n, 1), bz = cpow(oB + n, 2);
-                for (int t = 0; t <= ax + bx; ++t) {
-                    const double et = dab * E[0][ax][bx][t];
-                    for (int u = 0; u <= ay + by; ++u) {
-                        const double etu = et * E[1][ay][by][u];
-                        for (int v = 0; v <= az + bz; ++v) h[tuv_idx(t, u, v)] += etu * E[2][az][bz][v];
+        for (int la = 0; la <= pr.lA; ++la) {
+            if (pr.psA[la] < 0) continue;
+            for (int m = 0; m < ncart(la); ++m) {
+                const double da = dps[((int64_t)pr.psA[la] * 6 + m) * nF + f];
+                if (da == 0.0) continue;
+                const int ax = cpow(loff(la) + m, 0), ay = cpow(loff(la) + m, 1), az = cpow(loff(la) + m, 2);
+                for (int lb = 0; lb <= pr.lB; ++lb) {
+                    if (pr.psB[lb] < 0) continue;
+                    for (int n = 0; n < ncart(lb); ++n) {
+                        const double dab = da * dps[((int64_t)pr.psB[lb] * 6 + n) * nF + f];
+                        if (dab == 0.0) continue;
+                        const int bx = cpow(loff(lb) + n, 0), by = cpow(loff(lb) + n, 1), bz = cpow(loff(lb) + n, 2);
+                        for (int t = 0; t <= ax + bx; ++t) {
+                            const double et = dab * E[0][ax][bx][t];
+                            for (int u = 0; u <= ay + by; ++u) {
+                                const double etu = et * E[1][ay][by][u];
+                                for (int v = 0; v <= az + bz; ++v) h[tuv_idx(t, u, v)] += etu * E[2][az][bz][v];
+                            }
+                        }
                     }
                 }
             }
@@ -245,9 +255,9 @@ __global__ void __launch_bounds__(128) k_hermite(const PairDev *__restrict__ pai
     }
 }
 
-void launch_hermite(int npair, const PairDev *pairs, const double *dps, int nF, double *H, int ldh, hipStream_t s) {
-    if (npair <= 0) return;
-    hipLaunchKernelGGL(k_hermite, dim3(npair), dim3(128), 0, s, pairs, dps, nF, H, ldh);
+void launch_hermite(int ngroup, const PairDev *pairs, const double *dps, int nF, double *H, int ldh, hipStream_t s) {
+    if (ngroup <= 0) return;
+    hipLaunchKernelGGL(k_hermite, dim3(ngroup), dim3(128), 0, s, pairs, dps, nF, H, ldh);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -385,37 +395,45 @@ void launch_coulomb(const int *segL, const double4 *pairP, const int *pairRow, c
 }
 
 // ---------------------------------------------------------------------------------------------------
-// MFMA Coulomb.  Block = 4 wavefronts, tile of TP points x 64 FLO columns.  Per chunk of pairs:
-//   1. R_tuv(P - r_k) for every (pair, point) of the chunk into LDS (computed once, FP64 Boys/recursion)
-//   2. the chunk's H rows into LDS
-//   3. V_tile += R^T H with 16x16x4 MFMA (FP64, or FP32 with the chunk partial added to FP64 sums)
-// Fragment layout of v_mfma_*_16x16x4: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
-// D[4*(lane/16) + r][lane%16] in accumulator register r.
+// MFMA Coulomb.  V[f][k] = sum_rows R[k][row] H[row][f]: R_tuv(p, P - r_k) is generated on the fly (FP64 Boys
+// and Hermite recursion), H is the Hermite-coefficient matrix of every exponent-pair group for every FLO.
+// Block = 8 wavefronts, tile of TP points x NFB FLO columns (NFB covers all FLOs up to 256, so R is computed
+// once per (point, group)); wave (wr, wc) owns points [wr*TP/2, (wr+1)*TP/2) and columns [wc*NFB/4,
+// (wc+1)*NFB/4), so every lane holds TP*NFB/512 accumulators.  Per chunk of groups:
+//   1. R of every (group, point) of the chunk into LDS,
+//   2. the chunk's H rows into LDS, HR rows at a time,
+//   3. 16x16x4 MFMAs into register accumulators; FP32 accumulators are added into FP64 sums after every chunk.
 // ---------------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kCoulThreads = 512;  // 8 wavefronts: 2 point halves x 4 column quarters
 
 template <typename T> struct MfmaTraits;
+// Operands of both 16x16x4 forms: lane l holds A[l%16][l/16] and B[l/16][l%16].  The accumulators differ:
+// FP32 register r of lane l holds D[4*(l/16) + r][l%16], FP64 register r holds D[l/16 + 4*r][l%16].
 template <> struct MfmaTraits<double> {
     using acc = v4d;
     static __device__ __forceinline__ acc mma(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lr, int r) { return lr + 4 * r; }
 };
 template <> struct MfmaTraits<float> {
     using acc = v4f;
     static __device__ __forceinline__ acc mma(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lr, int r) { return 4 * lr + r; }
 };
 
+// R of chunk `ch` for points k0 .. k0+TP-1 into Rl[row][point] (rows beyond the chunk's groups are zero)
 template <int L, typename T, int TP, int LDR>
 __device__ __forceinline__ void chunk_R(const ChunkDev &ch, const double4 *__restrict__ pP, const double *__restrict__ X,
                                         const double *__restrict__ Y, const double *__restrict__ Z, int64_t k0,
                                         int64_t M, const double *__restrict__ tab, T *Rl) {
     constexpr int nt = ntuv(L);
     const int np = ch.pe - ch.pb;
-    for (int task = threadIdx.x; task < np * TP; task += 256) {
+    for (int task = threadIdx.x; task < np * TP; task += kCoulThreads) {
         const int pl = task / TP, pt = task - pl * TP;
         const int64_t k = k0 + pt;
         double R[nt];
@@ -429,43 +447,44 @@ __device__ __forceinline__ void chunk_R(const ChunkDev &ch, const double4 *__res
 #pragma unroll
         for (int t = 0; t < nt; ++t) Rl[(pl * nt + t) * LDR + pt] = (T)R[t];
     }
-    for (int e = threadIdx.x; e < (ch.kr - np * nt) * TP; e += 256) {
+    for (int e = threadIdx.x; e < (ch.kr - np * nt) * TP; e += kCoulThreads) {
         const int r = np * nt + e / TP, pt = e % TP;
         Rl[r * LDR + pt] = (T)0;
     }
 }
 
-template <typename T, int TP>
-__global__ void __launch_bounds__(256) k_coulomb_mfma(const ChunkDev *__restrict__ chunks, int nchunks,
+template <typename T, int TP, int NFB, int HR>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_tile(const ChunkDev *__restrict__ chunks, int nchunks,
                                                        const double4 *__restrict__ pP, const T *__restrict__ H, int ldh,
                                                        const double *__restrict__ X, const double *__restrict__ Y,
                                                        const double *__restrict__ Z, int64_t M,
                                                        const double *__restrict__ tab, int nF, double *__restrict__ V,
                                                        int64_t ldv) {
-    constexpr int NFB = 64, NCB = NFB / 16, RBW = TP / 64;  // row blocks of 16 points per wave
-    static_assert(RBW >= 1, "tile must hold 64 points per wavefront group");
-    constexpr int LDR = TP + 4, LDH = NFB + 4;
+    constexpr int NRT = TP / 32, NCW = NFB / 64;  // 16x16 tiles per wave: points, columns
+    constexpr int PAD = sizeof(T) == 8 ? 8 : 16;  // row shift of 16 banks between the 4 row groups of a wave
+    constexpr int LDR = TP + PAD, LDH = NFB + PAD;
+    static_assert(TP % 32 == 0 && NFB % 64 == 0 && HR % 4 == 0 && kChunkRows % HR == 0, "tile shape");
     __shared__ T Rl[kChunkRows * LDR];
-    __shared__ T Hl[kChunkRows * LDH];
+    __shared__ T Hl[HR * LDH];
     using Tr = MfmaTraits<T>;
     using acc_t = typename Tr::acc;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int lr = lane >> 4, lc = lane & 15;
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int f0 = blockIdx.y * NFB;
-    acc_t acc[RBW][NCB];
-    double sum[RBW][NCB][4];
+    const int fb = blockIdx.y * NFB;
+    const int pw = (wave >> 2) * NRT * 16, cw = (wave & 3) * NCW * 16;
+    acc_t acc[NRT][NCW];
+    double sum[NRT][NCW][4];
 #pragma unroll
-    for (int i = 0; i < RBW; ++i)
+    for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int j = 0; j < NCB; ++j) {
+        for (int j = 0; j < NCW; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 acc[i][j][r] = (T)0;
                 sum[i][j][r] = 0.0;
             }
-        }
-    const int lr = lane >> 4, lc = lane & 15;
     for (int c = 0; c < nchunks; ++c) {
         const ChunkDev ch = chunks[c];
         switch (ch.L) {
@@ -475,57 +494,81 @@ __global__ void __launch_bounds__(256) k_coulomb_mfma(const ChunkDev *__restrict
             case 3: chunk_R<3, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
             default: chunk_R<4, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
         }
-        for (int e = threadIdx.x; e < ch.kr * NFB; e += 256) {
-            const int r = e / NFB, f = e - r * NFB;
-            Hl[r * LDH + f] = H[(int64_t)(ch.hrow0 + r) * ldh + f0 + f];
-        }
-        __syncthreads();
-        for (int ks = 0; ks < ch.kr; ks += 4) {
-            T a[RBW], b[NCB];
+        const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
+        for (int h0 = 0; h0 < ch.kr; h0 += HR) {
+            const int hr = min(HR, ch.kr - h0);
+            if (h0 > 0) __syncthreads();  // the previous H stage has been consumed
+            for (int e = threadIdx.x; e < hr * NFB; e += kCoulThreads) {
+                const int r = e / NFB, f = e - r * NFB;
+                Hl[r * LDH + f] = h0 + r < nreal ? H[(int64_t)(ch.hrow0 + h0 + r) * ldh + fb + f] : (T)0;
+            }
+            __syncthreads();
+            for (int ks = 0; ks < hr; ks += 4) {
+                T a[NRT], b[NCW];
 #pragma unroll
-            for (int i = 0; i < RBW; ++i) a[i] = Rl[(ks + lr) * LDR + (wave * RBW + i) * 16 + lc];
+                for (int i = 0; i < NRT; ++i) a[i] = Rl[(h0 + ks + lr) * LDR + pw + i * 16 + lc];
 #pragma unroll
-            for (int j = 0; j < NCB; ++j) b[j] = Hl[(ks + lr) * LDH + j * 16 + lc];
+                for (int j = 0; j < NCW; ++j) b[j] = Hl[(ks + lr) * LDH + cw + j * 16 + lc];
 #pragma unroll
-            for (int i = 0; i < RBW; ++i)
+                for (int i = 0; i < NRT; ++i)
 #pragma unroll
-                for (int j = 0; j < NCB; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
+                    for (int j = 0; j < NCW; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
+            }
         }
         if constexpr (sizeof(T) == 4) {  // FP64 sum of each chunk's FP32 partial
 #pragma unroll
-            for (int i = 0; i < RBW; ++i)
+            for (int i = 0; i < NRT; ++i)
 #pragma unroll
-                for (int j = 0; j < NCB; ++j)
+                for (int j = 0; j < NCW; ++j)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         sum[i][j][r] += (double)acc[i][j][r];
                         acc[i][j][r] = 0.0f;
                     }
         }
-        __syncthreads();
+        __syncthreads();  // Rl and Hl are rewritten by the next chunk
     }
 #pragma unroll
-    for (int i = 0; i < RBW; ++i)
+    for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int j = 0; j < NCB; ++j) {
-            const int f = f0 + j * 16 + lc;
+        for (int j = 0; j < NCW; ++j) {
+            const int f = fb + cw + j * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t k = k0 + (wave * RBW + i) * 16 + 4 * lr + r;
+                const int64_t k = k0 + pw + i * 16 + Tr::row(lr, r);
                 const double v = sizeof(T) == 4 ? sum[i][j][r] : (double)acc[i][j][r];
                 if (f < nF && k < M) V[(int64_t)f * ldv + k] = v;
             }
         }
 }
 
+// FLO columns per block: all of them up to 256 (ldh is nF rounded up to a multiple of this)
+int coulomb_cols(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 256; }
+
+template <typename T, int TP, int HR>
+static void launch_tile(int nfb, const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
+                        const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
+                        double *V, int64_t ldv, hipStream_t s) {
+    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)(ldh / nfb)), blk(kCoulThreads);
+#define FLOSIC_TILE(NFB)                                                                                            \
+    hipLaunchKernelGGL((k_coulomb_tile<T, TP, NFB, HR>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, \
+                       btab, nF, V, ldv)
+    switch (nfb) {
+        case 64: FLOSIC_TILE(64); break;
+        case 128: FLOSIC_TILE(128); break;
+        case 192: FLOSIC_TILE(192); break;
+        default: FLOSIC_TILE(256); break;
+    }
+#undef FLOSIC_TILE
+}
+
 template <typename T>
 void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
                          const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
                          double *V, int64_t ldv, hipStream_t s) {
-    constexpr int TP = sizeof(T) == 8 ? 64 : 128;  // LDS: 70 KB (FP64) / 51 KB (FP32) per block
-    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)((nF + 63) / 64)), blk(256);
-    hipLaunchKernelGGL((k_coulomb_mfma<T, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab, nF,
-                       V, ldv);
+    // LDS per block, NFB = 192: FP64 36 KB R + 26 KB H; FP32 20 KB R + 27 KB H
+    constexpr int TP = 64, HR = sizeof(T) == 8 ? 16 : 32;
+    launch_tile<T, TP, HR>(coulomb_cols(nF), chunks, nchunks, pairP, H, ldh, x, y, z, M, btab, nF, V, ldv, s);
 }
 template void launch_coulomb_mfma<double>(const ChunkDev *, int, const double4 *, const double *, int, const double *,
                                           const double *, const double *, int64_t, const double *, int, double *,
@@ -538,12 +581,14 @@ template void launch_coulomb_mfma<float>(const ChunkDev *, int, const double4 *,
 // Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces (each the rounded remainder of the previous), so the
 // product x*y is carried by x1y1 + x1y2 + x2y1 (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error,
 // every term on the 16x16x32 bf16 MFMA.  Operand fragments are 8 consecutive Hermite rows of one point /
-// one FLO column, so both tiles are stored row-fastest ([point][row], [column][row]) for 16-byte reads.
+// one FLO column, so both tiles are stored row-fastest ([point][row], [column][row]) for 16-byte reads.  H is
+// split once per evaluation into Hs[piece][column][padded row] (k_split_h); R is split as it is generated.
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
 
 template <int NP>
-__device__ __forceinline__ void split_bf16(double x, __bf16 *out, int stride) {
+__device__ __forceinline__ void split_bf16(double x, __bf16 *out, int64_t stride) {
     double r = x;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
@@ -551,6 +596,29 @@ __device__ __forceinline__ void split_bf16(double x, __bf16 *out, int stride) {
         out[p * stride] = b;
         r -= (double)(float)b;
     }
+}
+
+// Hs[p][f][prow0 + r] = piece p of H[hrow0 + r][f], zero beyond the chunk's rows and for padding columns;
+// one block per chunk of the bf16 chunk list
+template <int NP>
+__global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__restrict__ H, int ldh, int nF,
+                          int64_t ldp, __bf16 *__restrict__ Hs) {
+    const ChunkDev ch = chunks[blockIdx.x];
+    const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
+    for (int e = threadIdx.x; e < ldh * ch.kr; e += blockDim.x) {
+        const int f = e / ch.kr, r = e - f * ch.kr;
+        const double v = (r < nreal && f < nF) ? H[(int64_t)(ch.hrow0 + r) * ldh + f] : 0.0;
+        split_bf16<NP>(v, Hs + (int64_t)f * ldp + ch.prow0 + r, (int64_t)ldh * ldp);
+    }
+}
+
+void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, int64_t ldp,
+                    __bf16 *Hs, hipStream_t s) {
+    if (nchunks <= 0) return;
+    if (npieces == 2)
+        hipLaunchKernelGGL((k_split_h<2>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, ldp, Hs);
+    else
+        hipLaunchKernelGGL((k_split_h<3>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, ldp, Hs);
 }
 
 template <int L, int NP, int TP, int LDK>
@@ -561,7 +629,7 @@ __device__ __forceinline__ void chunk_R_bf16(const ChunkDev &ch, const double4 *
     constexpr int nt = ntuv(L);
     constexpr int PS = TP * LDK;  // piece stride
     const int np = ch.pe - ch.pb;
-    for (int task = threadIdx.x; task < np * TP; task += 256) {
+    for (int task = threadIdx.x; task < np * TP; task += kCoulThreads) {
         const int pl = task / TP, pt = task - pl * TP;
         const int64_t k = k0 + pt;
         double R[nt];
@@ -576,35 +644,40 @@ __device__ __forceinline__ void chunk_R_bf16(const ChunkDev &ch, const double4 *
         for (int t = 0; t < nt; ++t) split_bf16<NP>(R[t], Rb + pt * LDK + pl * nt + t, PS);
     }
     const int r0 = np * nt;
-    for (int e = threadIdx.x; e < (ch.kr - r0) * TP; e += 256) {
+    for (int e = threadIdx.x; e < (ch.kr - r0) * TP; e += kCoulThreads) {
         const int pt = e / (ch.kr - r0), r = r0 + e % (ch.kr - r0);
 #pragma unroll
         for (int p = 0; p < NP; ++p) Rb[p * PS + pt * LDK + r] = (__bf16)0.0f;
     }
 }
 
-template <int NP, int TP>
-__global__ void __launch_bounds__(256) k_coulomb_bf16(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                       const double4 *__restrict__ pP, const double *__restrict__ H,
-                                                       int ldh, const double *__restrict__ X,
-                                                       const double *__restrict__ Y, const double *__restrict__ Z,
-                                                       int64_t M, const double *__restrict__ tab, int nF,
-                                                       double *__restrict__ V, int64_t ldv) {
-    constexpr int NFB = 64, NCB = NFB / 16, RBW = TP / 64;
-    constexpr int LDK = kChunkRows + 8;  // 144-byte rows: conflict-free 16-byte fragment reads
+// Block = 8 wavefronts, TP points x NFB FLO columns, waves laid out as in k_coulomb_tile.  Per chunk: R pieces
+// of all its rows into LDS, then H pieces 32 rows at a time, NP*(NP+1)/2 products per 16x16x32 step.
+template <int NP, int TP, int NFB>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_split(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                        const double4 *__restrict__ pP, const __bf16 *__restrict__ Hs,
+                                                        int64_t ldp, int ldh, const double *__restrict__ X,
+                                                        const double *__restrict__ Y, const double *__restrict__ Z,
+                                                        int64_t M, const double *__restrict__ tab, int nF,
+                                                        double *__restrict__ V, int64_t ldv) {
+    constexpr int NRT = TP / 32, NCW = NFB / 64;
+    constexpr int LDK = kChunkRowsBf16 + 8;  // 144-byte point rows: conflict-free 16-byte fragment reads
+    constexpr int KS = 32, LDKH = KS + 8;    // 80-byte column rows of one staged K step
+    static_assert(TP % 32 == 0 && NFB % 64 == 0, "tile shape");
     __shared__ __attribute__((aligned(16))) __bf16 Rb[NP * TP * LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 Hb[NP * NFB * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Hb[NP * NFB * LDKH];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int lr = lane >> 4, lc = lane & 15;
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int f0 = blockIdx.y * NFB;
-    v4f acc[RBW][NCB];
-    double sum[RBW][NCB][4];
+    const int fb = blockIdx.y * NFB;
+    const int pw = (wave >> 2) * NRT * 16, cw = (wave & 3) * NCW * 16;
+    v4f acc[NRT][NCW];
+    double sum[NRT][NCW][4];
 #pragma unroll
-    for (int i = 0; i < RBW; ++i)
+    for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int j = 0; j < NCB; ++j)
+        for (int j = 0; j < NCW; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 acc[i][j][r] = 0.0f;
@@ -619,73 +692,88 @@ __global__ void __launch_bounds__(256) k_coulomb_bf16(const ChunkDev *__restrict
             case 3: chunk_R_bf16<3, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
             default: chunk_R_bf16<4, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
         }
-        for (int e = threadIdx.x; e < ch.kr * NFB; e += 256) {
-            const int r = e / NFB, f = e - r * NFB;
-            split_bf16<NP>(H[(int64_t)(ch.hrow0 + r) * ldh + f0 + f], Hb + f * LDK + r, NFB * LDK);
-        }
-        __syncthreads();
-        for (int ks = 0; ks < ch.kr; ks += 32) {
-            v8bf a[NP][RBW], b[NP][NCB];
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-#pragma unroll
-                for (int i = 0; i < RBW; ++i)
-                    a[p][i] = *(const v8bf *)(Rb + p * TP * LDK + ((wave * RBW + i) * 16 + lc) * LDK + ks + 8 * lr);
-#pragma unroll
-                for (int j = 0; j < NCB; ++j)
-                    b[p][j] = *(const v8bf *)(Hb + p * NFB * LDK + (j * 16 + lc) * LDK + ks + 8 * lr);
+        for (int ks = 0; ks < ch.kr; ks += KS) {
+            if (ks > 0) __syncthreads();  // the previous H stage has been consumed
+            for (int e = threadIdx.x; e < NP * NFB * (KS / 8); e += kCoulThreads) {
+                const int q = e & 3, pf = e >> 2;
+                const int p = pf / NFB, f = pf - p * NFB;
+                const __bf16 *src = Hs + ((int64_t)p * ldh + fb + f) * ldp + ch.prow0 + ks + 8 * q;
+                *(v8s *)(Hb + (p * NFB + f) * LDKH + 8 * q) = *(const v8s *)src;
             }
+            __syncthreads();
+            v8bf a[NP][NRT];
 #pragma unroll
-            for (int i = 0; i < RBW; ++i)
+            for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int j = 0; j < NCB; ++j) {
+                for (int i = 0; i < NRT; ++i)
+                    a[p][i] = *(const v8bf *)(Rb + p * TP * LDK + (pw + i * 16 + lc) * LDK + ks + 8 * lr);
+#pragma unroll
+            for (int j = 0; j < NCW; ++j) {
+                v8bf b[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) b[p] = *(const v8bf *)(Hb + (p * NFB + cw + j * 16 + lc) * LDKH + 8 * lr);
+#pragma unroll
+                for (int i = 0; i < NRT; ++i) {
                     v4f cc = acc[i][j];
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0], cc, 0, 0, 0);
                     if constexpr (NP == 3) {
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0], cc, 0, 0, 0);
                     }
                     acc[i][j] = cc;
                 }
+            }
         }
 #pragma unroll
-        for (int i = 0; i < RBW; ++i)
+        for (int i = 0; i < NRT; ++i)
 #pragma unroll
-            for (int j = 0; j < NCB; ++j)
+            for (int j = 0; j < NCW; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     sum[i][j][r] += (double)acc[i][j][r];
                     acc[i][j][r] = 0.0f;
                 }
-        __syncthreads();
+        __syncthreads();  // Rb and Hb are rewritten by the next chunk
     }
 #pragma unroll
-    for (int i = 0; i < RBW; ++i)
+    for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int j = 0; j < NCB; ++j) {
-            const int f = f0 + j * 16 + lc;
+        for (int j = 0; j < NCW; ++j) {
+            const int f = fb + cw + j * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t k = k0 + (wave * RBW + i) * 16 + 4 * lr + r;
+                const int64_t k = k0 + pw + i * 16 + 4 * lr + r;
                 if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[i][j][r];
             }
         }
 }
 
-void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const double *H,
-                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
-                         int nF, double *V, int64_t ldv, hipStream_t s) {
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const __bf16 *Hs,
+                         int64_t ldp, int ldh, const double *x, const double *y, const double *z, int64_t M,
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s) {
     constexpr int TP = 64;
-    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)((nF + 63) / 64)), blk(256);
-    if (npieces == 2)
-        hipLaunchKernelGGL((k_coulomb_bf16<2, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab,
-                           nF, V, ldv);
-    else
-        hipLaunchKernelGGL((k_coulomb_bf16<3, TP>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, btab,
-                           nF, V, ldv);
+    const int nfb = coulomb_cols(nF);
+    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)(ldh / nfb)), blk(kCoulThreads);
+#define FLOSIC_SPLIT(NP, NFB)                                                                                       \
+    hipLaunchKernelGGL((k_coulomb_split<NP, TP, NFB>), grid, blk, 0, s, chunks, nchunks, pairP, Hs, ldp, ldh, x, y, z, \
+                       M, btab, nF, V, ldv)
+#define FLOSIC_SPLIT_NFB(NP)            \
+    switch (nfb) {                      \
+        case 64: FLOSIC_SPLIT(NP, 64); break;   \
+        case 128: FLOSIC_SPLIT(NP, 128); break; \
+        case 192: FLOSIC_SPLIT(NP, 192); break; \
+        default: FLOSIC_SPLIT(NP, 256); break;  \
+    }
+    if (npieces == 2) {
+        FLOSIC_SPLIT_NFB(2)
+    } else {
+        FLOSIC_SPLIT_NFB(3)
+    }
+#undef FLOSIC_SPLIT_NFB
+#undef FLOSIC_SPLIT
 }
 
 __global__ void k_convert_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
@@ -996,57 +1084,106 @@ void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, dou
 }
 
 // ---------------------------------------------------------------------------------------------------
-// V^i_ab partial sums over point chunks.  Block (i, chunk); tile of 64 points staged in LDS as [p][a]
+// V^i_ab = sum_k Vtot_i(k) [w phi_a phi_b](k): a GEMM over the points with the pair-product operand generated in
+// registers.  Block (pair block, point slice, i block): pair block = 8 a x 8 b with a-block <= b-block (V is
+// symmetric in a,b; the mirror is stored too), 4 wavefronts, wave w takes a = 8A + 2w + {0,1} and all 8 b, i.e.
+// 16 pair columns, against up to 16*NIT i rows.  Points are staged 32 at a time; 16x16x4 FP64 MFMAs.  Partial
+// sums per slice: part[slice][i][b][a].
 // ---------------------------------------------------------------------------------------------------
-constexpr int kVabMaxN = 128;
-constexpr int kVabTile = 32;
+constexpr int kVabKS = 32;
 
-__global__ void __launch_bounds__(256) k_vab(const double *__restrict__ phi0, const double *__restrict__ Vtot,
-                                              const double *__restrict__ w, int64_t M, int n, int nchunk,
-                                              double *__restrict__ partial) {
-    __shared__ double sphi[kVabTile][kVabMaxN + 1];
-    __shared__ double su[kVabTile];
-    const int i = blockIdx.x, c = blockIdx.y;
-    const int64_t k0 = M * c / nchunk, k1 = M * (c + 1) / nchunk;
-    const int nout = n * n;
-    constexpr int kMaxPer = kVabMaxN * kVabMaxN / 256;
-    double acc[kMaxPer];
+template <int NIT>
+__global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi, const double *__restrict__ Vt,
+                                                  const double *__restrict__ w, int64_t M, int n, int nb8,
+                                                  int64_t kslice, double *__restrict__ part) {
+    constexpr int NI = 16 * NIT, LDV = kVabKS + 1;
+    __shared__ double Vl[NI * LDV];
+    __shared__ double Pa[8 * LDV], Pb[8 * LDV], Wl[kVabKS];
+    // pair block index -> (A, B), A <= B, row-major over the upper triangle
+    int A = 0, rem = blockIdx.x;
+    while (rem >= nb8 - A) {
+        rem -= nb8 - A;
+        ++A;
+    }
+    const int B = A + rem;
+    const int s = blockIdx.y, i0 = blockIdx.z * NI;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int la = 2 * wave + (lc >> 3), lb = lc & 7;  // this lane's pair inside the 8x8 block
+    const int64_t kb = (int64_t)s * kslice, ke = min(M, kb + kslice);
+    v4d acc[NIT];
 #pragma unroll
-    for (int j = 0; j < kMaxPer; ++j) acc[j] = 0.0;
-    for (int64_t t0 = k0; t0 < k1; t0 += kVabTile) {
-        const int np = (int)((k1 - t0) < kVabTile ? (k1 - t0) : kVabTile);
-        for (int e = threadIdx.x; e < kVabTile * n; e += 256) {
-            const int p = e % kVabTile, a = e / kVabTile;
-            sphi[p][a] = p < np ? phi0[(int64_t)a * M + t0 + p] : 0.0;
+    for (int it = 0; it < NIT; ++it) acc[it] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int64_t t0 = kb; t0 < ke; t0 += kVabKS) {
+        for (int e = threadIdx.x; e < NI * kVabKS; e += 256) {
+            const int i = e / kVabKS, kk = e - i * kVabKS;
+            const int64_t k = t0 + kk;
+            Vl[i * LDV + kk] = (i0 + i < n && k < ke) ? Vt[(int64_t)(i0 + i) * M + k] : 0.0;
         }
-        if (threadIdx.x < kVabTile) {
-            const int p = threadIdx.x;
-            su[p] = p < np ? w[t0 + p] * Vtot[(int64_t)i * M + t0 + p] : 0.0;
+        for (int e = threadIdx.x; e < 16 * kVabKS; e += 256) {
+            const int r = e / kVabKS, kk = e - r * kVabKS;
+            const int64_t k = t0 + kk;
+            const int f = r < 8 ? 8 * A + r : 8 * B + r - 8;
+            const double v = (f < n && k < ke) ? phi[(int64_t)f * M + k] : 0.0;
+            if (r < 8)
+                Pa[r * LDV + kk] = v;
+            else
+                Pb[(r - 8) * LDV + kk] = v;
         }
+        if (threadIdx.x < kVabKS) Wl[threadIdx.x] = t0 + threadIdx.x < ke ? w[t0 + threadIdx.x] : 0.0;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kMaxPer; ++j) {
-            const int o = threadIdx.x + 256 * j;
-            if (o < nout) {
-                const int a = o % n, b = o / n;
-                double s = acc[j];
-                for (int p = 0; p < kVabTile; ++p) s = fma(sphi[p][a] * su[p], sphi[p][b], s);
-                acc[j] = s;
-            }
+        for (int ks = 0; ks < kVabKS; ks += 4) {
+            const int kk = ks + lr;
+            const double bv = Wl[kk] * Pa[la * LDV + kk] * Pb[lb * LDV + kk];
+#pragma unroll
+            for (int it = 0; it < NIT; ++it)
+                acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[(it * 16 + lc) * LDV + kk], bv, acc[it], 0, 0, 0);
         }
         __syncthreads();
     }
+    // FP64 accumulator layout: register r of lane l holds C[l/16 + 4r][l%16]
+    const int a = 8 * A + 2 * wave + (lc >> 3), b = 8 * B + (lc & 7);
+    if (a >= n || b >= n) return;
+    const int64_t nn = (int64_t)n * n, base = (int64_t)s * n * nn;
 #pragma unroll
-    for (int j = 0; j < kMaxPer; ++j) {
-        const int o = threadIdx.x + 256 * j;
-        if (o < nout) partial[((int64_t)c * n + i) * nout + o] = acc[j];
-    }
+    for (int it = 0; it < NIT; ++it)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = i0 + it * 16 + lr + 4 * r;
+            if (i >= n) continue;
+            part[base + i * nn + (int64_t)b * n + a] = acc[it][r];
+            if (A != B) part[base + i * nn + (int64_t)a * n + b] = acc[it][r];
+        }
 }
 
-void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nchunk, double *partial,
+int vab_slices(int64_t M) {
+    // point slices per spin: enough blocks to fill the chip, slices of at least 1024 points
+    int s = 64;
+    while (s > 1 && M / s < 1024) --s;
+    return s;
+}
+
+void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nslice, double *part,
                 hipStream_t s) {
-    hipLaunchKernelGGL(k_vab, dim3((unsigned)n, (unsigned)nchunk), dim3(256), 0, s, phi0, Vtot, w, M, n, nchunk,
-                       partial);
+    if (n <= 0) return;
+    const int nb8 = (n + 7) / 8;
+    const int64_t kslice = ((M + nslice - 1) / nslice + kVabKS - 1) / kVabKS * kVabKS;
+    const int nit = std::min(8, (n + 15) / 16);
+    const dim3 grid((unsigned)(nb8 * (nb8 + 1) / 2), (unsigned)nslice, (unsigned)((n + 16 * nit - 1) / (16 * nit)));
+#define FLOSIC_VAB(NIT) \
+    hipLaunchKernelGGL((k_vab_mfma<NIT>), grid, dim3(256), 0, s, phi0, Vtot, w, M, n, nb8, kslice, part)
+    switch (nit) {
+        case 1: FLOSIC_VAB(1); break;
+        case 2: FLOSIC_VAB(2); break;
+        case 3: FLOSIC_VAB(3); break;
+        case 4: FLOSIC_VAB(4); break;
+        case 5: FLOSIC_VAB(5); break;
+        case 6: FLOSIC_VAB(6); break;
+        case 7: FLOSIC_VAB(7); break;
+        default: FLOSIC_VAB(8); break;
+    }
+#undef FLOSIC_VAB
 }
 
 // ---------------------------------------------------------------------------------------------------

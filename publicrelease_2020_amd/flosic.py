@@ -218,13 +218,13 @@ class Flosic:
         return out[:n.value]
 
     def timings(self) -> np.ndarray:
-        t = np.zeros(9)
-        self._chk(self._L.flosic_get_timings(self._h, 9, _d(t)))
+        t = np.zeros(10)
+        self._chk(self._L.flosic_get_timings(self._h, 10, _d(t)))
         return t
 
     def counts(self) -> np.ndarray:
-        c = np.zeros(5, np.int64)
-        self._chk(self._L.flosic_get_counts(self._h, 5, c.ctypes.data_as(_lp)))
+        c = np.zeros(7, np.int64)
+        self._chk(self._L.flosic_get_counts(self._h, 7, c.ctypes.data_as(_lp)))
         return c
 
     def results(self) -> dict:
