@@ -108,6 +108,18 @@ module flosic_hip
       integer(c_int32_t), value :: ispn, iorb
       real(c_double) :: pot(*), vh(*)
     end function
+    integer(c_int32_t) function flosic_lagmult_slv(ctx, ispn, pot, ldsic, sic) bind(C, name="flosic_lagmult_slv")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn, ldsic
+      real(c_double) :: pot(*), sic(ldsic, *)
+    end function
+    integer(c_int32_t) function flosic_sic_hamiltonian(ctx, ispn, hao) bind(C, name="flosic_sic_hamiltonian")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn
+      real(c_double) :: hao(*)
+    end function
     integer(c_int32_t) function flosic_get_pairs(ctx, npair, list) bind(C, name="flosic_get_pairs")
       import :: c_int32_t, c_ptr
       type(c_ptr), value :: ctx

@@ -110,6 +110,19 @@ int32_t flosic_fod_gradient(flosic_ctx *ctx, int32_t ispn, double *grad);
  * mesh; vh (may be NULL) receives V_H alone. */
 int32_t flosic_get_potential(flosic_ctx *ctx, int32_t ispn, int32_t iorb, double *pot, double *vh);
 
+/* LAGMULT_SLV (lagmult_slv.F:2-187, caller NUMFORCE numforce.F:83-90): adds the DFT part of the Lagrange
+ * multipliers to the SIC block of spin ispn,
+ *   sic[j + ldsic*k] += sum_mesh w phi_j (-1/2 lap phi_k + pot phi_k),   j, k = 1..nfrm (FLOs of ispn),
+ * with pot the DFT potential of that spin on the mesh (POTOUT(NMSH*(ISPN-1)+1 ..), nmsh values).  The FLOs are
+ * those of the last evaluation; sic is typically the array flosic_siclagm returned (ldsic = nwf). */
+int32_t flosic_lagmult_slv(flosic_ctx *ctx, int32_t ispn, const double *pot, int32_t ldsic, double *sic);
+
+/* SIC Hamiltonian in the AO basis as SICHAM/APATCH assemble it (sicham.F:920-945, sicmat.F:146-262 with MM=2):
+ *   H_ab = sum_i 1/2 ( <chi_a|V_i|phi_i><phi_i|chi_b> + <chi_b|V_i|phi_i><phi_i|chi_a> )
+ * over the FLOs i of spin ispn, both brackets integrated on the mesh, V_i = V_H + V_xc of rho_i.  hao is
+ * nbas x nbas column-major (full, symmetric). */
+int32_t flosic_sic_hamiltonian(flosic_ctx *ctx, int32_t ispn, double *hao);
+
 /* Primitive pair list built by flosic_set_basis (coupot_sic.F:228-310): npair rows of
  * (atomA, lA, bareA, atomB, lB, bareB), 0-based, in canonical order.  Pass list = NULL to query npair. */
 int32_t flosic_get_pairs(flosic_ctx *ctx, int32_t *npair, int32_t *list);

@@ -1188,6 +1188,92 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
 
 int oracle_nbas(const ob_basis *b) { return basis_nbas(b); }
 
+/* FLO coefficients of one spin: FRMORB2(0,0) + FRMIORB + UNRAVEL2 (frmorb2.F:140-525, unravel2.f:107-225);
+ * cflo is nbas x nocc column-major.  Returns the Lowdin status (2 when an eigenvalue is below 1e-8). */
+int oracle_flo_coef(const ob_basis *b, const ob_spin *s, double *cflo) {
+    int nbas = basis_nbas(b), n = s->nocc;
+    double *fm = malloc(sizeof(double) * 4 * n * n), *ov = malloc(sizeof(double) * n * n);
+    double *q = malloc(sizeof(double) * n), *U = malloc(sizeof(double) * n * n);
+    double *sh = malloc(sizeof(double) * n * n), *tm = malloc(sizeof(double) * n * n);
+    int st = frmorb(b, nbas, s, fm, ov, q, U, sh, tm, cflo);
+    free(fm); free(ov); free(q); free(U); free(sh); free(tm);
+    return st;
+}
+
+/* FLOs of one spin on the mesh: value planes (and the Laplacian when lap != NULL), [f][k] point fastest */
+static void flo_fields(const ob_basis *b, int64_t npts, const double *pts, int n, const double *cflo, double *phi,
+                       double *lap) {
+    int nbas = basis_nbas(b);
+    int ng = lap ? 10 : 1;
+    double *chi = malloc(sizeof(double) * (size_t)ng * npts * nbas);
+    oracle_basis_eval(b, lap ? 2 : 0, (int)npts, pts, chi);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < npts; ++k)
+        for (int f = 0; f < n; ++f) {
+            const double *cf = cflo + (size_t)f * nbas;
+            double v = 0, l = 0;
+            for (int mu = 0; mu < nbas; ++mu) {
+                v += chi[(size_t)k * nbas + mu] * cf[mu];
+                if (lap)
+                    l += (chi[((size_t)4 * npts + k) * nbas + mu] + chi[((size_t)5 * npts + k) * nbas + mu] +
+                          chi[((size_t)6 * npts + k) * nbas + mu]) * cf[mu];
+            }
+            phi[(size_t)f * npts + k] = v;
+            if (lap) lap[(size_t)f * npts + k] = l;
+        }
+    free(chi);
+}
+
+/* LAGMULT_SLV (lagmult_slv.F:139-170): lam[j + n*kk] = sum_k w phi_j (-1/2 lap phi_kk + pot phi_kk) for the
+ * n FLOs with coefficients cflo (nbas x n) and the DFT potential pot on the mesh */
+void oracle_lagmult(const ob_basis *b, int64_t npts, const double *pts, const double *w, int n, const double *cflo,
+                    const double *pot, double *lam) {
+    double *phi = malloc(sizeof(double) * (size_t)n * npts), *lap = malloc(sizeof(double) * (size_t)n * npts);
+    flo_fields(b, npts, pts, n, cflo, phi, lap);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < n; ++j)
+        for (int kk = 0; kk < n; ++kk) {
+            ksum acc = {0, 0};
+            for (int64_t k = 0; k < npts; ++k)
+                kadd(&acc, w[k] * phi[(size_t)j * npts + k] *
+                               (-0.5 * lap[(size_t)kk * npts + k] + pot[k] * phi[(size_t)kk * npts + k]));
+            lam[j + (size_t)n * kk] = kval(&acc);
+        }
+    free(phi); free(lap);
+}
+
+/* APATCH with MM=2 (sicham.F:920-945, sicmat.F:146-262): H[a + N*b] = sum_i 1/2 (G_ai B_bi + G_bi B_ai) with
+ * B_ai = sum_k w phi_i chi_a and G_ai = sum_k w V_i phi_i chi_a (vpot: n x npts, V_i = V_H + V_xc) */
+void oracle_sic_hamiltonian(const ob_basis *b, int64_t npts, const double *pts, const double *w, int n,
+                            const double *cflo, const double *vpot, double *H) {
+    int nbas = basis_nbas(b);
+    double *chi = malloc(sizeof(double) * (size_t)npts * nbas), *phi = malloc(sizeof(double) * (size_t)n * npts);
+    oracle_basis_eval(b, 0, (int)npts, pts, chi);
+    flo_fields(b, npts, pts, n, cflo, phi, NULL);
+    double *B = malloc(sizeof(double) * (size_t)nbas * n), *G = malloc(sizeof(double) * (size_t)nbas * n);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < nbas; ++a) {
+            ksum sb = {0, 0}, sg = {0, 0};
+            for (int64_t k = 0; k < npts; ++k) {
+                double t = w[k] * phi[(size_t)i * npts + k] * chi[(size_t)k * nbas + a];
+                kadd(&sb, t);
+                kadd(&sg, t * vpot[(size_t)i * npts + k]);
+            }
+            B[a + (size_t)nbas * i] = kval(&sb);
+            G[a + (size_t)nbas * i] = kval(&sg);
+        }
+    for (int bb = 0; bb < nbas; ++bb)
+        for (int a = 0; a < nbas; ++a) {
+            double acc = 0;
+            for (int i = 0; i < n; ++i)
+                acc += 0.5 * (G[a + (size_t)nbas * i] * B[bb + (size_t)nbas * i] +
+                              G[bb + (size_t)nbas * i] * B[a + (size_t)nbas * i]);
+            H[a + (size_t)nbas * bb] = acc;
+        }
+    free(chi); free(phi); free(B); free(G);
+}
+
 /* kernel-level access for tests: which 0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v)        */
 /* 3 PBECOR(rs,zet,t,uu,vv,ww).  out: 0: exl,exn,vx  1: ec,vcup  2: exl,exn,vx  3: ec,vcup,h,dvcup      */
 void oracle_xc_kernel(int which, const double *in, double *out) {

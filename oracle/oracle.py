@@ -59,6 +59,10 @@ def lib():
         L.oracle_fod_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]
         L.oracle_xc_kernel.argtypes = [C.c_int, _dp, _dp]
         L.oracle_stage_times.argtypes = [_dp]
+        L.oracle_flo_coef.restype = C.c_int
+        L.oracle_flo_coef.argtypes = [C.POINTER(ObBasis), C.POINTER(ObSpin), _dp]
+        L.oracle_lagmult.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_sic_hamiltonian.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_nbas.restype = C.c_int
         L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
         _lib = L
@@ -209,3 +213,55 @@ def evaluate(sysm, nthreads: int = 0, want_potentials: bool = False, want_vab: b
             debdax=o["debdax"], grad=o["grad"], vpot=o["vpot"], vh=o["vh"])
         res["spins"].append(d)
     return res
+
+
+def flo_coefs(sysm, ispn: int, fods=None) -> np.ndarray:
+    """FLO AO coefficients (nbas x nocc) of spin ispn (1 or 2)."""
+    h = _BasisHold(sysm.basis)
+    C_ = np.asfortranarray(sysm.coef[ispn - 1], np.float64)
+    fod = np.ascontiguousarray((fods or sysm.fod)[ispn - 1], np.float64)
+    sp = ObSpin()
+    sp.nwf = C_.shape[1]
+    sp.nocc = fod.shape[0]
+    sp.coef = _d(C_)
+    sp.fod = _d(fod)
+    out = np.zeros((fod.shape[0], sysm.basis.nbas))  # row f = FLO f (column-major nbas x n)
+    st = lib().oracle_flo_coef(C.byref(h.s), C.byref(sp), _d(out))
+    if st:
+        raise RuntimeError(f"oracle_flo_coef status {st}")
+    return out.T.copy()
+
+
+def lagmult(sysm, ispn: int, pot) -> np.ndarray:
+    """LAGMULT_SLV: [j, k] = <phi_j| -1/2 lap + pot |phi_k> over the FLOs of spin ispn, on the mesh."""
+    cf = np.asfortranarray(flo_coefs(sysm, ispn))
+    h = _BasisHold(sysm.basis)
+    pts = np.ascontiguousarray(sysm.mesh.xyz, np.float64)
+    w = np.ascontiguousarray(sysm.mesh.w, np.float64)
+    n = cf.shape[1]
+    lam = np.zeros(n * n)
+    lib().oracle_lagmult(C.byref(h.s), w.size, _d(pts), _d(w), n, _d(cf), _d(np.ascontiguousarray(pot, np.float64)),
+                         _d(lam))
+    return lam.reshape(n, n).T.copy()
+
+
+def sic_hamiltonian(sysm, ispn: int, vpot) -> np.ndarray:
+    """APATCH (MM=2): H_ab = sum_i 1/2 (<chi_a|V_i|phi_i><phi_i|chi_b> + (a <-> b)), both brackets on the mesh;
+    vpot[i] = V_i on the mesh (n x npts)."""
+    cf = np.asfortranarray(flo_coefs(sysm, ispn))
+    h = _BasisHold(sysm.basis)
+    pts = np.ascontiguousarray(sysm.mesh.xyz, np.float64)
+    w = np.ascontiguousarray(sysm.mesh.w, np.float64)
+    n = cf.shape[1]
+    N = sysm.basis.nbas
+    H = np.zeros(N * N)
+    lib().oracle_sic_hamiltonian(C.byref(h.s), w.size, _d(pts), _d(w), n, _d(cf),
+                                 _d(np.ascontiguousarray(vpot, np.float64)), _d(H))
+    return H.reshape(N, N).T.copy()
+
+
+def flo_fields(sysm, ispn: int):
+    """FLO values on the mesh, [f, k]."""
+    cf = flo_coefs(sysm, ispn)
+    chi = basis_eval(sysm.basis, sysm.mesh.xyz, 0)[0]
+    return (chi @ cf).T.copy()

@@ -106,7 +106,10 @@ struct flosic_ctx {
     DevBuf<ChunkDev> d_chunks, d_chunks_bf16;
     int nchunks = 0, nchunks_bf16 = 0;
     int64_t rows_padded = 0, rows_padded_bf16 = 0;
-    DevBuf<float> d_H32;
+    DevBuf<double> d_lapplane, d_lap, d_pot, d_wop, d_small, d_B, d_hao;  // LAGMULT_SLV / AO SIC Hamiltonian
+    bool lapplane_ok = false;
+    DevBuf<float> d_H32;   // H in FP32 MFMA operand order
+    DevBuf<double> d_H64;  // H in FP64 MFMA operand order
     DevBuf<__bf16> d_Hs;
     DevBuf<double4> d_pairP;
     DevBuf<double> d_btab;
@@ -819,6 +822,32 @@ int run_frmorb(flosic_ctx *c) {
     return 0;
 }
 
+// out (rows x n, column-major) = A^T B with A (M x rows) and B (M x n) column-major, leading dimension M: the
+// point dimension is split into slices (the products are skinny with K = M) and the slice partials are summed
+// in a fixed order
+int gemm_tn_points(flosic_ctx *c, int rows, int n, const double *A, const double *B, double *out) {
+    const int64_t M = c->M;
+    if (rows <= 0 || n <= 0) return 0;
+    const double one = 1.0, zero = 0.0;
+    int nsl = 64;
+    while (nsl > 1 && M / nsl < 4096) --nsl;
+    const int64_t ms = M / nsl, rest = M - ms * nsl;  // nsl slices of ms points, then the remainder
+    ALLOC(c, c->d_Gpart, (size_t)(nsl + 1) * rows * n);
+    BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, rows, n,
+                                             (rocblas_int)ms, &one, A, (rocblas_int)M, ms, B, (rocblas_int)M, ms,
+                                             &zero, c->d_Gpart.p, rows, (rocblas_stride)rows * n, nsl));
+    int nparts = nsl;
+    if (rest > 0) {
+        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, rows, n,
+                                 (rocblas_int)rest, &one, A + ms * nsl, (rocblas_int)M, B + ms * nsl, (rocblas_int)M,
+                                 &zero, c->d_Gpart.p + (size_t)nsl * rows * n, rows));
+        nparts = nsl + 1;
+    }
+    launch_reduce_partials(c->d_Gpart.p, 1, nparts, rows * n, out, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 int evaluate(flosic_ctx *c) {
     int r = check_ready(c);
     if (r) return r;
@@ -879,24 +908,27 @@ int evaluate(flosic_ctx *c) {
                            c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else if (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) {
             const int np = c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2;
+            const int64_t nsteps = c->rows_padded_bf16 / 32;
             ALLOC(c, c->d_Hs, (size_t)np * c->ldh * c->rows_padded_bf16);
-            launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, c->rows_padded_bf16, c->d_Hs.p,
-                           c->stream);
+            launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, nsteps, c->d_Hs.p, c->stream);
             HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb_bf16(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_pairP.p, c->d_Hs.p, c->rows_padded_bf16,
-                                c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            launch_coulomb_bf16(np, c->d_chunks_bf16.p, c->nchunks_bf16, nsteps, c->d_pairP.p, c->d_Hs.p, c->ldh,
+                                c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else if (c->prec == FLOSIC_PREC_MIXED) {
-            const size_t hcnt = (size_t)c->nrows * c->ldh;
-            ALLOC(c, c->d_H32, hcnt);
-            launch_convert_f32(c->d_H.p, c->d_H32.p, (int64_t)hcnt, c->stream);
+            ALLOC(c, c->d_H32, (size_t)c->rows_padded * c->ldh);
+            launch_frag_h<float>(c->d_chunks.p, c->nchunks, c->d_H.p, c->ldh, nF, c->d_H32.p, c->stream);
             HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H32.p, c->ldh, c->d_x.p, c->d_y.p,
-                                       c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
+                                       c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
         } else {
+            ALLOC(c, c->d_H64, (size_t)c->rows_padded * c->ldh);
+            launch_frag_h<double>(c->d_chunks.p, c->nchunks, c->d_H.p, c->ldh, nF, c->d_H64.p, c->stream);
             HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->d_pairP.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p,
-                                        c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H64.p,
+                                        c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M,
+                                        c->stream);
         }
+        HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     // 5. XC, total potential, energies
@@ -919,25 +951,8 @@ int evaluate(flosic_ctx *c) {
     for (int s = 0; s < 2; ++s) lam_cnt += (size_t)c->sp[s].nwf * c->sp[s].nocc;
     ALLOC(c, c->d_lambda, lam_cnt);
     {
-        const double one = 1.0, zero = 0.0, mone = -1.0;
-        int nsl = 64;
-        while (nsl > 1 && M / nsl < 4096) --nsl;
-        const int64_t ms = M / nsl, rest = M - ms * nsl;  // nsl slices of ms points, then the remainder
-        ALLOC(c, c->d_Gpart, (size_t)(nsl + 1) * N * nF);
-        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF,
-                                                 (rocblas_int)ms, &one, c->d_planes.p, (rocblas_int)M, ms, c->d_U.p,
-                                                 (rocblas_int)M, ms, &zero, c->d_Gpart.p, N, (rocblas_stride)N * nF,
-                                                 nsl));
-        int nparts = nsl;
-        if (rest > 0) {
-            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, N, nF,
-                                     (rocblas_int)rest, &one, c->d_planes.p + ms * nsl, (rocblas_int)M,
-                                     c->d_U.p + ms * nsl, (rocblas_int)M, &zero, c->d_Gpart.p + (size_t)nsl * N * nF,
-                                     N));
-            nparts = nsl + 1;
-        }
-        launch_reduce_partials(c->d_Gpart.p, 1, nparts, N * nF, c->d_G.p, c->stream);
-        HIPCHK(c, hipGetLastError());
+        const double zero = 0.0, mone = -1.0;
+        if ((r = gemm_tn_points(c, N, nF, c->d_planes.p, c->d_U.p, c->d_G.p))) return r;
         size_t lo = 0;
         for (int s = 0; s < 2; ++s) {
             Spin &sp = c->sp[s];
@@ -1155,6 +1170,7 @@ int32_t flosic_set_basis(flosic_ctx *c, int32_t natom, const double *xyz, const 
     int r = 0;
     if ((r = build_basis_device(c)) || (r = build_pairs(c))) return r;
     c->planes_ok = false;
+    c->lapplane_ok = false;
     c->evaluated = false;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_basis sync");
     return FLOSIC_OK;
@@ -1184,6 +1200,7 @@ int32_t flosic_set_mesh(flosic_ctx *c, int64_t nmsh, const double *rmsh, const d
         (r = upload(c, c->d_w, w)))
         return r;
     c->planes_ok = false;
+    c->lapplane_ok = false;
     c->evaluated = false;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_mesh sync");
     return FLOSIC_OK;
@@ -1330,6 +1347,83 @@ int32_t flosic_get_potential(flosic_ctx *c, int32_t ispn, int32_t iorb, double *
     const int f = (s == 0 ? 0 : c->sp[0].nocc) + iorb - 1;
     if (pot) HIPCHK(c, hipMemcpy(pot, c->d_Vtot.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
     if (vh) HIPCHK(c, hipMemcpy(vh, c->d_VH.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    return FLOSIC_OK;
+}
+
+int32_t flosic_lagmult_slv(flosic_ctx *c, int32_t ispn, const double *pot, int32_t ldsic, double *sic) {
+    if (!c || !pot || !sic) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    int r = need_eval(c);
+    if (r) return r;
+    const int n = c->sp[s].nocc, N = c->nbas, nF = c->nF;
+    const int64_t M = c->M;
+    if (ldsic < n) return fail(c, FLOSIC_EINVAL, "ldsic %d < nfrm %d", ldsic, n);
+    if (n == 0) return FLOSIC_OK;
+    const int off = s == 0 ? 0 : c->sp[0].nocc;
+    const double one = 1.0, zero = 0.0;
+    // Laplacians of this spin's FLOs on the mesh: from the Hessian planes (PBE) or a Laplacian plane built once
+    ALLOC(c, c->d_lap, (size_t)n * M);
+    const double *cf = c->d_cflo.p + (size_t)off * N;
+    if (c->planes_nd == 10) {
+        for (int d = 4; d < 7; ++d)
+            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, N, &one,
+                                     c->d_planes.p + (size_t)d * N * M, (rocblas_int)M, cf, N, d == 4 ? &zero : &one,
+                                     c->d_lap.p, (rocblas_int)M));
+    } else {
+        if (!c->lapplane_ok) {
+            ALLOC(c, c->d_lapplane, (size_t)N * M);
+            HIPCHK(c, hipMemsetAsync(c->d_lapplane.p, 0, sizeof(double) * N * M, c->stream));
+            launch_basis_planes<double>(basis_view(c), c->d_x.p, c->d_y.p, c->d_z.p, M, -1, c->d_lapplane.p, M,
+                                        c->stream);
+            HIPCHK(c, hipGetLastError());
+            c->lapplane_ok = true;
+        }
+        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, N, &one,
+                                 c->d_lapplane.p, (rocblas_int)M, cf, N, &zero, c->d_lap.p, (rocblas_int)M));
+    }
+    // T_k = w (-1/2 lap phi_k + V phi_k), L = phi^T T
+    if ((r = upload(c, c->d_pot, pot, (size_t)M))) return r;
+    ALLOC(c, c->d_wop, (size_t)n * M);
+    launch_weighted_op(n, M, c->d_w.p, c->d_pot.p, c->d_phi.p + (size_t)off * M, c->d_lap.p, -0.5, c->d_wop.p,
+                       c->stream);
+    HIPCHK(c, hipGetLastError());
+    ALLOC(c, c->d_small, (size_t)n * n);
+    if ((r = gemm_tn_points(c, n, n, c->d_phi.p + (size_t)off * M, c->d_wop.p, c->d_small.p))) return r;
+    std::vector<double> L((size_t)n * n);
+    HIPCHK(c, hipMemcpyAsync(L.data(), c->d_small.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j) sic[j + (size_t)ldsic * k] += L[j + (size_t)n * k];
+    (void)nF;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_sic_hamiltonian(flosic_ctx *c, int32_t ispn, double *hao) {
+    if (!c || !hao) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    int r = need_eval(c);
+    if (r) return r;
+    const int n = c->sp[s].nocc, N = c->nbas;
+    const int64_t M = c->M;
+    std::fill(hao, hao + (size_t)N * N, 0.0);
+    if (n == 0) return FLOSIC_OK;
+    const int off = s == 0 ? 0 : c->sp[0].nocc;
+    // B_ai = <chi_a|phi_i> and G_ai = <chi_a|V_i|phi_i> on the mesh (G from the evaluation)
+    ALLOC(c, c->d_wop, (size_t)n * M);
+    launch_weighted_op(n, M, c->d_w.p, nullptr, c->d_phi.p + (size_t)off * M, nullptr, 0.0, c->d_wop.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    ALLOC(c, c->d_B, (size_t)N * n);
+    if ((r = gemm_tn_points(c, N, n, c->d_planes.p, c->d_wop.p, c->d_B.p))) return r;
+    ALLOC(c, c->d_hao, (size_t)N * N);
+    const double half = 0.5, zero = 0.0;
+    BLASCHK(c, rocblas_dsyr2k(c->blas, rocblas_fill_upper, rocblas_operation_none, N, n, &half,
+                              c->d_G.p + (size_t)off * N, N, c->d_B.p, N, &zero, c->d_hao.p, N));
+    HIPCHK(c, hipMemcpyAsync(hao, c->d_hao.p, sizeof(double) * N * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < N; ++b)
+        for (int a = b + 1; a < N; ++a) hao[a + (size_t)N * b] = hao[b + (size_t)N * a];
     return FLOSIC_OK;
 }
 

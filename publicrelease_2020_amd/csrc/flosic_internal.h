@@ -56,9 +56,14 @@ int coulomb_cols(int nF);
 // per-phase launch helpers (stream-ordered)
 void boys_table(double *h_tab);  // host: fills kBoysRows x kBoysCols
 
+// nd = 1 (values), 4 (+ gradient), 10 (+ Hessian xx,yy,zz,xy,xz,yz) planes, or -1 (one Laplacian plane)
 template <typename T>
 void launch_basis_planes(const BasisDev &B, const double *x, const double *y, const double *z, int64_t M, int nd,
                          T *phi, int64_t ldphi, hipStream_t s);
+
+// T[f][k] = w_k (a * lap[f][k] + pot[k] * phi[f][k]); lap may be null, pot null means 1
+void launch_weighted_op(int n, int64_t M, const double *w, const double *pot, const double *phi, const double *lap,
+                        double a, double *T, hipStream_t s);
 
 void launch_prim_coef(int nps, const int *ps_f0, const int *ps_stride, const int *ps_ncart, const int *ps_c0,
                       const int *ps_nc, const double *ps_coef, const double *cflo, int nF, int ldc, double *dps,
@@ -73,22 +78,24 @@ void launch_coulomb(const int *segL /*6 offsets*/, const double4 *pairP, const i
                     int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
                     int nF, double *V, int64_t ldv, hipStream_t s);
 
-// MFMA form: R_tuv of a chunk staged in LDS once per (point, pair) and contracted with the chunk's H rows
-// for 64 FLO columns per block.  T = double (FP64 MFMA) or float (FP32 MFMA, FP64 sums per chunk).
+// MFMA forms.  H is first rearranged into MFMA operand order over the chunk list's zero-extended rows
+// (nsteps K steps: rows/4 for the FP64/FP32 list, rows/32 for the bf16 list); the Coulomb kernels then generate
+// R_tuv of every (point, group) once and stream H.  T = double (FP64 MFMA) or float (FP32 MFMA, FP64 sums per
+// chunk).
 template <typename T>
-void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
-                         const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
-                         double *V, int64_t ldv, hipStream_t s);
-
-void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s);
+void launch_frag_h(const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, T *Hf, hipStream_t s);
+template <typename T>
+void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP, const T *Hf,
+                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
+                         int nF, double *V, int64_t ldv, hipStream_t s);
 
 // split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,
 // FP32 accumulation inside a chunk, FP64 sums across chunks.  H is split once per evaluation into
-// Hs[piece][ldh columns][ldp padded rows] (ldp = total zero-extended rows of the bf16 chunk list).
-void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, int64_t ldp,
-                    __bf16 *Hs, hipStream_t s);
-void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const __bf16 *Hs,
-                         int64_t ldp, int ldh, const double *x, const double *y, const double *z, int64_t M,
+// Hs[piece][nsteps][ldh/16][64 lanes][8].
+void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF,
+                    int64_t nsteps, __bf16 *Hs, hipStream_t s);
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
+                         const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
                          const double *btab, int nF, double *V, int64_t ldv, hipStream_t s);
 
 // XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)

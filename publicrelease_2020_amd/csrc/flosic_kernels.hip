@@ -68,15 +68,22 @@ __device__ __forceinline__ void cart_eval(double x, double y, double z, double S
     }
 }
 
+// ND = 1, 4 or 10 planes (value, gradient, Hessian); ND = -1: one plane holding the Laplacian
 template <typename T, int ND, int L, int M0>
 __device__ __forceinline__ void store_comps(double x, double y, double z, double S0, double S1, double S2, T *phi,
                                             int64_t ldphi, int64_t plane, int f0, int stride, int64_t k) {
     if constexpr (M0 < ncart(L)) {
-        double o[ND];
-        cart_eval<loff(L) + M0, ND>(x, y, z, S0, S1, S2, o);
         const int64_t col = (int64_t)(f0 + M0 * stride) * ldphi + k;
+        if constexpr (ND < 0) {
+            double o[10];
+            cart_eval<loff(L) + M0, 10>(x, y, z, S0, S1, S2, o);
+            phi[col] = (T)(o[4] + o[5] + o[6]);
+        } else {
+            double o[ND];
+            cart_eval<loff(L) + M0, ND>(x, y, z, S0, S1, S2, o);
 #pragma unroll
-        for (int d = 0; d < ND; ++d) phi[d * plane + col] = (T)o[d];
+            for (int d = 0; d < ND; ++d) phi[d * plane + col] = (T)o[d];
+        }
         store_comps<T, ND, L, M0 + 1>(x, y, z, S0, S1, S2, phi, ldphi, plane, f0, stride, k);
     }
 }
@@ -124,12 +131,33 @@ void launch_basis_planes(const BasisDev &B, const double *x, const double *y, co
     const dim3 grid((unsigned)((M + 255) / 256)), blk(256);
     if (nd == 1) hipLaunchKernelGGL((k_basis_planes<T, 1>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
     else if (nd == 4) hipLaunchKernelGGL((k_basis_planes<T, 4>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
+    else if (nd == -1) hipLaunchKernelGGL((k_basis_planes<T, -1>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
     else hipLaunchKernelGGL((k_basis_planes<T, 10>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
 }
 template void launch_basis_planes<double>(const BasisDev &, const double *, const double *, const double *, int64_t,
                                           int, double *, int64_t, hipStream_t);
 template void launch_basis_planes<float>(const BasisDev &, const double *, const double *, const double *, int64_t,
                                          int, float *, int64_t, hipStream_t);
+
+// T[f][k] = w_k (a * lap[f][k] + pot[k] * phi[f][k]) over n FLOs (lap may be null: T = w * pot * phi, or with
+// pot null: T = w * phi)
+__global__ void k_weighted_op(int n, int64_t M, const double *__restrict__ w, const double *__restrict__ pot,
+                              const double *__restrict__ phi, const double *__restrict__ lap, double a,
+                              double *__restrict__ T) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * M) return;
+    const int64_t k = e % M;
+    double v = pot ? pot[k] * phi[e] : phi[e];
+    if (lap) v += a * lap[e];
+    T[e] = w[k] * v;
+}
+void launch_weighted_op(int n, int64_t M, const double *w, const double *pot, const double *phi, const double *lap,
+                        double a, double *T, hipStream_t s) {
+    const int64_t cnt = (int64_t)n * M;
+    if (cnt <= 0) return;
+    hipLaunchKernelGGL(k_weighted_op, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, n, M, w, pot, phi, lap, a,
+                       T);
+}
 
 // ---------------------------------------------------------------------------------------------------
 // FLO coefficients on primitive shells: dps[(ps*6 + m)*nF + f] = sum_c con_c * cflo[f*ldc + f0 + m*stride + c]
@@ -397,16 +425,35 @@ void launch_coulomb(const int *segL, const double4 *pairP, const int *pairRow, c
 // ---------------------------------------------------------------------------------------------------
 // MFMA Coulomb.  V[f][k] = sum_rows R[k][row] H[row][f]: R_tuv(p, P - r_k) is generated on the fly (FP64 Boys
 // and Hermite recursion), H is the Hermite-coefficient matrix of every exponent-pair group for every FLO.
-// Block = 8 wavefronts, tile of TP points x NFB FLO columns (NFB covers all FLOs up to 256, so R is computed
-// once per (point, group)); wave (wr, wc) owns points [wr*TP/2, (wr+1)*TP/2) and columns [wc*NFB/4,
-// (wc+1)*NFB/4), so every lane holds TP*NFB/512 accumulators.  Per chunk of groups:
-//   1. R of every (group, point) of the chunk into LDS,
-//   2. the chunk's H rows into LDS, HR rows at a time,
-//   3. 16x16x4 MFMAs into register accumulators; FP32 accumulators are added into FP64 sums after every chunk.
+// Block = 12 wavefronts on a tile of TP = 64 points x NFB FLO columns (NFB covers all FLOs up to 256, so R is
+// computed once per (point, group)):
+//   * waves 0-3 (producers) generate R of chunk c+1 into one of two LDS buffers,
+//   * waves 4-11 (consumers, 2 point halves x 4 column quarters) run the MFMAs of chunk c from the other
+//     buffer, with their H fragments streamed from global memory in MFMA operand order (one coalesced load
+//     per fragment, prefetched one K step ahead across chunk boundaries),
+//   * one barrier per chunk swaps the buffers.
+// FP32 and split-BF16 accumulators are added into FP64 sums after every chunk.
 // ---------------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-constexpr int kCoulThreads = 512;  // 8 wavefronts: 2 point halves x 4 column quarters
+constexpr int kProducers = 256;            // 4 wavefronts
+constexpr int kCoulThreads = 256 + 512;    // + 8 consumer wavefronts
+
+// one wave64 fragment of H in operand order: lane l holds row 4s + l/16 and column 16*ct + l%16 (FP64/FP32),
+// or rows 32s + 8*(l/16) .. +7 and column 16*ct + l%16 (bf16, 16 bytes per lane)
+template <typename T>
+__global__ void k_frag_h(const ChunkDev *__restrict__ chunks, const double *__restrict__ H, int ldh, int nF,
+                         T *__restrict__ Hf) {
+    const ChunkDev ch = chunks[blockIdx.x];
+    const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
+    const int nct = ldh / 16;
+    for (int e = threadIdx.x; e < ch.kr * ldh; e += blockDim.x) {
+        const int r = e / ldh, f = e - r * ldh;
+        const double v = (r < nreal && f < nF) ? H[(int64_t)(ch.hrow0 + r) * ldh + f] : 0.0;
+        const int64_t s = (ch.prow0 + r) / 4;
+        Hf[(s * nct + f / 16) * 64 + (r % 4) * 16 + f % 16] = (T)v;
+    }
+}
 
 template <typename T> struct MfmaTraits;
 // Operands of both 16x16x4 forms: lane l holds A[l%16][l/16] and B[l/16][l%16].  The accumulators differ:
@@ -426,14 +473,16 @@ template <> struct MfmaTraits<float> {
     static __device__ __forceinline__ int row(int lr, int r) { return 4 * lr + r; }
 };
 
-// R of chunk `ch` for points k0 .. k0+TP-1 into Rl[row][point] (rows beyond the chunk's groups are zero)
+// R of chunk `ch` for points k0 .. k0+TP-1 into Rl[row][point] by the producer threads (ptid < kProducers);
+// rows beyond the chunk's groups are zero
 template <int L, typename T, int TP, int LDR>
-__device__ __forceinline__ void chunk_R(const ChunkDev &ch, const double4 *__restrict__ pP, const double *__restrict__ X,
-                                        const double *__restrict__ Y, const double *__restrict__ Z, int64_t k0,
-                                        int64_t M, const double *__restrict__ tab, T *Rl) {
+__device__ __forceinline__ void chunk_R(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
+                                        const double *__restrict__ X, const double *__restrict__ Y,
+                                        const double *__restrict__ Z, int64_t k0, int64_t M,
+                                        const double *__restrict__ tab, T *Rl) {
     constexpr int nt = ntuv(L);
     const int np = ch.pe - ch.pb;
-    for (int task = threadIdx.x; task < np * TP; task += kCoulThreads) {
+    for (int task = ptid; task < np * TP; task += kProducers) {
         const int pl = task / TP, pt = task - pl * TP;
         const int64_t k = k0 + pt;
         double R[nt];
@@ -447,33 +496,58 @@ __device__ __forceinline__ void chunk_R(const ChunkDev &ch, const double4 *__res
 #pragma unroll
         for (int t = 0; t < nt; ++t) Rl[(pl * nt + t) * LDR + pt] = (T)R[t];
     }
-    for (int e = threadIdx.x; e < (ch.kr - np * nt) * TP; e += kCoulThreads) {
+    for (int e = ptid; e < (ch.kr - np * nt) * TP; e += kProducers) {
         const int r = np * nt + e / TP, pt = e % TP;
         Rl[r * LDR + pt] = (T)0;
     }
 }
 
-template <typename T, int TP, int NFB, int HR>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_tile(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                       const double4 *__restrict__ pP, const T *__restrict__ H, int ldh,
-                                                       const double *__restrict__ X, const double *__restrict__ Y,
-                                                       const double *__restrict__ Z, int64_t M,
-                                                       const double *__restrict__ tab, int nF, double *__restrict__ V,
-                                                       int64_t ldv) {
-    constexpr int NRT = TP / 32, NCW = NFB / 64;  // 16x16 tiles per wave: points, columns
-    constexpr int PAD = sizeof(T) == 8 ? 8 : 16;  // row shift of 16 banks between the 4 row groups of a wave
-    constexpr int LDR = TP + PAD, LDH = NFB + PAD;
-    static_assert(TP % 32 == 0 && NFB % 64 == 0 && HR % 4 == 0 && kChunkRows % HR == 0, "tile shape");
-    __shared__ T Rl[kChunkRows * LDR];
-    __shared__ T Hl[HR * LDH];
+template <typename T, int TP, int LDR>
+__device__ __forceinline__ void produce_R(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
+                                          const double *__restrict__ X, const double *__restrict__ Y,
+                                          const double *__restrict__ Z, int64_t k0, int64_t M,
+                                          const double *__restrict__ tab, T *Rl) {
+    switch (ch.L) {
+        case 0: chunk_R<0, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+        case 1: chunk_R<1, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+        case 2: chunk_R<2, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+        case 3: chunk_R<3, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+        default: chunk_R<4, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+    }
+}
+
+template <typename T, int NFB>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                              int64_t nsteps, const double4 *__restrict__ pP,
+                                                              const T *__restrict__ Hf, int nct,
+                                                              const double *__restrict__ X,
+                                                              const double *__restrict__ Y,
+                                                              const double *__restrict__ Z, int64_t M,
+                                                              const double *__restrict__ tab, int nF,
+                                                              double *__restrict__ V, int64_t ldv) {
+    constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns
+    constexpr int PAD = sizeof(T) == 8 ? 8 : 16;
+    constexpr int LDR = TP + PAD;
+    __shared__ T Rl[2][kChunkRows * LDR];
     using Tr = MfmaTraits<T>;
     using acc_t = typename Tr::acc;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int lr = lane >> 4, lc = lane & 15;
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int fb = blockIdx.y * NFB;
-    const int pw = (wave >> 2) * NRT * 16, cw = (wave & 3) * NCW * 16;
+    const int wave = threadIdx.x >> 6;
+    if (wave < 4) {  // producers
+        const int ptid = threadIdx.x;
+        produce_R<T, TP, LDR>(ptid, chunks[0], pP, X, Y, Z, k0, M, tab, Rl[0]);
+        __syncthreads();
+        for (int c = 0; c < nchunks; ++c) {
+            if (c + 1 < nchunks)
+                produce_R<T, TP, LDR>(ptid, chunks[c + 1], pP, X, Y, Z, k0, M, tab, Rl[(c + 1) & 1]);
+            __syncthreads();
+        }
+        return;
+    }
+    const int cwv = wave - 4;
+    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+    const int pw = (cwv >> 2) * NRT * 16;                       // first point of this wave
+    const int ct0 = (blockIdx.y * NFB + (cwv & 3) * NCW * 16) / 16;  // first column tile
     acc_t acc[NRT][NCW];
     double sum[NRT][NCW][4];
 #pragma unroll
@@ -485,37 +559,32 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_tile(const ChunkDev *_
                 acc[i][j][r] = (T)0;
                 sum[i][j][r] = 0.0;
             }
+    // H fragments of global K step g are at Hf[(g*nct + ct)*64 + lane]; the stream is contiguous over chunks
+    T bn[NCW];
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) bn[j] = Hf[(int64_t)(ct0 + j) * 64 + lane];
+    int64_t g = 0;
+    __syncthreads();
     for (int c = 0; c < nchunks; ++c) {
-        const ChunkDev ch = chunks[c];
-        switch (ch.L) {
-            case 0: chunk_R<0, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-            case 1: chunk_R<1, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-            case 2: chunk_R<2, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-            case 3: chunk_R<3, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-            default: chunk_R<4, T, TP, LDR>(ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-        }
-        const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
-        for (int h0 = 0; h0 < ch.kr; h0 += HR) {
-            const int hr = min(HR, ch.kr - h0);
-            if (h0 > 0) __syncthreads();  // the previous H stage has been consumed
-            for (int e = threadIdx.x; e < hr * NFB; e += kCoulThreads) {
-                const int r = e / NFB, f = e - r * NFB;
-                Hl[r * LDH + f] = h0 + r < nreal ? H[(int64_t)(ch.hrow0 + h0 + r) * ldh + fb + f] : (T)0;
+        const int ns = chunks[c].kr / 4;
+        const T *R = Rl[c & 1];
+        for (int s = 0; s < ns; ++s, ++g) {
+            T b[NCW];
+#pragma unroll
+            for (int j = 0; j < NCW; ++j) b[j] = bn[j];
+            if (g + 1 < nsteps) {
+#pragma unroll
+                for (int j = 0; j < NCW; ++j) bn[j] = Hf[((g + 1) * nct + ct0 + j) * 64 + lane];
             }
-            __syncthreads();
-            for (int ks = 0; ks < hr; ks += 4) {
-                T a[NRT], b[NCW];
+            T a[NRT];
 #pragma unroll
-                for (int i = 0; i < NRT; ++i) a[i] = Rl[(h0 + ks + lr) * LDR + pw + i * 16 + lc];
+            for (int i = 0; i < NRT; ++i) a[i] = R[(4 * s + lr) * LDR + pw + i * 16 + lc];
 #pragma unroll
-                for (int j = 0; j < NCW; ++j) b[j] = Hl[(ks + lr) * LDH + cw + j * 16 + lc];
+            for (int i = 0; i < NRT; ++i)
 #pragma unroll
-                for (int i = 0; i < NRT; ++i)
-#pragma unroll
-                    for (int j = 0; j < NCW; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
-            }
+                for (int j = 0; j < NCW; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
         }
-        if constexpr (sizeof(T) == 4) {  // FP64 sum of each chunk's FP32 partial
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
             for (int i = 0; i < NRT; ++i)
 #pragma unroll
@@ -526,13 +595,13 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_tile(const ChunkDev *_
                         acc[i][j][r] = 0.0f;
                     }
         }
-        __syncthreads();  // Rl and Hl are rewritten by the next chunk
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < NRT; ++i)
 #pragma unroll
         for (int j = 0; j < NCW; ++j) {
-            const int f = fb + cw + j * 16 + lc;
+            const int f = (ct0 + j) * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t k = k0 + pw + i * 16 + Tr::row(lr, r);
@@ -542,50 +611,14 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_tile(const ChunkDev *_
         }
 }
 
-// FLO columns per block: all of them up to 256 (ldh is nF rounded up to a multiple of this)
-int coulomb_cols(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 256; }
-
-template <typename T, int TP, int HR>
-static void launch_tile(int nfb, const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
-                        const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
-                        double *V, int64_t ldv, hipStream_t s) {
-    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)(ldh / nfb)), blk(kCoulThreads);
-#define FLOSIC_TILE(NFB)                                                                                            \
-    hipLaunchKernelGGL((k_coulomb_tile<T, TP, NFB, HR>), grid, blk, 0, s, chunks, nchunks, pairP, H, ldh, x, y, z, M, \
-                       btab, nF, V, ldv)
-    switch (nfb) {
-        case 64: FLOSIC_TILE(64); break;
-        case 128: FLOSIC_TILE(128); break;
-        case 192: FLOSIC_TILE(192); break;
-        default: FLOSIC_TILE(256); break;
-    }
-#undef FLOSIC_TILE
-}
-
-template <typename T>
-void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, const double4 *pairP, const T *H, int ldh,
-                         const double *x, const double *y, const double *z, int64_t M, const double *btab, int nF,
-                         double *V, int64_t ldv, hipStream_t s) {
-    // LDS per block, NFB = 192: FP64 36 KB R + 26 KB H; FP32 20 KB R + 27 KB H
-    constexpr int TP = 64, HR = sizeof(T) == 8 ? 16 : 32;
-    launch_tile<T, TP, HR>(coulomb_cols(nF), chunks, nchunks, pairP, H, ldh, x, y, z, M, btab, nF, V, ldv, s);
-}
-template void launch_coulomb_mfma<double>(const ChunkDev *, int, const double4 *, const double *, int, const double *,
-                                          const double *, const double *, int64_t, const double *, int, double *,
-                                          int64_t, hipStream_t);
-template void launch_coulomb_mfma<float>(const ChunkDev *, int, const double4 *, const float *, int, const double *,
-                                         const double *, const double *, int64_t, const double *, int, double *,
-                                         int64_t, hipStream_t);
-
 // ---------------------------------------------------------------------------------------------------
 // Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces (each the rounded remainder of the previous), so the
 // product x*y is carried by x1y1 + x1y2 + x2y1 (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error,
-// every term on the 16x16x32 bf16 MFMA.  Operand fragments are 8 consecutive Hermite rows of one point /
-// one FLO column, so both tiles are stored row-fastest ([point][row], [column][row]) for 16-byte reads.  H is
-// split once per evaluation into Hs[piece][column][padded row] (k_split_h); R is split as it is generated.
+// every term on the 16x16x32 bf16 MFMA (lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]; FP32
+// register r holds D[4*(l/16) + r][l%16]).  Same block organisation as k_coulomb_ws; R pieces are stored
+// [point][row] so an A operand is one 16-byte LDS read, H pieces come from k_split_h in operand order.
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef short v8s __attribute__((ext_vector_type(8)));
 
 template <int NP>
 __device__ __forceinline__ void split_bf16(double x, __bf16 *out, int64_t stride) {
@@ -598,38 +631,32 @@ __device__ __forceinline__ void split_bf16(double x, __bf16 *out, int64_t stride
     }
 }
 
-// Hs[p][f][prow0 + r] = piece p of H[hrow0 + r][f], zero beyond the chunk's rows and for padding columns;
-// one block per chunk of the bf16 chunk list
+// Hs[p][g][ct][lane][e]: piece p of H row 32g + 8*(lane/16) + e, column 16ct + lane%16 (zero-extended rows and
+// columns); one block per chunk of the bf16 chunk list
 template <int NP>
 __global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__restrict__ H, int ldh, int nF,
-                          int64_t ldp, __bf16 *__restrict__ Hs) {
+                          int64_t nsteps, __bf16 *__restrict__ Hs) {
     const ChunkDev ch = chunks[blockIdx.x];
     const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
+    const int nct = ldh / 16;
     for (int e = threadIdx.x; e < ldh * ch.kr; e += blockDim.x) {
         const int f = e / ch.kr, r = e - f * ch.kr;
         const double v = (r < nreal && f < nF) ? H[(int64_t)(ch.hrow0 + r) * ldh + f] : 0.0;
-        split_bf16<NP>(v, Hs + (int64_t)f * ldp + ch.prow0 + r, (int64_t)ldh * ldp);
+        const int64_t R = ch.prow0 + r;
+        const int64_t idx = ((R / 32) * nct + f / 16) * 512 + (16 * ((R % 32) / 8) + f % 16) * 8 + R % 8;
+        split_bf16<NP>(v, Hs + idx, nsteps * nct * 512);
     }
 }
 
-void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, int64_t ldp,
-                    __bf16 *Hs, hipStream_t s) {
-    if (nchunks <= 0) return;
-    if (npieces == 2)
-        hipLaunchKernelGGL((k_split_h<2>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, ldp, Hs);
-    else
-        hipLaunchKernelGGL((k_split_h<3>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, ldp, Hs);
-}
-
 template <int L, int NP, int TP, int LDK>
-__device__ __forceinline__ void chunk_R_bf16(const ChunkDev &ch, const double4 *__restrict__ pP,
+__device__ __forceinline__ void chunk_R_bf16(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
                                              const double *__restrict__ X, const double *__restrict__ Y,
                                              const double *__restrict__ Z, int64_t k0, int64_t M,
                                              const double *__restrict__ tab, __bf16 *Rb) {
     constexpr int nt = ntuv(L);
     constexpr int PS = TP * LDK;  // piece stride
     const int np = ch.pe - ch.pb;
-    for (int task = threadIdx.x; task < np * TP; task += kCoulThreads) {
+    for (int task = ptid; task < np * TP; task += kProducers) {
         const int pl = task / TP, pt = task - pl * TP;
         const int64_t k = k0 + pt;
         double R[nt];
@@ -644,34 +671,57 @@ __device__ __forceinline__ void chunk_R_bf16(const ChunkDev &ch, const double4 *
         for (int t = 0; t < nt; ++t) split_bf16<NP>(R[t], Rb + pt * LDK + pl * nt + t, PS);
     }
     const int r0 = np * nt;
-    for (int e = threadIdx.x; e < (ch.kr - r0) * TP; e += kCoulThreads) {
+    for (int e = ptid; e < (ch.kr - r0) * TP; e += kProducers) {
         const int pt = e / (ch.kr - r0), r = r0 + e % (ch.kr - r0);
 #pragma unroll
         for (int p = 0; p < NP; ++p) Rb[p * PS + pt * LDK + r] = (__bf16)0.0f;
     }
 }
 
-// Block = 8 wavefronts, TP points x NFB FLO columns, waves laid out as in k_coulomb_tile.  Per chunk: R pieces
-// of all its rows into LDS, then H pieces 32 rows at a time, NP*(NP+1)/2 products per 16x16x32 step.
-template <int NP, int TP, int NFB>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_split(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                        const double4 *__restrict__ pP, const __bf16 *__restrict__ Hs,
-                                                        int64_t ldp, int ldh, const double *__restrict__ X,
-                                                        const double *__restrict__ Y, const double *__restrict__ Z,
-                                                        int64_t M, const double *__restrict__ tab, int nF,
-                                                        double *__restrict__ V, int64_t ldv) {
-    constexpr int NRT = TP / 32, NCW = NFB / 64;
-    constexpr int LDK = kChunkRowsBf16 + 8;  // 144-byte point rows: conflict-free 16-byte fragment reads
-    constexpr int KS = 32, LDKH = KS + 8;    // 80-byte column rows of one staged K step
-    static_assert(TP % 32 == 0 && NFB % 64 == 0, "tile shape");
-    __shared__ __attribute__((aligned(16))) __bf16 Rb[NP * TP * LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 Hb[NP * NFB * LDKH];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int lr = lane >> 4, lc = lane & 15;
+template <int NP, int TP, int LDK>
+__device__ __forceinline__ void produce_R_bf16(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
+                                               const double *__restrict__ X, const double *__restrict__ Y,
+                                               const double *__restrict__ Z, int64_t k0, int64_t M,
+                                               const double *__restrict__ tab, __bf16 *Rb) {
+    switch (ch.L) {
+        case 0: chunk_R_bf16<0, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+        case 1: chunk_R_bf16<1, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+        case 2: chunk_R_bf16<2, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+        case 3: chunk_R_bf16<3, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+        default: chunk_R_bf16<4, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
+    }
+}
+
+template <int NP, int NFB>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                                   int64_t nsteps, const double4 *__restrict__ pP,
+                                                                   const __bf16 *__restrict__ Hs, int nct,
+                                                                   const double *__restrict__ X,
+                                                                   const double *__restrict__ Y,
+                                                                   const double *__restrict__ Z, int64_t M,
+                                                                   const double *__restrict__ tab, int nF,
+                                                                   double *__restrict__ V, int64_t ldv) {
+    constexpr int TP = 64, NRT = 2, NCW = NFB / 64;
+    constexpr int LDK = kChunkRowsBf16 + 8;  // 144-byte point rows: conflict-free 16-byte operand reads
+    __shared__ __attribute__((aligned(16))) __bf16 Rb[2][NP * TP * LDK];
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int fb = blockIdx.y * NFB;
-    const int pw = (wave >> 2) * NRT * 16, cw = (wave & 3) * NCW * 16;
+    const int wave = threadIdx.x >> 6;
+    if (wave < 4) {  // producers
+        const int ptid = threadIdx.x;
+        produce_R_bf16<NP, TP, LDK>(ptid, chunks[0], pP, X, Y, Z, k0, M, tab, Rb[0]);
+        __syncthreads();
+        for (int c = 0; c < nchunks; ++c) {
+            if (c + 1 < nchunks)
+                produce_R_bf16<NP, TP, LDK>(ptid, chunks[c + 1], pP, X, Y, Z, k0, M, tab, Rb[(c + 1) & 1]);
+            __syncthreads();
+        }
+        return;
+    }
+    const int cwv = wave - 4;
+    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+    const int pw = (cwv >> 2) * NRT * 16;
+    const int ct0 = (blockIdx.y * NFB + (cwv & 3) * NCW * 16) / 16;
+    const int64_t pstride = nsteps * nct * 512;
     v4f acc[NRT][NCW];
     double sum[NRT][NCW][4];
 #pragma unroll
@@ -683,45 +733,45 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_split(const ChunkDev *
                 acc[i][j][r] = 0.0f;
                 sum[i][j][r] = 0.0;
             }
+    v8bf bn[NP][NCW];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int j = 0; j < NCW; ++j) bn[p][j] = *(const v8bf *)(Hs + p * pstride + ((int64_t)(ct0 + j) * 64 + lane) * 8);
+    int64_t g = 0;
+    __syncthreads();
     for (int c = 0; c < nchunks; ++c) {
-        const ChunkDev ch = chunks[c];
-        switch (ch.L) {
-            case 0: chunk_R_bf16<0, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-            case 1: chunk_R_bf16<1, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-            case 2: chunk_R_bf16<2, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-            case 3: chunk_R_bf16<3, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-            default: chunk_R_bf16<4, NP, TP, LDK>(ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-        }
-        for (int ks = 0; ks < ch.kr; ks += KS) {
-            if (ks > 0) __syncthreads();  // the previous H stage has been consumed
-            for (int e = threadIdx.x; e < NP * NFB * (KS / 8); e += kCoulThreads) {
-                const int q = e & 3, pf = e >> 2;
-                const int p = pf / NFB, f = pf - p * NFB;
-                const __bf16 *src = Hs + ((int64_t)p * ldh + fb + f) * ldp + ch.prow0 + ks + 8 * q;
-                *(v8s *)(Hb + (p * NFB + f) * LDKH + 8 * q) = *(const v8s *)src;
-            }
-            __syncthreads();
-            v8bf a[NP][NRT];
+        const int ns = chunks[c].kr / 32;
+        const __bf16 *R = Rb[c & 1];
+        for (int s = 0; s < ns; ++s, ++g) {
+            v8bf b[NP][NCW];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int i = 0; i < NRT; ++i)
-                    a[p][i] = *(const v8bf *)(Rb + p * TP * LDK + (pw + i * 16 + lc) * LDK + ks + 8 * lr);
+                for (int j = 0; j < NCW; ++j) b[p][j] = bn[p][j];
+            if (g + 1 < nsteps) {
 #pragma unroll
-            for (int j = 0; j < NCW; ++j) {
-                v8bf b[NP];
+                for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int p = 0; p < NP; ++p) b[p] = *(const v8bf *)(Hb + (p * NFB + cw + j * 16 + lc) * LDKH + 8 * lr);
+                    for (int j = 0; j < NCW; ++j)
+                        bn[p][j] = *(const v8bf *)(Hs + p * pstride + (((g + 1) * nct + ct0 + j) * 64 + lane) * 8);
+            }
 #pragma unroll
-                for (int i = 0; i < NRT; ++i) {
+            for (int i = 0; i < NRT; ++i) {
+                v8bf a[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    a[p] = *(const v8bf *)(R + p * TP * LDK + (pw + i * 16 + lc) * LDK + 32 * s + 8 * lr);
+#pragma unroll
+                for (int j = 0; j < NCW; ++j) {
                     v4f cc = acc[i][j];
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0][j], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1][j], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0][j], cc, 0, 0, 0);
                     if constexpr (NP == 3) {
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2][j], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1][j], cc, 0, 0, 0);
+                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0][j], cc, 0, 0, 0);
                     }
                     acc[i][j] = cc;
                 }
@@ -736,13 +786,13 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_split(const ChunkDev *
                     sum[i][j][r] += (double)acc[i][j][r];
                     acc[i][j][r] = 0.0f;
                 }
-        __syncthreads();  // Rb and Hb are rewritten by the next chunk
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < NRT; ++i)
 #pragma unroll
         for (int j = 0; j < NCW; ++j) {
-            const int f = fb + cw + j * 16 + lc;
+            const int f = (ct0 + j) * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t k = k0 + pw + i * 16 + 4 * lr + r;
@@ -751,38 +801,75 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_split(const ChunkDev *
         }
 }
 
-void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, const double4 *pairP, const __bf16 *Hs,
-                         int64_t ldp, int ldh, const double *x, const double *y, const double *z, int64_t M,
-                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s) {
-    constexpr int TP = 64;
+// FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
+// split-BF16 kernel takes at most 192 (its register budget), 128 beyond that
+int coulomb_cols(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 256; }
+static int coulomb_cols_split(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 128; }
+
+template <typename T>
+void launch_frag_h(const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, T *Hf, hipStream_t s) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL((k_frag_h<T>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, Hf);
+}
+template void launch_frag_h<double>(const ChunkDev *, int, const double *, int, int, double *, hipStream_t);
+template void launch_frag_h<float>(const ChunkDev *, int, const double *, int, int, float *, hipStream_t);
+
+template <typename T>
+void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP, const T *Hf,
+                         int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
+                         int nF, double *V, int64_t ldv, hipStream_t s) {
+    if (nchunks <= 0 || M <= 0) return;
     const int nfb = coulomb_cols(nF);
-    const dim3 grid((unsigned)((M + TP - 1) / TP), (unsigned)(ldh / nfb)), blk(kCoulThreads);
-#define FLOSIC_SPLIT(NP, NFB)                                                                                       \
-    hipLaunchKernelGGL((k_coulomb_split<NP, TP, NFB>), grid, blk, 0, s, chunks, nchunks, pairP, Hs, ldp, ldh, x, y, z, \
-                       M, btab, nF, V, ldv)
-#define FLOSIC_SPLIT_NFB(NP)            \
-    switch (nfb) {                      \
-        case 64: FLOSIC_SPLIT(NP, 64); break;   \
-        case 128: FLOSIC_SPLIT(NP, 128); break; \
-        case 192: FLOSIC_SPLIT(NP, 192); break; \
-        default: FLOSIC_SPLIT(NP, 256); break;  \
+    const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
+#define FLOSIC_WS(NFB)                                                                                            \
+    hipLaunchKernelGGL((k_coulomb_ws<T, NFB>), grid, blk, 0, s, chunks, nchunks, nsteps, pairP, Hf, ldh / 16, x, y, \
+                       z, M, btab, nF, V, ldv)
+    switch (nfb) {
+        case 64: FLOSIC_WS(64); break;
+        case 128: FLOSIC_WS(128); break;
+        case 192: FLOSIC_WS(192); break;
+        default: FLOSIC_WS(256); break;
     }
-    if (npieces == 2) {
-        FLOSIC_SPLIT_NFB(2)
-    } else {
-        FLOSIC_SPLIT_NFB(3)
-    }
-#undef FLOSIC_SPLIT_NFB
-#undef FLOSIC_SPLIT
+#undef FLOSIC_WS
+}
+template void launch_coulomb_mfma<double>(const ChunkDev *, int, int64_t, const double4 *, const double *, int,
+                                          const double *, const double *, const double *, int64_t, const double *, int,
+                                          double *, int64_t, hipStream_t);
+template void launch_coulomb_mfma<float>(const ChunkDev *, int, int64_t, const double4 *, const float *, int,
+                                         const double *, const double *, const double *, int64_t, const double *, int,
+                                         double *, int64_t, hipStream_t);
+
+void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF,
+                    int64_t nsteps, __bf16 *Hs, hipStream_t s) {
+    if (nchunks <= 0) return;
+    if (npieces == 2)
+        hipLaunchKernelGGL((k_split_h<2>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, nsteps, Hs);
+    else
+        hipLaunchKernelGGL((k_split_h<3>), dim3(nchunks), dim3(256), 0, s, chunks, H, ldh, nF, nsteps, Hs);
 }
 
-__global__ void k_convert_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = (float)src[i];
-}
-void launch_convert_f32(const double *src, float *dst, int64_t n, hipStream_t s) {
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
+                         const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s) {
+    if (nchunks <= 0 || M <= 0) return;
+    const int nfb = coulomb_cols_split(nF);
+    const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
+#define FLOSIC_WSB(NP, NFB)                                                                                        \
+    hipLaunchKernelGGL((k_coulomb_ws_bf16<NP, NFB>), grid, blk, 0, s, chunks, nchunks, nsteps, pairP, Hs, ldh / 16, \
+                       x, y, z, M, btab, nF, V, ldv)
+#define FLOSIC_WSB_NFB(NP)                        \
+    switch (nfb) {                                \
+        case 64: FLOSIC_WSB(NP, 64); break;       \
+        case 128: FLOSIC_WSB(NP, 128); break;     \
+        default: FLOSIC_WSB(NP, 192); break;      \
+    }
+    if (npieces == 2) {
+        FLOSIC_WSB_NFB(2)
+    } else {
+        FLOSIC_WSB_NFB(3)
+    }
+#undef FLOSIC_WSB_NFB
+#undef FLOSIC_WSB
 }
 
 // ---------------------------------------------------------------------------------------------------

@@ -63,6 +63,8 @@ def library():
             "flosic_fod_forces": [vp, C.c_int32, _dp],
             "flosic_fod_gradient": [vp, C.c_int32, _dp],
             "flosic_get_potential": [vp, C.c_int32, C.c_int32, _dp, _dp],
+            "flosic_lagmult_slv": [vp, C.c_int32, _dp, C.c_int32, _dp],
+            "flosic_sic_hamiltonian": [vp, C.c_int32, _dp],
             "flosic_get_pairs": [vp, _ip, _ip],
             "flosic_get_timings": [vp, C.c_int32, _dp],
             "flosic_get_counts": [vp, C.c_int32, _lp],
@@ -81,8 +83,8 @@ def exported_symbols() -> list:
     return ["flosic_create", "flosic_destroy", "flosic_last_error", "flosic_set_basis", "flosic_set_mesh",
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
             "flosic_evaluate", "flosic_apotnl_sic", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
-            "flosic_fod_gradient", "flosic_get_potential", "flosic_get_pairs", "flosic_get_timings",
-            "flosic_get_counts"]
+            "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
+            "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts"]
 
 
 def _d(a):
@@ -209,6 +211,22 @@ class Flosic:
         vh = np.zeros(self.npts)
         self._chk(self._L.flosic_get_potential(self._h, ispn, iorb, _d(p), _d(vh)))
         return p, vh
+
+    def lagmult_slv(self, ispn: int, pot, sic=None) -> np.ndarray:
+        """LAGMULT_SLV: sic[j, k] += <phi_j| -1/2 lap + pot |phi_k> over the FLOs of spin ispn (sic defaults to
+        zeros, so the DFT part alone is returned)."""
+        n = self._n[ispn - 1]
+        out = np.zeros((n, n)) if sic is None else np.array(sic, dtype=np.float64)
+        f = np.asfortranarray(out)
+        p = np.ascontiguousarray(pot, np.float64)
+        self._chk(self._L.flosic_lagmult_slv(self._h, ispn, _d(p), f.shape[0], _d(f)))
+        return np.ascontiguousarray(f)
+
+    def sic_hamiltonian(self, ispn: int) -> np.ndarray:
+        """APATCH: H_ab = sum_i 1/2 (<chi_a|V_i|phi_i><phi_i|chi_b> + (a <-> b)) over the FLOs of spin ispn."""
+        h = np.zeros(self.nbas * self.nbas)
+        self._chk(self._L.flosic_sic_hamiltonian(self._h, ispn, _d(h)))
+        return h.reshape(self.nbas, self.nbas).T.copy()
 
     def pairs(self) -> np.ndarray:
         n = C.c_int32()

@@ -225,3 +225,31 @@ def test_bf16x2_error_is_bounded(small_case):
     m.load_system(sysm, F.PREC_BF16X2)
     e = m.evaluate()
     assert abs(e - ref["esic"]) < 1e-4
+
+
+def test_lagmult_slv_parity(small_case):
+    """LAGMULT_SLV: <phi_j| -1/2 lap + V_DFT |phi_k> added to the SIC block (Laplacian plane for LSDA, Hessian
+    planes for PBE) against the oracle, with a synthetic DFT potential on the mesh."""
+    from oracle import oracle as O
+    sysm, ref, ctx, got = small_case
+    r = np.linalg.norm(sysm.mesh.xyz - sysm.mesh.xyz.mean(0), axis=1)
+    pot = -1.5 / np.sqrt(r * r + 0.3) + 0.05 * sysm.mesh.xyz[:, 0]
+    for s in (1, 2):
+        want = O.lagmult(sysm, s, pot)
+        _close(ctx.lagmult_slv(s, pot), want, 1e-10)
+        base = ctx.siclagm(s)
+        n = sysm.nocc[s - 1]
+        out = ctx.lagmult_slv(s, pot, base)
+        _close(out[:n, :n], base[:n, :n] + want, 1e-10)
+        assert np.array_equal(out[n:, :], base[n:, :])
+
+
+def test_sic_hamiltonian_parity(small_case):
+    """APATCH: the AO-basis SIC Hamiltonian against the oracle."""
+    from oracle import oracle as O
+    sysm, ref, ctx, got = small_case
+    for s in (1, 2):
+        want = O.sic_hamiltonian(sysm, s, ref["spins"][s - 1]["vpot"])
+        h = ctx.sic_hamiltonian(s)
+        _close(h, want, 1e-10)
+        assert np.array_equal(h, h.T)

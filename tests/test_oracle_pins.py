@@ -291,3 +291,110 @@ def test_pair_list_follows_screening_rule(tiny_system):
     got = O.pairs(b)
     assert got.shape[0] == len(want)
     assert np.array_equal(got, np.array(want, dtype=np.int32))
+
+
+# ------------------------------------------------------------------------- LAGMULT_SLV / APATCH
+_CART = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
+
+
+def _basis_functions(basis):
+    """(centre, powers, [(alpha, coef)]) per basis function, in the C-ABI order (atom, l, component,
+    contraction)."""
+    out = []
+    for a in range(basis.natom):
+        b0, b1 = basis.bare_off[a], basis.bare_off[a + 1]
+        al = basis.alpha[b0:b1]
+        for l in range(3):
+            nc = int(basis.ncon[a, l])
+            if nc == 0:
+                continue
+            blk = basis.con[basis.con_off[a, l]:basis.con_off[a, l] + nc * (b1 - b0)].reshape(nc, b1 - b0)
+            for m in range([1, 3, 6][l]):
+                for c in range(nc):
+                    out.append((basis.xyz[a], _CART[[0, 1, 4][l] + m], [(al[i], blk[c, i]) for i in range(b1 - b0)
+                                                                         if blk[c, i] != 0.0]))
+    return out
+
+
+def _overlap_1d(i, j, a, b, A, B):
+    """int (x-A)^i (x-B)^j exp(-a(x-A)^2 - b(x-B)^2) dx by Gauss-Hermite-free closed form (binomial expansion
+    about P)."""
+    p = a + b
+    P = (a * A + b * B) / p
+    K = math.exp(-a * b / p * (A - B) ** 2)
+    s = 0.0
+    for u in range(i + 1):
+        for v in range(j + 1):
+            n = u + v
+            if n % 2:
+                continue
+            mom = math.gamma((n + 1) / 2.0) / p ** ((n + 1) / 2.0)  # int t^n e^{-p t^2}
+            s += math.comb(i, u) * math.comb(j, v) * (P - A) ** (i - u) * (P - B) ** (j - v) * mom
+    return K * s
+
+
+def _kinetic_matrix(basis):
+    """T_mn = -1/2 <chi_m|lap|chi_n> in closed form: per axis, d2/dx2 of (x-B)^j e^{-b(x-B)^2} is
+    j(j-1)(x-B)^(j-2) - 2b(2j+1)(x-B)^j + 4b^2(x-B)^(j+2)."""
+    fns = _basis_functions(basis)
+    n = len(fns)
+    T = np.zeros((n, n))
+    for m, (A, pa, pra) in enumerate(fns):
+        for k, (B, pb, prb) in enumerate(fns):
+            t = 0.0
+            for a, ca in pra:
+                for b, cb in prb:
+                    S = [_overlap_1d(pa[d], pb[d], a, b, A[d], B[d]) for d in range(3)]
+                    for d in range(3):
+                        j = pb[d]
+                        D = -2 * b * (2 * j + 1) * S[d] + 4 * b * b * _overlap_1d(pa[d], j + 2, a, b, A[d], B[d])
+                        if j >= 2:
+                            D += j * (j - 1) * _overlap_1d(pa[d], j - 2, a, b, A[d], B[d])
+                        t += ca * cb * (-0.5) * D * S[(d + 1) % 3] * S[(d + 2) % 3]
+            T[m, k] = t
+    return T
+
+
+def test_basis_function_table_matches_basis_eval(tiny_system):
+    """The closed-form helper reads the basis the way the library does (values at random points)."""
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-2.0, 2.0, (7, 3)) + tiny_system.basis.xyz[0]
+    ref = O.basis_eval(tiny_system.basis, pts, 0)[0]
+    for m, (A, pw, prims) in enumerate(_basis_functions(tiny_system.basis)):
+        r = pts - A
+        v = np.prod(r ** np.array(pw), axis=1) * sum(c * np.exp(-a * (r ** 2).sum(1)) for a, c in prims)
+        np.testing.assert_allclose(ref[:, m], v, rtol=1e-12, atol=1e-13)  # screening: alpha r^2 < 40, |c g| > e^-40
+    assert len(_basis_functions(tiny_system.basis)) == tiny_system.basis.nbas
+
+
+def test_lagmult_kinetic_matches_closed_form():
+    """LAGMULT_SLV with pot = 0 is <phi_j|-1/2 lap|phi_k> on the mesh; it converges to c^T T c with the
+    analytic kinetic matrix T, and pot = c adds c times the mesh overlap of the FLOs."""
+    sy = S.make_synthetic(2, 2, 5, nrad=80, nang=302, box=4.0)
+    T = _kinetic_matrix(sy.basis)
+    for s in (1, 2):
+        cf = O.flo_coefs(sy, s)
+        exact = cf.T @ T @ cf
+        lam0 = O.lagmult(sy, s, np.zeros(sy.mesh.npts))
+        assert np.abs(lam0 - exact).max() < 2e-4 * np.abs(exact).max()
+        phi = O.flo_fields(sy, s)
+        ovl = (phi * sy.mesh.w) @ phi.T
+        lam1 = O.lagmult(sy, s, np.full(sy.mesh.npts, 0.75))
+        np.testing.assert_allclose(lam1 - lam0, 0.75 * ovl, atol=1e-12)
+
+
+def test_sic_hamiltonian_projects_onto_the_sic_matrix(tiny_system, tiny_oracle):
+    """Projected on the FLOs, the APATCH matrix is sum_i 1/2 (V^i_ji O_ik + V^i_ki O_ij) with O the mesh overlap
+    of the FLOs and V^i_ab the (independently computed) SIC matrix elements."""
+    r = O.evaluate(tiny_system, want_potentials=True)
+    for s in (1, 2):
+        H = O.sic_hamiltonian(tiny_system, s, r["spins"][s - 1]["vpot"])
+        np.testing.assert_allclose(H, H.T, atol=1e-14)
+        cf = O.flo_coefs(tiny_system, s)
+        phi = O.flo_fields(tiny_system, s)
+        ovl = (phi * tiny_system.mesh.w) @ phi.T
+        vab = r["spins"][s - 1]["vab"]
+        n = cf.shape[1]
+        d = np.array([[vab[i, j, i] for i in range(n)] for j in range(n)])  # d[j, i] = <phi_j|V_i|phi_i>
+        want = 0.5 * (d @ ovl + (d @ ovl).T)
+        np.testing.assert_allclose(cf.T @ H @ cf, want, atol=1e-12 * np.abs(want).max())
