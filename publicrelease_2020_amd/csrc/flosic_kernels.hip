@@ -354,7 +354,9 @@ __device__ __forceinline__ void hermite_R(double p, double X, double Y, double Z
     constexpr int TOP = L < 2 ? 2 : L;
     const double T = p * (X * X + Y * Y + Z * Z);
     double F[TOP + 1];
-    boys<TOP>(T, exp(-T), tab, F);
+    // beyond T = 60, e^-T (< 9e-27) is below half an ulp of every term it enters (F_4(60) ~ 6e-8), so
+    // dropping it leaves each F_m bitwise unchanged and saves the exponential for most (point, group) pairs
+    boys<TOP>(T, T > 60.0 ? 0.0 : exp(-T), tab, F);
     double pw[L + 1];
     pw[0] = 1.0;
 #pragma unroll
@@ -424,23 +426,66 @@ void launch_coulomb(const int *segL, const double4 *pairP, const int *pairRow, c
 
 // ---------------------------------------------------------------------------------------------------
 // MFMA Coulomb.  V[f][k] = sum_rows R[k][row] H[row][f]: R_tuv(p, P - r_k) is generated on the fly (FP64 Boys
-// and Hermite recursion), H is the Hermite-coefficient matrix of every exponent-pair group for every FLO.
-// Block = 12 wavefronts on a tile of TP = 64 points x NFB FLO columns (NFB covers all FLOs up to 256, so R is
-// computed once per (point, group)):
-//   * waves 0-3 (producers) generate R of chunk c+1 into one of two LDS buffers,
-//   * waves 4-11 (consumers, 2 point halves x 4 column quarters) run the MFMAs of chunk c from the other
-//     buffer, with their H fragments streamed from global memory in MFMA operand order (one coalesced load
-//     per fragment, prefetched one K step ahead across chunk boundaries),
-//   * one barrier per chunk swaps the buffers.
+// and Hermite recursion), H is the Hermite-coefficient matrix of every exponent-pair group for every FLO, stored
+// in MFMA operand order.  Block = 16 wavefronts on a tile of TP = 64 points x NFB FLO columns (NFB covers all
+// FLOs up to 256, so R is computed once per (point, group)).  A chunk of Hermite rows is consumed in sub-steps
+// of SR rows; per sub-step, separated by one barrier:
+//   * R waves generate their share of R of the next chunk into the other of two LDS tiles,
+//   * copy waves move the H rows of the sub-step two ahead into a 3-slot LDS ring with asynchronous
+//     global->LDS loads (global_load_lds_dwordx4: lane l of a wave lands at base + 16 l bytes), then wait until
+//     the rows of the next sub-step have landed (vmcnt counts only their own copies),
+//   * 8 consumer waves (2 point halves x 4 column quarters) run the MFMAs of the current sub-step from LDS.
 // FP32 and split-BF16 accumulators are added into FP64 sums after every chunk.
 // ---------------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-constexpr int kProducers = 256;            // 4 wavefronts
-constexpr int kCoulThreads = 256 + 512;    // + 8 consumer wavefronts
+constexpr int kCoulThreads = 1024;  // 16 wavefronts: R waves, copy waves, then 8 consumer waves
+constexpr int kConsumerWave0 = 8;
+constexpr int kSlots = 3;
 
-// one wave64 fragment of H in operand order: lane l holds row 4s + l/16 and column 16*ct + l%16 (FP64/FP32),
-// or rows 32s + 8*(l/16) .. +7 and column 16*ct + l%16 (bf16, 16 bytes per lane)
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt with vmcnt <= n and no wait on the other counters (gfx9 encoding: vmcnt bits 3:0 and 15:14)
+template <int N>
+__device__ __forceinline__ void wait_vm_le() {
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+__device__ __forceinline__ void wait_vm(int n) {
+    switch (n) {
+#define FLOSIC_W(k) \
+    case k: wait_vm_le<k>(); break;
+        FLOSIC_W(0) FLOSIC_W(1) FLOSIC_W(2) FLOSIC_W(3) FLOSIC_W(4) FLOSIC_W(5) FLOSIC_W(6) FLOSIC_W(7)
+        FLOSIC_W(8) FLOSIC_W(9) FLOSIC_W(10) FLOSIC_W(11) FLOSIC_W(12) FLOSIC_W(13) FLOSIC_W(14) FLOSIC_W(15)
+        FLOSIC_W(16) FLOSIC_W(17) FLOSIC_W(18) FLOSIC_W(19) FLOSIC_W(20) FLOSIC_W(21) FLOSIC_W(22) FLOSIC_W(23)
+        FLOSIC_W(24) FLOSIC_W(25) FLOSIC_W(26) FLOSIC_W(27) FLOSIC_W(28) FLOSIC_W(29) FLOSIC_W(30) FLOSIC_W(31)
+#undef FLOSIC_W
+        default: wait_vm_le<0>(); break;
+    }
+}
+
+// KiB units [0, units) of a contiguous 16-byte aligned global run into LDS; unit u goes to copy wave
+// u % ncw; returns the number of loads this wave issued
+__device__ __forceinline__ int lds_copy_async(const char *src, char *dst, int units, int cw, int ncw, int lane,
+                                              int first) {
+    int n = 0;
+    for (int u = (cw - first % ncw + ncw) % ncw; u < units; u += ncw, ++n)
+        __builtin_amdgcn_global_load_lds((const void *)(src + (size_t)u * 1024 + lane * 16),
+                                         (lds_void *)(dst + (size_t)u * 1024), 16, 0, 0);
+    return n;
+}
+
+// position in the chunk / sub-step sequence
+struct SubStep {
+    int c, q;
+};
+__device__ __forceinline__ void advance(SubStep &p, const ChunkDev *chunks, int sr) {
+    if (++p.q * sr >= chunks[p.c].kr) {
+        p.q = 0;
+        ++p.c;
+    }
+}
+
+// one wave64 fragment of H in operand order: lane l holds row 4s + l/16 and column 16*ct + l%16 (FP64/FP32)
 template <typename T>
 __global__ void k_frag_h(const ChunkDev *__restrict__ chunks, const double *__restrict__ H, int ldh, int nF,
                          T *__restrict__ Hf) {
@@ -473,81 +518,164 @@ template <> struct MfmaTraits<float> {
     static __device__ __forceinline__ int row(int lr, int r) { return 4 * lr + r; }
 };
 
-// R of chunk `ch` for points k0 .. k0+TP-1 into Rl[row][point] by the producer threads (ptid < kProducers);
-// rows beyond the chunk's groups are zero
-template <int L, typename T, int TP, int LDR>
-__device__ __forceinline__ void chunk_R(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                        const double *__restrict__ X, const double *__restrict__ Y,
-                                        const double *__restrict__ Z, int64_t k0, int64_t M,
+// R of the groups [np*q/nq, np*(q+1)/nq) of chunk `ch` (share q of nq) for the block's 64 points: R wave rw of
+// nrw takes every nrw-th group (wave-uniform, so the group's centre and exponent are scalar loads) and lane l
+// the point k0 + l, whose coordinates (px, py, pz) stay in registers.  LAYOUT 0: Rl[row * LD + point] in T;
+// LAYOUT 2: Rl[point * LD + row] in T; LAYOUT 1 (split-BF16): NP bf16 pieces, piece p at Rl + p * 4096, row r of
+// point l at l*64 + 8*((r/8) ^ (l%8)) + r%8 (16-byte units XOR-swizzled so the 16-byte operand reads of 8 points
+// hit distinct banks).  Share 0 also zeroes the rows beyond the groups.
+template <int NP>
+__device__ __forceinline__ void put_bf16(__bf16 *Rb, int lane, int r, double v) {
+    const int idx = lane * 64 + ((((r >> 3) ^ (lane & 7)) << 3) | (r & 7));
+    double x = v;
+#pragma unroll
+    for (int pc = 0; pc < NP; ++pc) {
+        const __bf16 b = (__bf16)(float)x;
+        Rb[pc * 4096 + idx] = b;
+        x -= (double)(float)b;
+    }
+}
+
+template <int L, typename T, int LD, int LAYOUT, int NP>
+__device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, double py, double pz, bool valid, int q,
+                                        int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
                                         const double *__restrict__ tab, T *Rl) {
     constexpr int nt = ntuv(L);
     const int np = ch.pe - ch.pb;
-    for (int task = ptid; task < np * TP; task += kProducers) {
-        const int pl = task / TP, pt = task - pl * TP;
-        const int64_t k = k0 + pt;
+    const int glo = np * q / nq, ghi = np * (q + 1) / nq;
+    for (int g = glo + rw; g < ghi; g += nrw) {
+        const double4 P = pP[ch.pb + g];
         double R[nt];
-        if (k < M) {
-            const double4 P = pP[ch.pb + pl];
-            hermite_R<L>(P.w, P.x - X[k], P.y - Y[k], P.z - Z[k], tab, R);
+        if (valid) {
+            hermite_R<L>(P.w, P.x - px, P.y - py, P.z - pz, tab, R);
         } else {
 #pragma unroll
             for (int t = 0; t < nt; ++t) R[t] = 0.0;
         }
 #pragma unroll
-        for (int t = 0; t < nt; ++t) Rl[(pl * nt + t) * LDR + pt] = (T)R[t];
+        for (int t = 0; t < nt; ++t) {
+            if constexpr (LAYOUT == 0)
+                Rl[(g * nt + t) * LD + lane] = (T)R[t];
+            else if constexpr (LAYOUT == 2)
+                Rl[lane * LD + g * nt + t] = (T)R[t];
+            else
+                put_bf16<NP>((__bf16 *)Rl, lane, g * nt + t, R[t]);
+        }
     }
-    for (int e = ptid; e < (ch.kr - np * nt) * TP; e += kProducers) {
-        const int r = np * nt + e / TP, pt = e % TP;
-        Rl[r * LDR + pt] = (T)0;
+    if (q == 0)
+        for (int r = np * nt + rw; r < ch.kr; r += nrw) {
+            if constexpr (LAYOUT == 0)
+                Rl[r * LD + lane] = (T)0;
+            else if constexpr (LAYOUT == 2)
+                Rl[lane * LD + r] = (T)0;
+            else
+                put_bf16<NP>((__bf16 *)Rl, lane, r, 0.0);
+        }
+}
+
+template <typename T, int LD, int LAYOUT, int NP>
+__device__ __forceinline__ void produce_R(int rw, int nrw, int lane, double px, double py, double pz, bool valid,
+                                          int q, int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
+                                          const double *__restrict__ tab, T *Rl) {
+    switch (ch.L) {
+        case 0: chunk_R<0, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        case 1: chunk_R<1, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        case 2: chunk_R<2, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        case 3: chunk_R<3, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        default: chunk_R<4, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
     }
 }
 
-template <typename T, int TP, int LDR>
-__device__ __forceinline__ void produce_R(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                          const double *__restrict__ X, const double *__restrict__ Y,
-                                          const double *__restrict__ Z, int64_t k0, int64_t M,
-                                          const double *__restrict__ tab, T *Rl) {
-    switch (ch.L) {
-        case 0: chunk_R<0, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-        case 1: chunk_R<1, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-        case 2: chunk_R<2, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-        case 3: chunk_R<3, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
-        default: chunk_R<4, T, TP, LDR>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rl); break;
+// H rows of sub-step p (FP operand stream, rows multiple of 4) for block columns [fb, fb + NFB) into an LDS slot
+// laid out [step][NFB/16 column tiles][64 lanes]; returns this copy wave's load count
+template <typename T, int NFB, int SR>
+__device__ __forceinline__ int stage_h_fp(const ChunkDev *__restrict__ chunks, SubStep p, const T *__restrict__ Hf,
+                                          int nct, int fb, T *slot, int cw, int ncw, int lane) {
+    constexpr int UPS = NFB * 4 * (int)sizeof(T) / 1024;  // KiB per K step
+    const ChunkDev ch = chunks[p.c];
+    const int64_t r0 = ch.prow0 + (int64_t)p.q * SR;
+    const int nstep = min(SR, ch.kr - p.q * SR) / 4;
+    int n = 0;
+    for (int s = 0; s < nstep; ++s) {
+        const char *src = (const char *)(Hf + ((r0 / 4 + s) * (int64_t)nct + fb / 16) * 64);
+        n += lds_copy_async(src, (char *)(slot + (size_t)s * NFB * 4), UPS, cw, ncw, lane, s * UPS);
     }
+    return n;
 }
 
 template <typename T, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                              int64_t nsteps, const double4 *__restrict__ pP,
-                                                              const T *__restrict__ Hf, int nct,
-                                                              const double *__restrict__ X,
+                                                              const double4 *__restrict__ pP, const T *__restrict__ Hf,
+                                                              int nct, const double *__restrict__ X,
                                                               const double *__restrict__ Y,
                                                               const double *__restrict__ Z, int64_t M,
                                                               const double *__restrict__ tab, int nF,
                                                               double *__restrict__ V, int64_t ldv) {
-    constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns
+    constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns (TP = wave size)
+    constexpr int SR = sizeof(T) == 4 ? 32 : (NFB <= 192 ? 16 : 8);  // Hermite rows per sub-step
+    constexpr int NRW = 7, NCPW = 1;                                 // R waves, copy waves
     constexpr int PAD = sizeof(T) == 8 ? 8 : 16;
     constexpr int LDR = TP + PAD;
     __shared__ T Rl[2][kChunkRows * LDR];
+    __shared__ __attribute__((aligned(16))) T Hl[kSlots][SR * NFB];
     using Tr = MfmaTraits<T>;
     using acc_t = typename Tr::acc;
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int wave = threadIdx.x >> 6;
-    if (wave < 4) {  // producers
-        const int ptid = threadIdx.x;
-        produce_R<T, TP, LDR>(ptid, chunks[0], pP, X, Y, Z, k0, M, tab, Rl[0]);
+    const int fb = blockIdx.y * NFB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
+        const int64_t k = k0 + lane;
+        const bool valid = k < M;
+        const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
+        produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, tab, Rl[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
-            if (c + 1 < nchunks)
-                produce_R<T, TP, LDR>(ptid, chunks[c + 1], pP, X, Y, Z, k0, M, tab, Rl[(c + 1) & 1]);
-            __syncthreads();
+            const int nsub = (chunks[c].kr + SR - 1) / SR;
+            for (int q = 0; q < nsub; ++q) {
+                if (c + 1 < nchunks)
+                    produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, tab,
+                                            Rl[(c + 1) & 1]);
+                __syncthreads();
+            }
         }
         return;
     }
-    const int cwv = wave - 4;
-    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
-    const int pw = (cwv >> 2) * NRT * 16;                       // first point of this wave
-    const int ct0 = (blockIdx.y * NFB + (cwv & 3) * NCW * 16) / 16;  // first column tile
+    if (wave < kConsumerWave0) {  // copy waves: keep the sub-step two ahead in flight
+        const int cw = wave - NRW;
+        SubStep nxt{0, 0};
+        int gs_ahead = 0, pend = 0;
+        pend = stage_h_fp<T, NFB, SR>(chunks, nxt, Hf, nct, fb, Hl[0], cw, NCPW, lane);
+        advance(nxt, chunks, SR);
+        ++gs_ahead;
+        if (nxt.c < nchunks) {
+            pend = stage_h_fp<T, NFB, SR>(chunks, nxt, Hf, nct, fb, Hl[1], cw, NCPW, lane);
+            advance(nxt, chunks, SR);
+            ++gs_ahead;
+        } else {
+            pend = 0;
+        }
+        wait_vm(pend);  // sub-step 0 has landed
+        // raw barrier: a __syncthreads() fence would also wait for the copies still in flight
+        __builtin_amdgcn_s_barrier();
+        for (int c = 0; c < nchunks; ++c) {
+            const int nsub = (chunks[c].kr + SR - 1) / SR;
+            for (int q = 0; q < nsub; ++q) {
+                int issued = 0;
+                if (nxt.c < nchunks) {
+                    issued = stage_h_fp<T, NFB, SR>(chunks, nxt, Hf, nct, fb, Hl[gs_ahead % kSlots], cw, NCPW, lane);
+                    advance(nxt, chunks, SR);
+                    ++gs_ahead;
+                }
+                wait_vm(issued);  // everything but the newest sub-step has landed
+                __builtin_amdgcn_s_barrier();
+            }
+        }
+        return;
+    }
+    const int cwv = wave - kConsumerWave0;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int pw = (cwv >> 2) * NRT * 16;  // first point of this wave
+    const int cl0 = (cwv & 3) * NCW;        // first column tile of this wave inside the block
     acc_t acc[NRT][NCW];
     double sum[NRT][NCW][4];
 #pragma unroll
@@ -559,49 +687,47 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
                 acc[i][j][r] = (T)0;
                 sum[i][j][r] = 0.0;
             }
-    // H fragments of global K step g are at Hf[(g*nct + ct)*64 + lane]; the stream is contiguous over chunks
-    T bn[NCW];
-#pragma unroll
-    for (int j = 0; j < NCW; ++j) bn[j] = Hf[(int64_t)(ct0 + j) * 64 + lane];
-    int64_t g = 0;
     __syncthreads();
+    int gs = 0;
     for (int c = 0; c < nchunks; ++c) {
-        const int ns = chunks[c].kr / 4;
+        const int kr = chunks[c].kr;
+        const int nsub = (kr + SR - 1) / SR;
         const T *R = Rl[c & 1];
-        for (int s = 0; s < ns; ++s, ++g) {
-            T b[NCW];
+        for (int q = 0; q < nsub; ++q, ++gs) {
+            const T *Hs = Hl[gs % kSlots];
+            const int ns = min(SR, kr - q * SR) / 4;
+            for (int s = 0; s < ns; ++s) {
+                T a[NRT], b[NCW];
 #pragma unroll
-            for (int j = 0; j < NCW; ++j) b[j] = bn[j];
-            if (g + 1 < nsteps) {
+                for (int j = 0; j < NCW; ++j) b[j] = Hs[(s * (NFB / 16) + cl0 + j) * 64 + lane];
 #pragma unroll
-                for (int j = 0; j < NCW; ++j) bn[j] = Hf[((g + 1) * nct + ct0 + j) * 64 + lane];
+                for (int i = 0; i < NRT; ++i) a[i] = R[(q * SR + 4 * s + lr) * LDR + pw + i * 16 + lc];
+#pragma unroll
+                for (int i = 0; i < NRT; ++i)
+#pragma unroll
+                    for (int j = 0; j < NCW; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
             }
-            T a[NRT];
+            if (q + 1 == nsub) {
+                if constexpr (sizeof(T) == 4) {
 #pragma unroll
-            for (int i = 0; i < NRT; ++i) a[i] = R[(4 * s + lr) * LDR + pw + i * 16 + lc];
+                    for (int i = 0; i < NRT; ++i)
 #pragma unroll
-            for (int i = 0; i < NRT; ++i)
+                        for (int j = 0; j < NCW; ++j)
 #pragma unroll
-                for (int j = 0; j < NCW; ++j) acc[i][j] = Tr::mma(a[i], b[j], acc[i][j]);
+                            for (int r = 0; r < 4; ++r) {
+                                sum[i][j][r] += (double)acc[i][j][r];
+                                acc[i][j][r] = 0.0f;
+                            }
+                }
+            }
+            __syncthreads();
         }
-        if constexpr (sizeof(T) == 4) {
-#pragma unroll
-            for (int i = 0; i < NRT; ++i)
-#pragma unroll
-                for (int j = 0; j < NCW; ++j)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        sum[i][j][r] += (double)acc[i][j][r];
-                        acc[i][j][r] = 0.0f;
-                    }
-        }
-        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < NRT; ++i)
 #pragma unroll
         for (int j = 0; j < NCW; ++j) {
-            const int f = (ct0 + j) * 16 + lc;
+            const int f = fb + (cl0 + j) * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t k = k0 + pw + i * 16 + Tr::row(lr, r);
@@ -615,8 +741,9 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
 // Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces (each the rounded remainder of the previous), so the
 // product x*y is carried by x1y1 + x1y2 + x2y1 (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error,
 // every term on the 16x16x32 bf16 MFMA (lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]; FP32
-// register r holds D[4*(l/16) + r][l%16]).  Same block organisation as k_coulomb_ws; R pieces are stored
-// [point][row] so an A operand is one 16-byte LDS read, H pieces come from k_split_h in operand order.
+// register r holds D[4*(l/16) + r][l%16]).  Same block organisation as k_coulomb_ws with one 32-row K step per
+// sub-step and consumer waves of 16 points x half the columns.  R is kept in LDS as FP32 [point][row]; each
+// consumer splits its 8-row A operand when it loads it; H pieces come from k_split_h in operand order.
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -648,48 +775,20 @@ __global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__r
     }
 }
 
-template <int L, int NP, int TP, int LDK>
-__device__ __forceinline__ void chunk_R_bf16(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                             const double *__restrict__ X, const double *__restrict__ Y,
-                                             const double *__restrict__ Z, int64_t k0, int64_t M,
-                                             const double *__restrict__ tab, __bf16 *Rb) {
-    constexpr int nt = ntuv(L);
-    constexpr int PS = TP * LDK;  // piece stride
-    const int np = ch.pe - ch.pb;
-    for (int task = ptid; task < np * TP; task += kProducers) {
-        const int pl = task / TP, pt = task - pl * TP;
-        const int64_t k = k0 + pt;
-        double R[nt];
-        if (k < M) {
-            const double4 P = pP[ch.pb + pl];
-            hermite_R<L>(P.w, P.x - X[k], P.y - Y[k], P.z - Z[k], tab, R);
-        } else {
-#pragma unroll
-            for (int t = 0; t < nt; ++t) R[t] = 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < nt; ++t) split_bf16<NP>(R[t], Rb + pt * LDK + pl * nt + t, PS);
+// K step of sub-step p (32 rows) of every piece for block columns [fb, fb + NFB) into an LDS slot
+// [piece][NFB/16][64][8]; returns this copy wave's load count
+template <int NP, int NFB>
+__device__ __forceinline__ int stage_h_bf16(const ChunkDev *__restrict__ chunks, SubStep p,
+                                            const __bf16 *__restrict__ Hs, int64_t nsteps, int nct, int fb,
+                                            __bf16 *slot, int cw, int ncw, int lane) {
+    constexpr int UPP = NFB / 16;  // KiB per piece
+    const int64_t g = chunks[p.c].prow0 / 32 + p.q;
+    int n = 0;
+    for (int pc = 0; pc < NP; ++pc) {
+        const char *src = (const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512);
+        n += lds_copy_async(src, (char *)(slot + (size_t)pc * UPP * 512), UPP, cw, ncw, lane, pc * UPP);
     }
-    const int r0 = np * nt;
-    for (int e = ptid; e < (ch.kr - r0) * TP; e += kProducers) {
-        const int pt = e / (ch.kr - r0), r = r0 + e % (ch.kr - r0);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) Rb[p * PS + pt * LDK + r] = (__bf16)0.0f;
-    }
-}
-
-template <int NP, int TP, int LDK>
-__device__ __forceinline__ void produce_R_bf16(int ptid, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                               const double *__restrict__ X, const double *__restrict__ Y,
-                                               const double *__restrict__ Z, int64_t k0, int64_t M,
-                                               const double *__restrict__ tab, __bf16 *Rb) {
-    switch (ch.L) {
-        case 0: chunk_R_bf16<0, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-        case 1: chunk_R_bf16<1, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-        case 2: chunk_R_bf16<2, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-        case 3: chunk_R_bf16<3, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-        default: chunk_R_bf16<4, NP, TP, LDK>(ptid, ch, pP, X, Y, Z, k0, M, tab, Rb); break;
-    }
+    return n;
 }
 
 template <int NP, int NFB>
@@ -701,104 +800,137 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev
                                                                    const double *__restrict__ Z, int64_t M,
                                                                    const double *__restrict__ tab, int nF,
                                                                    double *__restrict__ V, int64_t ldv) {
-    constexpr int TP = 64, NRT = 2, NCW = NFB / 64;
-    constexpr int LDK = kChunkRowsBf16 + 8;  // 144-byte point rows: conflict-free 16-byte operand reads
-    __shared__ __attribute__((aligned(16))) __bf16 Rb[2][NP * TP * LDK];
+    constexpr int TP = 64, NCW = NFB / 32, SR = 32;  // consumer wave: 16 points x NFB/2 columns
+    constexpr int NCPW = NP == 3 ? 2 : 1, NRW = kConsumerWave0 - NCPW;
+    constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
+    constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
+    __shared__ __attribute__((aligned(16))) float Rf[2][TP * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlots][SLOT];
     const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int wave = threadIdx.x >> 6;
-    if (wave < 4) {  // producers
-        const int ptid = threadIdx.x;
-        produce_R_bf16<NP, TP, LDK>(ptid, chunks[0], pP, X, Y, Z, k0, M, tab, Rb[0]);
+    const int fb = blockIdx.y * NFB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
+        const int64_t k = k0 + lane;
+        const bool valid = k < M;
+        const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
+        produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, tab, Rf[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
-            if (c + 1 < nchunks)
-                produce_R_bf16<NP, TP, LDK>(ptid, chunks[c + 1], pP, X, Y, Z, k0, M, tab, Rb[(c + 1) & 1]);
-            __syncthreads();
+            const int nsub = chunks[c].kr / SR;
+            for (int q = 0; q < nsub; ++q) {
+                if (c + 1 < nchunks)
+                    produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, tab,
+                                                Rf[(c + 1) & 1]);
+                __syncthreads();
+            }
         }
         return;
     }
-    const int cwv = wave - 4;
-    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
-    const int pw = (cwv >> 2) * NRT * 16;
-    const int ct0 = (blockIdx.y * NFB + (cwv & 3) * NCW * 16) / 16;
-    const int64_t pstride = nsteps * nct * 512;
-    v4f acc[NRT][NCW];
-    double sum[NRT][NCW][4];
-#pragma unroll
-    for (int i = 0; i < NRT; ++i)
-#pragma unroll
-        for (int j = 0; j < NCW; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                acc[i][j][r] = 0.0f;
-                sum[i][j][r] = 0.0;
-            }
-    v8bf bn[NP][NCW];
-#pragma unroll
-    for (int p = 0; p < NP; ++p)
-#pragma unroll
-        for (int j = 0; j < NCW; ++j) bn[p][j] = *(const v8bf *)(Hs + p * pstride + ((int64_t)(ct0 + j) * 64 + lane) * 8);
-    int64_t g = 0;
-    __syncthreads();
-    for (int c = 0; c < nchunks; ++c) {
-        const int ns = chunks[c].kr / 32;
-        const __bf16 *R = Rb[c & 1];
-        for (int s = 0; s < ns; ++s, ++g) {
-            v8bf b[NP][NCW];
-#pragma unroll
-            for (int p = 0; p < NP; ++p)
-#pragma unroll
-                for (int j = 0; j < NCW; ++j) b[p][j] = bn[p][j];
-            if (g + 1 < nsteps) {
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-#pragma unroll
-                    for (int j = 0; j < NCW; ++j)
-                        bn[p][j] = *(const v8bf *)(Hs + p * pstride + (((g + 1) * nct + ct0 + j) * 64 + lane) * 8);
-            }
-#pragma unroll
-            for (int i = 0; i < NRT; ++i) {
-                v8bf a[NP];
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    a[p] = *(const v8bf *)(R + p * TP * LDK + (pw + i * 16 + lc) * LDK + 32 * s + 8 * lr);
-#pragma unroll
-                for (int j = 0; j < NCW; ++j) {
-                    v4f cc = acc[i][j];
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0][j], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1][j], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0][j], cc, 0, 0, 0);
-                    if constexpr (NP == 3) {
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2][j], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1][j], cc, 0, 0, 0);
-                        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0][j], cc, 0, 0, 0);
-                    }
-                    acc[i][j] = cc;
+    if (wave < kConsumerWave0) {  // copy waves
+        const int cw = wave - NRW;
+        SubStep nxt{0, 0};
+        int gs_ahead = 0, pend = 0;
+        pend = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[0], cw, NCPW, lane);
+        advance(nxt, chunks, SR);
+        ++gs_ahead;
+        if (nxt.c < nchunks) {
+            pend = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[1], cw, NCPW, lane);
+            advance(nxt, chunks, SR);
+            ++gs_ahead;
+        } else {
+            pend = 0;
+        }
+        wait_vm(pend);
+        __builtin_amdgcn_s_barrier();
+        for (int c = 0; c < nchunks; ++c) {
+            const int nsub = chunks[c].kr / SR;
+            for (int q = 0; q < nsub; ++q) {
+                int issued = 0;
+                if (nxt.c < nchunks) {
+                    issued = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[gs_ahead % kSlots], cw, NCPW,
+                                                   lane);
+                    advance(nxt, chunks, SR);
+                    ++gs_ahead;
                 }
+                wait_vm(issued);
+                __builtin_amdgcn_s_barrier();
             }
         }
+        return;
+    }
+    const int cwv = wave - kConsumerWave0;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int pw = (cwv >> 1) * 16;   // point tile of this wave
+    const int cl0 = (cwv & 1) * NCW;  // first column tile of this wave inside the block
+    v4f acc[NCW];
+    double sum[NCW][4];
 #pragma unroll
-        for (int i = 0; i < NRT; ++i)
+    for (int j = 0; j < NCW; ++j)
 #pragma unroll
-            for (int j = 0; j < NCW; ++j)
+        for (int r = 0; r < 4; ++r) {
+            acc[j][r] = 0.0f;
+            sum[j][r] = 0.0;
+        }
+    __syncthreads();
+    int gs = 0;
+    for (int c = 0; c < nchunks; ++c) {
+        const int nsub = chunks[c].kr / SR;
+        const float *R = Rf[c & 1];
+        for (int q = 0; q < nsub; ++q, ++gs) {
+            const __bf16 *Hs_ = Hl[gs % kSlots];
+            v8bf a[NP];
+            {  // 8 consecutive rows of this lane's point, split into NP pieces (each the rounded remainder)
+                const float *src = R + (pw + lc) * LDK + SR * q + 8 * lr;
+                const v4f x0 = *(const v4f *)src, x1 = *(const v4f *)(src + 4);
+                const float xs[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    sum[i][j][r] += (double)acc[i][j][r];
-                    acc[i][j][r] = 0.0f;
+                for (int e = 0; e < 8; ++e) {
+                    float r = xs[e];
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        const __bf16 hb = (__bf16)r;
+                        a[p][e] = hb;
+                        r -= (float)hb;
+                    }
                 }
-        __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < NCW; ++j) {
+                v8bf b[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) b[p] = *(const v8bf *)(Hs_ + ((p * (NFB / 16) + cl0 + j) * 64 + lane) * 8);
+                v4f cc = acc[j];
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], cc, 0, 0, 0);
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], cc, 0, 0, 0);
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], cc, 0, 0, 0);
+                if constexpr (NP == 3) {
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], cc, 0, 0, 0);
+                }
+                acc[j] = cc;
+            }
+            if (q + 1 == nsub) {
+#pragma unroll
+                for (int j = 0; j < NCW; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        sum[j][r] += (double)acc[j][r];
+                        acc[j][r] = 0.0f;
+                    }
+            }
+            __syncthreads();
+        }
     }
 #pragma unroll
-    for (int i = 0; i < NRT; ++i)
+    for (int j = 0; j < NCW; ++j) {
+        const int f = fb + (cl0 + j) * 16 + lc;
 #pragma unroll
-        for (int j = 0; j < NCW; ++j) {
-            const int f = (ct0 + j) * 16 + lc;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t k = k0 + pw + i * 16 + 4 * lr + r;
-                if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[i][j][r];
-            }
+        for (int r = 0; r < 4; ++r) {
+            const int64_t k = k0 + pw + 4 * lr + r;
+            if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[j][r];
         }
+    }
 }
 
 // FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
@@ -822,8 +954,8 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, co
     const int nfb = coulomb_cols(nF);
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
 #define FLOSIC_WS(NFB)                                                                                            \
-    hipLaunchKernelGGL((k_coulomb_ws<T, NFB>), grid, blk, 0, s, chunks, nchunks, nsteps, pairP, Hf, ldh / 16, x, y, \
-                       z, M, btab, nF, V, ldv)
+    hipLaunchKernelGGL((k_coulomb_ws<T, NFB>), grid, blk, 0, s, chunks, nchunks, pairP, Hf, ldh / 16, x, y, z, M,   \
+                       btab, nF, V, ldv)
     switch (nfb) {
         case 64: FLOSIC_WS(64); break;
         case 128: FLOSIC_WS(128); break;
