@@ -11,6 +11,10 @@ only if |E_SIC - E_SIC(FP64)| < 1e-6 Ha and max |force - force(FP64)| < 1e-5 Ha/
 throughput of the fastest accepted form.  Every evaluation is synchronous (results are on the host when
 flosic_evaluate returns), so wall time over the steps is the step time.
 
+The roofline block is for the Coulomb kernel of the reported form: achieved = algorithmic flops (2 M rows nF) over
+its HIP-event time on the context stream; traffic = HBM bytes per dispatch from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary in profiles/coulomb_k64_traffic.txt (K=64 workload only, else null).
+
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
 """
 from __future__ import annotations
@@ -26,6 +30,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": ("f64", 78.6), "mixed": ("f32", 157.3), "bf16x3": ("bf16", 2500.0), "bf16x2": ("bf16", 2500.0)}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "coulomb_k64_traffic.txt")
+
+
+def coulomb_kernel_name(prec: str, nF: int) -> str:
+    """Name of the Coulomb kernel instantiation the library launches for this form and FLO count."""
+    cols = 64 if nF <= 64 else 128 if nF <= 128 else 192 if nF <= 192 else 256
+    if prec == "fp64":
+        return f"k_coulomb_ws<double, {cols}>"
+    if prec == "mixed":
+        return f"k_coulomb_ws<float, {cols}>"
+    npc = 3 if prec == "bf16x3" else 2
+    return f"k_coulomb_ws_bf16<{npc}, {cols if nF <= 192 else 128}>"
+
+
+def measured_traffic(kernel: str, atoms: int):
+    """HBM bytes per dispatch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed for the K=64
+    workload (profiles/coulomb_k64_traffic.txt), or None."""
+    if atoms != 64 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("#"):
+            continue
+        parts = line.rsplit(None, 3)
+        if len(parts) == 4 and parts[0].strip() == kernel:
+            return float(parts[3])
+    return None
 DTYPE = {"fp64": "f64", "mixed": "f32", "bf16x3": "bf16x3", "bf16x2": "bf16x2"}
 E_TOL, F_TOL = 1e-6, 1e-5
 
@@ -165,6 +195,14 @@ def main() -> int:
     best = max(ok, key=lambda r: r["evals_s"]) if ok else ref
     flops = coulomb_flops(best["counts"])
     arith, peak = PEAK[best["prec"]]
+    kname = coulomb_kernel_name(best["prec"], int(best["counts"][4]))
+    # compulsory HBM bytes of one launch: H once in the form's operand layout, V written once, mesh coordinates
+    c = best["counts"]
+    hbytes = {"fp64": 8, "mixed": 4, "bf16x3": 6, "bf16x2": 4}[best["prec"]]
+    nfl = int(c[4])
+    cols = 64 if nfl <= 64 else 128 if nfl <= 128 else 192 if nfl <= 192 else 256
+    ldh = (nfl + cols - 1) // cols * cols
+    alg_bytes = float(c[6]) * ldh * hbytes + 8.0 * c[2] * c[4] + 24.0 * c[2]
     achieved = flops / (best["kernel_ms"] * 1e-3) / 1e12
     counts = best["counts"]
     out = {
@@ -194,10 +232,10 @@ def main() -> int:
                               "coulomb_kernel_ms": r["kernel_ms"]} for r in forms[1:]},
         "speedup_vs_fp64": best["evals_s"] / ref["evals_s"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": {"fp64": "k_coulomb_mfma<double>", "mixed": "k_coulomb_mfma<float>",
-                                "bf16x3": "k_coulomb_bf16<3>", "bf16x2": "k_coulomb_bf16<2>"}[best["prec"]],
-                     "mfma_type": arith, "kernel_ms": best["kernel_ms"], "flops_per_launch": flops},
+                     "frac": achieved / peak, "traffic": measured_traffic(kname, sysm.meta["K"]),
+                     "kernel": kname, "mfma_type": arith, "kernel_ms": best["kernel_ms"],
+                     "flops_per_launch": flops,
+                     "algorithmic_bytes": alg_bytes},
         "phases_ms": {k: float(v) for k, v in zip(
             ["fo_lowdin", "orbitals", "hermite", "coulomb", "xc_energy", "sic_matrix", "vab", "forces", "total",
              "coulomb_kernel"], best["phases"])},

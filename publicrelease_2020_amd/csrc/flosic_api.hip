@@ -802,7 +802,6 @@ int check_ready(flosic_ctx *c) {
             return fail(c, FLOSIC_EINVAL, "spin %d: number of FODs differs from occupied orbitals", s + 1);
     }
     if (c->sp[0].nocc + c->sp[1].nocc <= 0) return fail(c, FLOSIC_EINVAL, "no occupied orbitals");
-    if (c->sp[0].nocc > 128 || c->sp[1].nocc > 128) return fail(c, FLOSIC_EINVAL, "at most 128 FODs per spin");
     return 0;
 }
 
