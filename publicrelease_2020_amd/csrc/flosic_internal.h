@@ -49,6 +49,25 @@ struct ChunkDev {
     int prow0;  // first row of the chunk in the zero-extended row space of its list (split-BF16 H layout)
 };
 
+// One system's Coulomb contraction as the kernels see it.  H is in the operand order of the form (double or
+// float fragments over K steps of 4 rows, or bf16 pieces over K steps of 32 rows: nsteps steps); ncb column
+// blocks of NFB columns; V[f * ldv + k].
+template <typename T>
+struct CoulJob {
+    const ChunkDev *chunks;
+    int nchunks;
+    const double4 *pP;
+    const void *H;
+    int nct;  // ldh / 16
+    const double *X, *Y, *Z;
+    int64_t M;
+    int nF;
+    double *V;
+    int64_t ldv;
+    int64_t nsteps;
+    int ncb;
+};
+
 // FLO columns handled by one Coulomb block: all of them up to 256; the H leading dimension ldh is nF rounded
 // up to a multiple of this
 int coulomb_cols(int nF);
@@ -88,6 +107,16 @@ template <typename T>
 void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP, const T *Hf,
                          int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
                          int nF, double *V, int64_t ldv, hipStream_t s);
+
+// Batched form: one launch over the (system, point tile, column block) items of jobs of one Coulomb form and
+// one column block width nfb (= coulomb_cols of every job's nF; split-BF16: coulomb_cols_split).  item_off[j]
+// is the first item of job j (a job has ceil(M/64) * ncb items, tile-major); nitems < 2^31.
+template <typename T>
+void launch_coulomb_batch(const CoulJob<T> *jobs, const int64_t *item_off, int njobs, int64_t nitems, int nfb,
+                          const double *btab, hipStream_t s);
+void launch_coulomb_bf16_batch(int npieces, const CoulJob<__bf16> *jobs, const int64_t *item_off, int njobs,
+                               int64_t nitems, int nfb, const double *btab, hipStream_t s);
+int coulomb_cols_split(int nF);
 
 // split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,
 // FP32 accumulation inside a chunk, FP64 sums across chunks.  H is split once per evaluation into

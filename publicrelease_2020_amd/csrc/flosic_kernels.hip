@@ -603,14 +603,21 @@ __device__ __forceinline__ int stage_h_fp(const ChunkDev *__restrict__ chunks, S
     return n;
 }
 
+// One (point tile, column block) of one system's Coulomb contraction, executed by a whole block.  All waves
+// return from here together; the persistent batch kernel calls it once per work item.
 template <typename T, int NFB>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                              const double4 *__restrict__ pP, const T *__restrict__ Hf,
-                                                              int nct, const double *__restrict__ X,
-                                                              const double *__restrict__ Y,
-                                                              const double *__restrict__ Z, int64_t M,
-                                                              const double *__restrict__ tab, int nF,
-                                                              double *__restrict__ V, int64_t ldv) {
+__device__ __forceinline__ void coulomb_tile_fp(const CoulJob<T> &J, const double *__restrict__ tab, int64_t tile,
+                                                int cb) {
+    const ChunkDev *__restrict__ chunks = J.chunks;
+    const int nchunks = J.nchunks;
+    const double4 *__restrict__ pP = J.pP;
+    const T *__restrict__ Hf = (const T *)J.H;
+    const int nct = J.nct;
+    const double *__restrict__ X = J.X, *__restrict__ Y = J.Y, *__restrict__ Z = J.Z;
+    const int64_t M = J.M;
+    const int nF = J.nF;
+    double *__restrict__ V = J.V;
+    const int64_t ldv = J.ldv;
     constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns (TP = wave size)
     constexpr int SR = sizeof(T) == 4 ? 32 : (NFB <= 192 ? 16 : 8);  // Hermite rows per sub-step
     constexpr int NRW = 7, NCPW = 1;                                 // R waves, copy waves
@@ -620,8 +627,8 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
     __shared__ __attribute__((aligned(16))) T Hl[kSlots][SR * NFB];
     using Tr = MfmaTraits<T>;
     using acc_t = typename Tr::acc;
-    const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int fb = blockIdx.y * NFB;
+    const int64_t k0 = tile * TP;
+    const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
@@ -638,9 +645,7 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
                 __syncthreads();
             }
         }
-        return;
-    }
-    if (wave < kConsumerWave0) {  // copy waves: keep the sub-step two ahead in flight
+    } else if (wave < kConsumerWave0) {  // copy waves: keep the sub-step two ahead in flight
         const int cw = wave - NRW;
         SubStep nxt{0, 0};
         int gs_ahead = 0, pend = 0;
@@ -670,8 +675,7 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
                 __builtin_amdgcn_s_barrier();
             }
         }
-        return;
-    }
+    } else {
     const int cwv = wave - kConsumerWave0;
     const int lr = lane >> 4, lc = lane & 15;
     const int pw = (cwv >> 2) * NRT * 16;  // first point of this wave
@@ -735,6 +739,12 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(const ChunkDev *__r
                 if (f < nF && k < M) V[(int64_t)f * ldv + k] = v;
             }
         }
+    }
+}
+
+template <typename T, int NFB>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(CoulJob<T> J, const double *__restrict__ tab) {
+    coulomb_tile_fp<T, NFB>(J, tab, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -792,22 +802,27 @@ __device__ __forceinline__ int stage_h_bf16(const ChunkDev *__restrict__ chunks,
 }
 
 template <int NP, int NFB>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev *__restrict__ chunks, int nchunks,
-                                                                   int64_t nsteps, const double4 *__restrict__ pP,
-                                                                   const __bf16 *__restrict__ Hs, int nct,
-                                                                   const double *__restrict__ X,
-                                                                   const double *__restrict__ Y,
-                                                                   const double *__restrict__ Z, int64_t M,
-                                                                   const double *__restrict__ tab, int nF,
-                                                                   double *__restrict__ V, int64_t ldv) {
+__device__ __forceinline__ void coulomb_tile_bf16(const CoulJob<__bf16> &J, const double *__restrict__ tab,
+                                                  int64_t tile, int cb) {
+    const ChunkDev *__restrict__ chunks = J.chunks;
+    const int nchunks = J.nchunks;
+    const int64_t nsteps = J.nsteps;
+    const double4 *__restrict__ pP = J.pP;
+    const __bf16 *__restrict__ Hs = (const __bf16 *)J.H;
+    const int nct = J.nct;
+    const double *__restrict__ X = J.X, *__restrict__ Y = J.Y, *__restrict__ Z = J.Z;
+    const int64_t M = J.M;
+    const int nF = J.nF;
+    double *__restrict__ V = J.V;
+    const int64_t ldv = J.ldv;
     constexpr int TP = 64, NCW = NFB / 32, SR = 32;  // consumer wave: 16 points x NFB/2 columns
     constexpr int NCPW = NP == 3 ? 2 : 1, NRW = kConsumerWave0 - NCPW;
     constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
     constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
     __shared__ __attribute__((aligned(16))) float Rf[2][TP * LDK];
     __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlots][SLOT];
-    const int64_t k0 = (int64_t)blockIdx.x * TP;
-    const int fb = blockIdx.y * NFB;
+    const int64_t k0 = tile * TP;
+    const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
@@ -824,9 +839,7 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev
                 __syncthreads();
             }
         }
-        return;
-    }
-    if (wave < kConsumerWave0) {  // copy waves
+    } else if (wave < kConsumerWave0) {  // copy waves
         const int cw = wave - NRW;
         SubStep nxt{0, 0};
         int gs_ahead = 0, pend = 0;
@@ -856,8 +869,7 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev
                 __builtin_amdgcn_s_barrier();
             }
         }
-        return;
-    }
+    } else {
     const int cwv = wave - kConsumerWave0;
     const int lr = lane >> 4, lc = lane & 15;
     const int pw = (cwv >> 1) * 16;   // point tile of this wave
@@ -931,12 +943,57 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(const ChunkDev
             if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[j][r];
         }
     }
+    }
+}
+
+template <int NP, int NFB>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(CoulJob<__bf16> J, const double *__restrict__ tab) {
+    coulomb_tile_bf16<NP, NFB>(J, tab, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Batched Coulomb: one launch over the (system, point tile, column block) items of many systems.  Block b
+// runs item b with the single-system tile code; the host orders the items (largest systems first) so that the
+// dispatcher, which hands blocks to CUs in index order as CUs free up, balances the systems like a work queue.
+// One block fills a CU (16 waves at 128 VGPRs), so the grid is the queue and no block loops.
+// ---------------------------------------------------------------------------------------------------
+// item -> job with item_off[j] the first item of job j (item_off[0] = 0, increasing)
+__device__ __forceinline__ int find_job(const int64_t *__restrict__ item_off, int njobs, int64_t it) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (item_off[mid] <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int NFB>
+__global__ void __launch_bounds__(kCoulThreads)
+    k_coulomb_ws_batch(const CoulJob<T> *__restrict__ jobs, const int64_t *__restrict__ item_off, int njobs,
+                       const double *__restrict__ tab) {
+    const int64_t it = blockIdx.x;
+    const int j = find_job(item_off, njobs, it);
+    const CoulJob<T> J = jobs[j];
+    const int64_t local = it - item_off[j];
+    coulomb_tile_fp<T, NFB>(J, tab, local / J.ncb, (int)(local % J.ncb));
+}
+
+template <int NP, int NFB>
+__global__ void __launch_bounds__(kCoulThreads)
+    k_coulomb_ws_bf16_batch(const CoulJob<__bf16> *__restrict__ jobs, const int64_t *__restrict__ item_off,
+                            int njobs, const double *__restrict__ tab) {
+    const int64_t it = blockIdx.x;
+    const int j = find_job(item_off, njobs, it);
+    const CoulJob<__bf16> J = jobs[j];
+    const int64_t local = it - item_off[j];
+    coulomb_tile_bf16<NP, NFB>(J, tab, local / J.ncb, (int)(local % J.ncb));
 }
 
 // FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
 // split-BF16 kernel takes at most 192 (its register budget), 128 beyond that
 int coulomb_cols(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 256; }
-static int coulomb_cols_split(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 128; }
+int coulomb_cols_split(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 128; }
 
 template <typename T>
 void launch_frag_h(const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF, T *Hf, hipStream_t s) {
@@ -952,17 +1009,14 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, co
                          int nF, double *V, int64_t ldv, hipStream_t s) {
     if (nchunks <= 0 || M <= 0) return;
     const int nfb = coulomb_cols(nF);
+    const CoulJob<T> J{chunks, nchunks, pairP, Hf, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb};
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
-#define FLOSIC_WS(NFB)                                                                                            \
-    hipLaunchKernelGGL((k_coulomb_ws<T, NFB>), grid, blk, 0, s, chunks, nchunks, pairP, Hf, ldh / 16, x, y, z, M,   \
-                       btab, nF, V, ldv)
     switch (nfb) {
-        case 64: FLOSIC_WS(64); break;
-        case 128: FLOSIC_WS(128); break;
-        case 192: FLOSIC_WS(192); break;
-        default: FLOSIC_WS(256); break;
+        case 64: hipLaunchKernelGGL((k_coulomb_ws<T, 64>), grid, blk, 0, s, J, btab); break;
+        case 128: hipLaunchKernelGGL((k_coulomb_ws<T, 128>), grid, blk, 0, s, J, btab); break;
+        case 192: hipLaunchKernelGGL((k_coulomb_ws<T, 192>), grid, blk, 0, s, J, btab); break;
+        default: hipLaunchKernelGGL((k_coulomb_ws<T, 256>), grid, blk, 0, s, J, btab); break;
     }
-#undef FLOSIC_WS
 }
 template void launch_coulomb_mfma<double>(const ChunkDev *, int, int64_t, const double4 *, const double *, int,
                                           const double *, const double *, const double *, int64_t, const double *, int,
@@ -985,10 +1039,9 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
                          const double *btab, int nF, double *V, int64_t ldv, hipStream_t s) {
     if (nchunks <= 0 || M <= 0) return;
     const int nfb = coulomb_cols_split(nF);
+    const CoulJob<__bf16> J{chunks, nchunks, pairP, Hs, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb};
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
-#define FLOSIC_WSB(NP, NFB)                                                                                        \
-    hipLaunchKernelGGL((k_coulomb_ws_bf16<NP, NFB>), grid, blk, 0, s, chunks, nchunks, nsteps, pairP, Hs, ldh / 16, \
-                       x, y, z, M, btab, nF, V, ldv)
+#define FLOSIC_WSB(NP, NFB) hipLaunchKernelGGL((k_coulomb_ws_bf16<NP, NFB>), grid, blk, 0, s, J, btab)
 #define FLOSIC_WSB_NFB(NP)                        \
     switch (nfb) {                                \
         case 64: FLOSIC_WSB(NP, 64); break;       \
@@ -1002,6 +1055,44 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
     }
 #undef FLOSIC_WSB_NFB
 #undef FLOSIC_WSB
+}
+
+template <typename T>
+void launch_coulomb_batch(const CoulJob<T> *jobs, const int64_t *item_off, int njobs, int64_t nitems, int nfb,
+                          const double *btab, hipStream_t s) {
+    if (njobs <= 0 || nitems <= 0) return;
+    const dim3 grid((unsigned)nitems), blk(kCoulThreads);
+    switch (nfb) {
+        case 64: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 64>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
+        case 128: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 128>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
+        case 192: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 192>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
+        default: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 256>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
+    }
+}
+template void launch_coulomb_batch<double>(const CoulJob<double> *, const int64_t *, int, int64_t, int,
+                                           const double *, hipStream_t);
+template void launch_coulomb_batch<float>(const CoulJob<float> *, const int64_t *, int, int64_t, int, const double *,
+                                          hipStream_t);
+
+void launch_coulomb_bf16_batch(int npieces, const CoulJob<__bf16> *jobs, const int64_t *item_off, int njobs,
+                               int64_t nitems, int nfb, const double *btab, hipStream_t s) {
+    if (njobs <= 0 || nitems <= 0) return;
+    const dim3 grid((unsigned)nitems), blk(kCoulThreads);
+#define FLOSIC_WSBB(NP, NFB)                                                                                      \
+    hipLaunchKernelGGL((k_coulomb_ws_bf16_batch<NP, NFB>), grid, blk, 0, s, jobs, item_off, njobs, btab)
+#define FLOSIC_WSBB_NFB(NP)                        \
+    switch (nfb) {                                 \
+        case 64: FLOSIC_WSBB(NP, 64); break;       \
+        case 128: FLOSIC_WSBB(NP, 128); break;     \
+        default: FLOSIC_WSBB(NP, 192); break;      \
+    }
+    if (npieces == 2) {
+        FLOSIC_WSBB_NFB(2)
+    } else {
+        FLOSIC_WSBB_NFB(3)
+    }
+#undef FLOSIC_WSBB_NFB
+#undef FLOSIC_WSBB
 }
 
 // ---------------------------------------------------------------------------------------------------
