@@ -137,5 +137,42 @@ module flosic_hip
       integer(c_int32_t), value :: n
       integer(c_int64_t) :: counts(*)
     end function
+    integer(c_int32_t) function flosic_set_mesh_block(ctx, block_points) bind(C, name="flosic_set_mesh_block")
+      import :: c_int32_t, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: block_points
+    end function
+    integer(c_int32_t) function flosic_batch_create(n, ctxs, use_graph, batch) bind(C, name="flosic_batch_create")
+      import :: c_int32_t, c_ptr
+      integer(c_int32_t), value :: n
+      type(c_ptr) :: ctxs(*)
+      integer(c_int32_t), value :: use_graph
+      type(c_ptr) :: batch
+    end function
+    integer(c_int32_t) function flosic_batch_evaluate(batch, zsic) bind(C, name="flosic_batch_evaluate")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double) :: zsic(*)
+    end function
+    type(c_ptr) function flosic_batch_last_error(batch) bind(C, name="flosic_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end function
+    integer(c_int32_t) function flosic_batch_get_timings(batch, n, ms) bind(C, name="flosic_batch_get_timings")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: n
+      real(c_double) :: ms(*)
+    end function
+    integer(c_int32_t) function flosic_batch_get_counts(batch, n, counts) bind(C, name="flosic_batch_get_counts")
+      import :: c_int32_t, c_ptr, c_int64_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: n
+      integer(c_int64_t) :: counts(*)
+    end function
+    integer(c_int32_t) function flosic_batch_destroy(batch) bind(C, name="flosic_batch_destroy")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+    end function
   end interface
 end module flosic_hip

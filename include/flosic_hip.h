@@ -23,6 +23,7 @@ extern "C" {
 #endif
 
 typedef struct flosic_ctx flosic_ctx;
+typedef struct flosic_batch flosic_batch;
 
 /* status codes */
 #define FLOSIC_OK 0
@@ -128,16 +129,48 @@ int32_t flosic_sic_hamiltonian(flosic_ctx *ctx, int32_t ispn, double *hao);
 int32_t flosic_get_pairs(flosic_ctx *ctx, int32_t *npair, int32_t *list);
 
 /* Phase timings of the last flosic_evaluate in milliseconds (HIP events on the context's stream):
- * [0] FO+Lowdin  [1] orbitals on mesh  [2] Hermite pair coefficients  [3] Coulomb  [4] XC+energies
- * [5] SIC matrix  [6] V^i_ab  [7] forces  [8] total  [9] the Coulomb kernel alone (inside [3]).
- * n <= 10 values are written. */
+ * [0] FO+Lowdin  [1] orbitals on mesh (mesh-block mode: the whole block loop, [4] and [6] then hold only what
+ * follows it)  [2] Hermite pair coefficients  [3] Coulomb operand + Coulomb  [4] XC+energies  [5] SIC matrix
+ * [6] V^i_ab  [7] results and forces  [8] total  [9] the Coulomb kernel alone (inside [3]; in a batch, from this
+ * system's operand being ready to the end of the batched launch).  After a graph-replayed batch evaluation only
+ * [8] is set.  n <= 10 values are written. */
 int32_t flosic_get_timings(flosic_ctx *ctx, int32_t n, double *ms);
 
 /* Counters of the last evaluation: [0] screened primitive shell pairs (the flosic_get_pairs list) [1] Hermite
  * rows (shell pairs that share both primitive exponents and centres are one group with one row block)
  * [2] nmsh [3] nbas [4] nflo (both spins) [5] exponent-pair groups [6] Hermite rows as the Coulomb kernel of
- * the current precision streams them (chunks zero-extended to its MFMA K step).  n <= 7 values are written. */
+ * the current precision streams them (chunks zero-extended to its MFMA K step) [7] peak device bytes the
+ * context has held since its creation.  n <= 8 values are written. */
 int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
+
+/* Mesh-block recomputation.  block_points = 0 (default): the FOD-independent basis planes (values, and for PBE
+ * gradients and Hessians: 10 planes of nbas x nmsh) stay resident between evaluations and the FLO fields of the
+ * whole mesh are stored during one.  block_points > 0 (rounded up to a multiple of 256): nothing of size
+ * nbas x nmsh or nflo x nmsh is kept except V_H; each evaluation walks the mesh in blocks of block_points points
+ * and builds, uses and drops the basis planes, FLO fields, V_tot and U of each block, as the reference's
+ * SICLMSLV does over its 25-point chunks (frmorb_mpi.F:184-572).  Energies are identical in both modes (the
+ * same 256-point partials); the SIC matrix and V^i_ab differ by FP64 summation order only.  No reference
+ * counterpart as an entry point. */
+int32_t flosic_set_mesh_block(flosic_ctx *ctx, int64_t block_points);
+
+/* Batches of independent systems (one context each, all on one device; no counterpart in the reference, whose
+ * GROUP mode spreads one system's orbitals over ranks, pasapotnl_grp.F90).  flosic_batch_evaluate is
+ * flosic_evaluate of every context, with the Coulomb contractions of all systems as one launch per class of
+ * equal precision and column-block width; zsic[i] (may be null) gets ZSIC of ctxs[i], and every per-context
+ * getter then reads that context's results.  With use_graph the device part of the evaluation (Hermite
+ * coefficients, batched Coulomb, fields, SIC matrix, V^i_ab of every system) is captured into one hipGraph on
+ * the second evaluation and replayed on later ones while no context's settings or buffers change (moving FODs
+ * keeps it valid); the FO/Lowdin algebra and the forces run on the host around it.  The contexts must not be
+ * used elsewhere while the batch evaluates, and must outlive the batch. */
+int32_t flosic_batch_create(int32_t n, flosic_ctx *const *ctxs, int32_t use_graph, flosic_batch **batch);
+int32_t flosic_batch_evaluate(flosic_batch *batch, double *zsic);
+const char *flosic_batch_last_error(const flosic_batch *batch);
+/* [0] host wall time of the last batch evaluation [1] its device part (batch stream events) [2] the batched
+ * Coulomb launches alone (0 when the device part was a graph) [3] 0 eager, 1 captured, 2 replayed; ms */
+int32_t flosic_batch_get_timings(flosic_batch *batch, int32_t n, double *ms);
+/* [0] evaluations [1] graph captures [2] graph replays [3] Coulomb launch classes [4] Coulomb work items */
+int32_t flosic_batch_get_counts(flosic_batch *batch, int32_t n, int64_t *counts);
+int32_t flosic_batch_destroy(flosic_batch *batch);
 
 #ifdef __cplusplus
 }

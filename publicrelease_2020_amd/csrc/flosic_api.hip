@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,10 @@ using namespace flosic;
 
 namespace {
 
+double wall_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 constexpr int kNcart[3] = {1, 3, 6};
 constexpr int kCpow[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
                               {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
@@ -33,18 +38,28 @@ struct Status {
     std::string msg;
 };
 
+// device bytes held by one context: current and high-water mark
+struct MemTally {
+    size_t cur = 0, peak = 0;
+    uint64_t allocs = 0;  // allocations so far: a captured graph is valid while this is unchanged
+};
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    MemTally *tally = nullptr;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (tally) tally->cur -= n * sizeof(T);
+        }
         p = nullptr;
         n = 0;
     }
     // (re)allocate to at least cnt elements; returns false on failure
-    bool ensure(size_t cnt) {
+    bool ensure(size_t cnt, MemTally *t = nullptr) {
         if (cnt <= n && p) return true;
         release();
         if (cnt == 0) cnt = 1;
@@ -53,6 +68,12 @@ struct DevBuf {
             return false;
         }
         n = cnt;
+        tally = t;
+        if (t) {
+            ++t->allocs;
+            t->cur += n * sizeof(T);
+            t->peak = std::max(t->peak, t->cur);
+        }
         return true;
     }
 };
@@ -78,6 +99,7 @@ struct Spin {
 }  // namespace
 
 struct flosic_ctx {
+    MemTally mem;  // first member: outlives every buffer below
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
@@ -118,10 +140,14 @@ struct flosic_ctx {
     DevBuf<double> d_x, d_y, d_z, d_w;
     int64_t kmax = 0;
     double rmax = 0.0;
-    // resident basis planes (FOD independent)
+    // resident basis planes (FOD independent), or, in mesh-block mode (block_pts > 0), the planes of one block of
+    // block_pts points, evaluated again for every block of every evaluation
     DevBuf<double> d_planes;
     int planes_nd = 0;
     bool planes_ok = false;
+    int64_t block_pts = 0;
+    uint64_t cfg_gen = 0;  // bumped by every setter that changes the launch sequence
+    bool timed = true;     // per-phase events (off while a batch graph is captured or replayed)
     // spins
     Spin sp[2];
     // evaluation buffers
@@ -133,6 +159,36 @@ struct flosic_ctx {
     bool evaluated = false, fo_done = false;
     double timings[10] = {};
     int64_t counts[8] = {};
+};
+
+// Many independent systems evaluated together: the Coulomb contractions of all of them run as one launch per
+// (Coulomb form, column block width) class, optionally with the whole device part captured once in a hipGraph
+// and replayed while no context's buffers or settings change.
+struct flosic_batch {
+    MemTally mem;
+    int device = 0;
+    std::vector<flosic_ctx *> ctxs;
+    bool use_graph = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_coul = nullptr, ev_end = nullptr, ev_t0 = nullptr, ev_c0 = nullptr,
+               ev_c1 = nullptr;
+    // launch plan: job descriptors and item offsets of every class, uploaded once per plan
+    struct Class {
+        int form = 0, npieces = 0, nfb = 0;
+        std::vector<int> members;
+        size_t job_off = 0, item_off = 0;  // byte offsets in d_plan
+        int64_t nitems = 0;
+    };
+    std::vector<Class> classes;
+    std::vector<unsigned char> h_plan;
+    DevBuf<unsigned char> d_plan;
+    std::vector<uint64_t> plan_sig;  // context signatures the plan was built for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> graph_sig;
+    int64_t evals = 0, captures = 0, replays = 0;
+    double timings[4] = {};
+    Status st;
 };
 
 namespace {
@@ -162,7 +218,7 @@ int fail(flosic_ctx *c, int code, const char *fmt, ...) {
     } while (0)
 #define ALLOC(ctx, buf, cnt)                                                                     \
     do {                                                                                         \
-        if (!(buf).ensure(cnt)) return fail(ctx, FLOSIC_ENOMEM, "device allocation of %zu elements failed", (size_t)(cnt)); \
+        if (!(buf).ensure(cnt, &(ctx)->mem)) return fail(ctx, FLOSIC_ENOMEM, "device allocation of %zu elements failed", (size_t)(cnt)); \
     } while (0)
 
 template <typename T>
@@ -778,10 +834,12 @@ BasisDev basis_view(const flosic_ctx *c) {
     return B;
 }
 
-// FOD-independent basis planes on the mesh, kept resident
+int xc_planes(const flosic_ctx *c) { return c->xc == FLOSIC_XC_PBE ? 10 : 1; }
+
+// FOD-independent basis planes on the whole mesh, kept resident (mesh-block mode keeps none)
 int ensure_planes(flosic_ctx *c) {
-    const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
-    if (c->planes_ok && c->planes_nd == nd) return 0;
+    const int nd = xc_planes(c);
+    if (c->block_pts > 0 || (c->planes_ok && c->planes_nd == nd)) return 0;
     const size_t cnt = (size_t)nd * c->nbas * (size_t)c->M;
     ALLOC(c, c->d_planes, cnt);
     HIPCHK(c, hipMemsetAsync(c->d_planes.p, 0, cnt * sizeof(double), c->stream));
@@ -791,6 +849,19 @@ int ensure_planes(flosic_ctx *c) {
     c->planes_ok = true;
     return 0;
 }
+
+// Basis planes of the mesh block [k0, k0 + m) into dst (leading dimension m); nd as launch_basis_planes
+int block_planes(flosic_ctx *c, int64_t k0, int64_t m, int nd, double *dst) {
+    const size_t cnt = (size_t)(nd < 0 ? 1 : nd) * c->nbas * (size_t)m;
+    launch_fill_zero(dst, cnt, c->stream);  // a kernel, not a memset node, so that the block loop can be captured
+    launch_basis_planes<double>(basis_view(c), c->d_x.p + k0, c->d_y.p + k0, c->d_z.p + k0, m, nd, dst, m, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// The mesh is walked in blocks: one block of M points with resident planes, else blocks of block_pts points
+// (a multiple of 256, so the 256-point energy partials fall on the same points in both modes)
+int64_t mesh_block(const flosic_ctx *c) { return c->block_pts > 0 ? std::min<int64_t>(c->block_pts, c->M) : c->M; }
 
 int check_ready(flosic_ctx *c) {
     if (c->nbas <= 0) return fail(c, FLOSIC_ESTATE, "basis not set");
@@ -821,11 +892,11 @@ int run_frmorb(flosic_ctx *c) {
     return 0;
 }
 
-// out (rows x n, column-major) = A^T B with A (M x rows) and B (M x n) column-major, leading dimension M: the
+// out (rows x n, column-major) (+)= A^T B with A (M x rows) and B (M x n) column-major, leading dimension M: the
 // point dimension is split into slices (the products are skinny with K = M) and the slice partials are summed
-// in a fixed order
-int gemm_tn_points(flosic_ctx *c, int rows, int n, const double *A, const double *B, double *out) {
-    const int64_t M = c->M;
+// in a fixed order, onto out when accumulate is set
+int gemm_tn_points(flosic_ctx *c, int rows, int n, const double *A, const double *B, int64_t M, double *out,
+                   bool accumulate = false) {
     if (rows <= 0 || n <= 0) return 0;
     const double one = 1.0, zero = 0.0;
     int nsl = 64;
@@ -842,28 +913,63 @@ int gemm_tn_points(flosic_ctx *c, int rows, int n, const double *A, const double
                                  &zero, c->d_Gpart.p + (size_t)nsl * rows * n, rows));
         nparts = nsl + 1;
     }
-    launch_reduce_partials(c->d_Gpart.p, 1, nparts, rows * n, out, c->stream);
+    launch_reduce_partials(c->d_Gpart.p, 1, nparts, rows * n, out, c->stream, accumulate);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
 
-int evaluate(flosic_ctx *c) {
+// phi (nd planes of [ncol][m], plane stride ncol*m) = planes (nd planes of [nbas][m]) . cf (nbas x ncol)
+int orbitals_on_points(flosic_ctx *c, const double *planes, int64_t m, int nd, const double *cf, int ncol,
+                       double *phi) {
+    const double one = 1.0, zero = 0.0;
+    const int N = c->nbas;
+    for (int d = 0; d < nd; ++d)
+        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, ncol, N,
+                                 &one, planes + (size_t)d * N * m, (rocblas_int)m, cf, N, &zero,
+                                 phi + (size_t)d * ncol * m, (rocblas_int)m));
+    return 0;
+}
+
+// Event slots of one evaluation
+enum { EV_START, EV_FO, EV_HERM, EV_COUL0, EV_COUL1, EV_PHI, EV_XC, EV_SIC, EV_VAB, EV_END };
+
+int mark(flosic_ctx *c, int slot) {
+    if (c->timed) HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+    return 0;
+}
+#define MARK(c, slot)                         \
+    do {                                      \
+        const int m_ = mark((c), (slot));     \
+        if (m_) return m_;                    \
+    } while (0)
+
+// Coulomb form of a context
+enum CoulForm { COUL_VALU, COUL_FP64, COUL_FP32, COUL_BF16 };
+CoulForm coulomb_form(const flosic_ctx *c) {
+    const char *alg = getenv("FLOSIC_COULOMB");
+    if (alg && !strcmp(alg, "valu")) return COUL_VALU;  // per-thread VALU form, FP64 (kept for A/B measurements)
+    if (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) return COUL_BF16;
+    return c->prec == FLOSIC_PREC_MIXED ? COUL_FP32 : COUL_FP64;
+}
+int coulomb_nfb(const flosic_ctx *c) {
+    return coulomb_form(c) == COUL_BF16 ? coulomb_cols_split(c->nF) : coulomb_cols(c->nF);
+}
+
+// Phase A1: Fermi orbitals, Lowdin, FLO coefficients (FP64; n x n algebra on the host), resident planes
+int eval_fo(flosic_ctx *c) {
     int r = check_ready(c);
     if (r) return r;
     c->evaluated = false;
     const int N = c->nbas;
     const int n0 = c->sp[0].nocc, n1 = c->sp[1].nocc;
     const int nF = n0 + n1;
-    const int64_t M = c->M;
-    const int nd = c->xc == FLOSIC_XC_PBE ? 10 : 1;
     c->nF = nF;
     {
         const int nfb = coulomb_cols(nF);
         c->ldh = (nF + nfb - 1) / nfb * nfb;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    // 1. Fermi orbitals, Lowdin, FLO coefficients (FP64; n x n algebra on the host)
+    HIPCHK(c, hipEventRecord(c->ev[EV_START], c->stream));
     if ((r = run_frmorb(c))) return r;
     for (int s = 0; s < 2; ++s) {
         Spin &sp = c->sp[s];
@@ -877,18 +983,14 @@ int evaluate(flosic_ctx *c) {
                                      sizeof(double) * N * (sp.nwf - sp.nocc), hipMemcpyHostToDevice, c->stream));
     }
     if ((r = ensure_planes(c))) return r;
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    // 2. FLOs (and derivatives) on the mesh: phi_d = Phi_d . cflo
-    ALLOC(c, c->d_phi, (size_t)nd * nF * M);
-    {
-        const double one = 1.0, zero = 0.0;
-        for (int d = 0; d < nd; ++d)
-            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, nF, N,
-                                     &one, c->d_planes.p + (size_t)d * N * M, (rocblas_int)M, c->d_cflo.p, N, &zero,
-                                     c->d_phi.p + (size_t)d * nF * M, (rocblas_int)M));
-    }
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    // 3. Hermite pair coefficients
+    MARK(c, EV_FO);
+    return 0;
+}
+
+// Phase A2 (device work only): Hermite pair coefficients H[row][f] of all FLOs and the Coulomb operand of the
+// context's form
+int eval_hermite(flosic_ctx *c) {
+    const int N = c->nbas, nF = c->nF;
     const int nps = (int)c->ps_atom.size();
     ALLOC(c, c->d_dps, (size_t)nps * 6 * nF);
     launch_prim_coef(nps, c->d_ps_f0.p, c->d_ps_stride.p, c->d_ps_ncart.p, c->d_ps_c0.p, c->d_ps_nc.p, c->d_ps_coef.p,
@@ -896,62 +998,125 @@ int evaluate(flosic_ctx *c) {
     ALLOC(c, c->d_H, (size_t)c->nrows * c->ldh);
     launch_hermite(c->ngroup, c->d_pairs.p, c->d_dps.p, nF, c->d_H.p, c->ldh, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    // 4. Coulomb potentials of all FLOs
-    ALLOC(c, c->d_VH, (size_t)nF * M);
-    {
-        const char *alg = getenv("FLOSIC_COULOMB");
-        if (alg && !strcmp(alg, "valu")) {  // per-thread VALU form, FP64 (kept for A/B measurements)
-            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
-                           c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-        } else if (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) {
+    MARK(c, EV_HERM);
+    // the Coulomb operand in MFMA order
+    switch (coulomb_form(c)) {
+        case COUL_BF16: {
             const int np = c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2;
-            const int64_t nsteps = c->rows_padded_bf16 / 32;
             ALLOC(c, c->d_Hs, (size_t)np * c->ldh * c->rows_padded_bf16);
-            launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, nsteps, c->d_Hs.p, c->stream);
-            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb_bf16(np, c->d_chunks_bf16.p, c->nchunks_bf16, nsteps, c->d_pairP.p, c->d_Hs.p, c->ldh,
-                                c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-        } else if (c->prec == FLOSIC_PREC_MIXED) {
+            launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, c->rows_padded_bf16 / 32,
+                           c->d_Hs.p, c->stream);
+            break;
+        }
+        case COUL_FP32:
             ALLOC(c, c->d_H32, (size_t)c->rows_padded * c->ldh);
             launch_frag_h<float>(c->d_chunks.p, c->nchunks, c->d_H.p, c->ldh, nF, c->d_H32.p, c->stream);
-            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
-            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
-                                       c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-        } else {
+            break;
+        case COUL_FP64:
             ALLOC(c, c->d_H64, (size_t)c->rows_padded * c->ldh);
             launch_frag_h<double>(c->d_chunks.p, c->nchunks, c->d_H.p, c->ldh, nF, c->d_H64.p, c->stream);
-            HIPCHK(c, hipEventRecord(c->ev[9], c->stream));
+            break;
+        default: break;
+    }
+    HIPCHK(c, hipGetLastError());
+    ALLOC(c, c->d_VH, (size_t)nF * c->M);
+    return 0;
+}
+
+// Phase B (one system): Coulomb potentials of all FLOs, V_H[f][k]
+int eval_coulomb(flosic_ctx *c) {
+    const int nF = c->nF;
+    const int64_t M = c->M;
+    MARK(c, EV_COUL0);
+    switch (coulomb_form(c)) {
+        case COUL_VALU:
+            launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
+                           c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            break;
+        case COUL_BF16:
+            launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks_bf16.p, c->nchunks_bf16,
+                                c->rows_padded_bf16 / 32, c->d_pairP.p, c->d_Hs.p, c->ldh, c->d_x.p, c->d_y.p,
+                                c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            break;
+        case COUL_FP32:
+            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
+                                       c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
+            break;
+        case COUL_FP64:
             launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H64.p,
                                         c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M,
                                         c->stream);
-        }
-        HIPCHK(c, hipGetLastError());
+            break;
     }
-    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-    // 5. XC, total potential, energies
+    HIPCHK(c, hipGetLastError());
+    MARK(c, EV_COUL1);
+    return 0;
+}
+
+// Phase C: FLOs on the mesh, XC, total potential, energies, SIC matrix and V^i_ab.  With resident planes the mesh
+// is one block; in mesh-block mode the planes, FLO fields, V_tot and U of each block are built, used and dropped,
+// and G, V^i_ab accumulate over the blocks in block order.
+int eval_fields(flosic_ctx *c) {
+    int r = 0;
+    const int N = c->nbas, nF = c->nF;
+    const int n0 = c->sp[0].nocc;
+    const int64_t M = c->M;
+    const int nd = xc_planes(c);
+    const bool blocked = c->block_pts > 0;
+    const int64_t mb = mesh_block(c);
     const int nblk = (int)((M + 255) / 256);
-    ALLOC(c, c->d_Vtot, (size_t)nF * M);
-    ALLOC(c, c->d_U, (size_t)nF * M);
     ALLOC(c, c->d_part, (size_t)nF * nblk * 9);
     ALLOC(c, c->d_energy, (size_t)nF * 9);
     ALLOC(c, c->d_asy, (size_t)nF);
-    launch_xc(c->xc, nd, c->d_phi.p, M, nF, c->d_VH.p, c->d_w.p, c->d_Vtot.p, c->d_U.p, c->d_part.p, nblk, c->stream);
-    HIPCHK(c, hipGetLastError());
-    launch_reduce_partials(c->d_part.p, nF, nblk, 9, c->d_energy.p, c->stream);
-    HIPCHK(c, hipMemcpy2DAsync(c->d_asy.p, sizeof(double), c->d_VH.p + c->kmax, (size_t)M * sizeof(double),
-                               sizeof(double), nF, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
-    // 6. SIC matrix: G = Phi_0^T U over point slices (split K: the product is N x nF with K = M), slice partials
-    //    summed in a fixed order; SIC_s = -Ctil_s^T G_s
     ALLOC(c, c->d_G, (size_t)N * nF);
+    ALLOC(c, c->d_phi, (size_t)nd * nF * mb);
+    ALLOC(c, c->d_Vtot, (size_t)nF * mb);
+    ALLOC(c, c->d_U, (size_t)nF * mb);
+    if (blocked) ALLOC(c, c->d_planes, (size_t)nd * N * mb);
+    size_t vab_cnt = 0, vpart_cnt = 0;
+    for (int s = 0; s < 2; ++s) {
+        const size_t n = (size_t)c->sp[s].nocc;
+        vab_cnt += n * n * n;
+        vpart_cnt = std::max(vpart_cnt, (size_t)vab_slices(mb) * n * n * n);
+    }
+    ALLOC(c, c->d_vab, vab_cnt);
+    ALLOC(c, c->d_vabpart, vpart_cnt);
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        const double *planes = c->d_planes.p;
+        if (blocked && (r = block_planes(c, k0, m, nd, c->d_planes.p))) return r;
+        if ((r = orbitals_on_points(c, planes, m, nd, c->d_cflo.p, nF, c->d_phi.p))) return r;
+        if (!blocked) MARK(c, EV_PHI);
+        launch_xc(c->xc, nd, c->d_phi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_Vtot.p, c->d_U.p, c->d_part.p,
+                  nblk, (int)(k0 / 256), c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (!blocked) MARK(c, EV_XC);
+        // G = Phi_0^T U over the points (split K, fixed-order partial sums)
+        if ((r = gemm_tn_points(c, N, nF, planes, c->d_U.p, m, c->d_G.p, k0 > 0))) return r;
+        if (!blocked) MARK(c, EV_SIC);
+        size_t vo = 0;
+        for (int s = 0; s < 2; ++s) {
+            const int n = c->sp[s].nocc;
+            if (n == 0) continue;
+            const int off = s == 0 ? 0 : n0;
+            const int nsl = vab_slices(m);
+            launch_vab(c->d_phi.p + (size_t)off * m, c->d_Vtot.p + (size_t)off * m, c->d_w.p + k0, m, n, nsl,
+                       c->d_vabpart.p, c->stream);
+            HIPCHK(c, hipGetLastError());
+            launch_reduce_partials(c->d_vabpart.p, 1, nsl, n * n * n, c->d_vab.p + vo, c->stream, k0 > 0);
+            vo += (size_t)n * n * n;
+        }
+    }
+    if (blocked) MARK(c, EV_PHI);
+    launch_reduce_partials(c->d_part.p, nF, nblk, 9, c->d_energy.p, c->stream);
+    launch_gather(c->d_VH.p + c->kmax, M, nF, c->d_asy.p, c->stream);  // V_H of every FLO at the outermost point
+    if (blocked) MARK(c, EV_XC);
+    // SIC_s = -Ctil_s^T G_s
     size_t lam_cnt = 0;
     for (int s = 0; s < 2; ++s) lam_cnt += (size_t)c->sp[s].nwf * c->sp[s].nocc;
     ALLOC(c, c->d_lambda, lam_cnt);
     {
         const double zero = 0.0, mone = -1.0;
-        if ((r = gemm_tn_points(c, N, nF, c->d_planes.p, c->d_U.p, c->d_G.p))) return r;
         size_t lo = 0;
         for (int s = 0; s < 2; ++s) {
             Spin &sp = c->sp[s];
@@ -963,34 +1128,14 @@ int evaluate(flosic_ctx *c) {
             lo += (size_t)sp.nwf * sp.nocc;
         }
     }
-    HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
-    // 7. V^i_ab per spin
-    size_t vab_cnt = 0, vpart_cnt = 0;
-    int nchunk[2] = {1, 1};
-    for (int s = 0; s < 2; ++s) {
-        const int n = c->sp[s].nocc;
-        if (n == 0) continue;
-        nchunk[s] = vab_slices(M);
-        vab_cnt += (size_t)n * n * n;
-        vpart_cnt = std::max(vpart_cnt, (size_t)nchunk[s] * n * n * n);
-    }
-    ALLOC(c, c->d_vab, vab_cnt);
-    ALLOC(c, c->d_vabpart, vpart_cnt);
-    {
-        size_t vo = 0;
-        for (int s = 0; s < 2; ++s) {
-            const int n = c->sp[s].nocc;
-            if (n == 0) continue;
-            const int off = s == 0 ? 0 : n0;
-            launch_vab(c->d_phi.p + (size_t)off * M, c->d_Vtot.p + (size_t)off * M, c->d_w.p, M, n, nchunk[s],
-                       c->d_vabpart.p, c->stream);
-            HIPCHK(c, hipGetLastError());
-            launch_reduce_partials(c->d_vabpart.p, 1, nchunk[s], n * n * n, c->d_vab.p + vo, c->stream);
-            vo += (size_t)n * n * n;
-        }
-    }
-    HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
-    // 8. results to host, forces
+    if (blocked) MARK(c, EV_SIC);
+    MARK(c, EV_VAB);
+    return 0;
+}
+
+// Phase D: results to the host, RESULTS(1:13), forces, timings
+int eval_finish(flosic_ctx *c) {
+    const int nF = c->nF;
     c->h_energy.resize((size_t)nF * 9);
     c->h_asy.resize(nF);
     HIPCHK(c, hipMemcpyAsync(c->h_energy.data(), c->d_energy.p, sizeof(double) * nF * 9, hipMemcpyDeviceToHost, c->stream));
@@ -1051,25 +1196,279 @@ int evaluate(flosic_ctx *c) {
             force_adjoint(sp, sp.grad.data());
         }
     }
-    HIPCHK(c, hipEventRecord(c->ev[8], c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev[8]));
-    float ms = 0.f;
-    for (int i = 0; i < 8; ++i) {
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    HIPCHK(c, hipEventRecord(c->ev[EV_END], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[EV_END]));
+    // timing slots: [0] FO+Lowdin [1] orbitals on the mesh (mesh-block mode: the whole block loop) [2] Hermite
+    // [3] operand + Coulomb [4] XC + energies [5] SIC matrix [6] V^i_ab [7] host results and forces [8] total
+    // [9] Coulomb kernel
+    const int from[10] = {EV_START, EV_COUL1, EV_FO, EV_HERM, EV_PHI, EV_XC, EV_SIC, EV_VAB, EV_START, EV_COUL0};
+    const int to[10] = {EV_FO, EV_PHI, EV_HERM, EV_COUL1, EV_XC, EV_SIC, EV_VAB, EV_END, EV_END, EV_COUL1};
+    for (int i = 0; i < 10; ++i) {
+        float ms = 0.f;
+        if (c->timed) HIPCHK(c, hipEventElapsedTime(&ms, c->ev[from[i]], c->ev[to[i]]));
         c->timings[i] = ms;
     }
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[8]));
-    c->timings[8] = ms;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[9], c->ev[4]));
-    c->timings[9] = ms;
+    if (!c->timed) {  // only the whole evaluation is timed
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[EV_START], c->ev[EV_END]));
+        c->timings[8] = ms;
+    }
     c->counts[0] = c->npair;
     c->counts[1] = c->nrows;
     c->counts[2] = c->M;
     c->counts[3] = c->nbas;
     c->counts[4] = nF;
     c->counts[5] = c->ngroup;
-    c->counts[6] = (c->prec == FLOSIC_PREC_BF16X3 || c->prec == FLOSIC_PREC_BF16X2) ? c->rows_padded_bf16 : c->rows_padded;
+    c->counts[6] = coulomb_form(c) == COUL_BF16 ? c->rows_padded_bf16 : c->rows_padded;
+    c->counts[7] = (int64_t)c->mem.peak;
     c->evaluated = true;
+    return 0;
+}
+
+int evaluate(flosic_ctx *c) {
+    int r = 0;
+    if ((r = eval_fo(c)) || (r = eval_hermite(c)) || (r = eval_coulomb(c)) || (r = eval_fields(c)) ||
+        (r = eval_finish(c)))
+        return r;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Batches of systems
+// ---------------------------------------------------------------------------------------------------
+int bfail(flosic_batch *b, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    b->st.code = code;
+    b->st.msg = buf;
+    return code;
+}
+int bfail_ctx(flosic_batch *b, int i, int code) {
+    return bfail(b, code, "system %d: %s", i, b->ctxs[i]->st.msg.c_str());
+}
+#define BHIPCHK(b, call)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return bfail(b, FLOSIC_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// what a captured graph or an uploaded plan depends on: every context's allocations and settings
+std::vector<uint64_t> batch_sig(const flosic_batch *b) {
+    std::vector<uint64_t> v;
+    for (const flosic_ctx *c : b->ctxs) {
+        v.push_back(c->mem.allocs);
+        v.push_back(c->cfg_gen);
+        v.push_back((uint64_t)coulomb_form(c));
+    }
+    return v;
+}
+
+// The Coulomb job of one context in its form
+CoulJob<double> coulomb_job(const flosic_ctx *c) {
+    CoulJob<double> J{};
+    const CoulForm form = coulomb_form(c);
+    const int nfb = coulomb_nfb(c);
+    J.chunks = form == COUL_BF16 ? c->d_chunks_bf16.p : c->d_chunks.p;
+    J.nchunks = form == COUL_BF16 ? c->nchunks_bf16 : c->nchunks;
+    J.pP = c->d_pairP.p;
+    J.H = form == COUL_BF16 ? (const void *)c->d_Hs.p : form == COUL_FP32 ? (const void *)c->d_H32.p
+                                                                           : (const void *)c->d_H64.p;
+    J.nct = c->ldh / 16;
+    J.X = c->d_x.p;
+    J.Y = c->d_y.p;
+    J.Z = c->d_z.p;
+    J.M = c->M;
+    J.nF = c->nF;
+    J.V = c->d_VH.p;
+    J.ldv = c->M;
+    J.nsteps = form == COUL_BF16 ? c->rows_padded_bf16 / 32 : c->rows_padded / 4;
+    J.ncb = c->ldh / nfb;
+    return J;
+}
+
+// Classes of equal (form, pieces, column block width); members heaviest first (work per item ~ K steps), so
+// that the longest items are dispatched first; job descriptors and item offsets uploaded in one buffer
+int build_plan(flosic_batch *b) {
+    static_assert(sizeof(CoulJob<double>) == sizeof(CoulJob<float>) && sizeof(CoulJob<double>) == sizeof(CoulJob<__bf16>),
+                  "job layout is independent of the operand type");
+    b->classes.clear();
+    for (int i = 0; i < (int)b->ctxs.size(); ++i) {
+        const flosic_ctx *c = b->ctxs[i];
+        const CoulForm form = coulomb_form(c);
+        if (form == COUL_VALU || c->nchunks <= 0) continue;
+        const int np = form == COUL_BF16 ? (c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2) : 0;
+        const int nfb = coulomb_nfb(c);
+        auto it = std::find_if(b->classes.begin(), b->classes.end(), [&](const flosic_batch::Class &k) {
+            return k.form == form && k.npieces == np && k.nfb == nfb;
+        });
+        if (it == b->classes.end()) {
+            b->classes.push_back({});
+            it = b->classes.end() - 1;
+            it->form = form;
+            it->npieces = np;
+            it->nfb = nfb;
+        }
+        it->members.push_back(i);
+    }
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    b->h_plan.clear();
+    for (auto &k : b->classes) {
+        std::vector<CoulJob<double>> jobs;
+        for (int i : k.members) jobs.push_back(coulomb_job(b->ctxs[i]));
+        std::vector<int> order(jobs.size());
+        for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].nsteps > jobs[y].nsteps; });
+        std::vector<CoulJob<double>> sorted;
+        std::vector<int64_t> off;
+        int64_t items = 0;
+        for (int j : order) {
+            sorted.push_back(jobs[j]);
+            off.push_back(items);
+            items += (jobs[j].M + 63) / 64 * jobs[j].ncb;
+        }
+        if (items >= (int64_t)INT32_MAX) return bfail(b, FLOSIC_EINVAL, "batch too large: %lld Coulomb items", (long long)items);
+        k.nitems = items;
+        k.job_off = align(b->h_plan.size());
+        b->h_plan.resize(k.job_off + sorted.size() * sizeof(CoulJob<double>));
+        std::memcpy(b->h_plan.data() + k.job_off, sorted.data(), sorted.size() * sizeof(CoulJob<double>));
+        k.item_off = align(b->h_plan.size());
+        b->h_plan.resize(k.item_off + off.size() * sizeof(int64_t));
+        std::memcpy(b->h_plan.data() + k.item_off, off.data(), off.size() * sizeof(int64_t));
+    }
+    if (!b->h_plan.empty()) {
+        if (!b->d_plan.ensure(b->h_plan.size(), &b->mem)) return bfail(b, FLOSIC_ENOMEM, "batch plan allocation");
+        BHIPCHK(b, hipMemcpy(b->d_plan.p, b->h_plan.data(), b->h_plan.size(), hipMemcpyHostToDevice));
+    }
+    b->plan_sig = batch_sig(b);
+    return 0;
+}
+
+int launch_classes(flosic_batch *b) {
+    for (const auto &k : b->classes) {
+        const auto *jobs = b->d_plan.p + k.job_off;
+        const auto *off = reinterpret_cast<const int64_t *>(b->d_plan.p + k.item_off);
+        const int nj = (int)k.members.size();
+        const double *btab = b->ctxs[k.members[0]]->d_btab.p;
+        if (k.form == COUL_FP64)
+            launch_coulomb_batch<double>(reinterpret_cast<const CoulJob<double> *>(jobs), off, nj, k.nitems, k.nfb,
+                                         btab, b->stream);
+        else if (k.form == COUL_FP32)
+            launch_coulomb_batch<float>(reinterpret_cast<const CoulJob<float> *>(jobs), off, nj, k.nitems, k.nfb, btab,
+                                        b->stream);
+        else
+            launch_coulomb_bf16_batch(k.npieces, reinterpret_cast<const CoulJob<__bf16> *>(jobs), off, nj, k.nitems,
+                                      k.nfb, btab, b->stream);
+        BHIPCHK(b, hipGetLastError());
+    }
+    return 0;
+}
+
+// The device part of a batch evaluation, issued on the batch stream and forked to the context streams:
+// Hermite coefficients and operands of every system, the batched Coulomb launches, then every system's fields.
+int issue_region(flosic_batch *b, bool eager) {
+    int r = 0;
+    const int n = (int)b->ctxs.size();
+    BHIPCHK(b, hipEventRecord(b->ev_fork, b->stream));
+    for (int i = 0; i < n; ++i) {
+        flosic_ctx *c = b->ctxs[i];
+        BHIPCHK(b, hipStreamWaitEvent(c->stream, b->ev_fork, 0));
+        if ((r = eval_hermite(c))) return bfail_ctx(b, i, r);
+        BHIPCHK(b, hipEventRecord(c->ev[EV_COUL0], c->stream));
+        BHIPCHK(b, hipStreamWaitEvent(b->stream, c->ev[EV_COUL0], 0));
+    }
+    if (eager && b->plan_sig != batch_sig(b) && (r = build_plan(b))) return r;
+    if (eager) BHIPCHK(b, hipEventRecord(b->ev_c0, b->stream));
+    if ((r = launch_classes(b))) return r;
+    if (eager) BHIPCHK(b, hipEventRecord(b->ev_c1, b->stream));
+    BHIPCHK(b, hipEventRecord(b->ev_coul, b->stream));
+    for (int i = 0; i < n; ++i) {
+        flosic_ctx *c = b->ctxs[i];
+        BHIPCHK(b, hipStreamWaitEvent(c->stream, b->ev_coul, 0));
+        if (coulomb_form(c) == COUL_VALU) {
+            if ((r = eval_coulomb(c))) return bfail_ctx(b, i, r);
+        } else {
+            if ((r = mark(c, EV_COUL1))) return bfail_ctx(b, i, r);
+        }
+        if ((r = eval_fields(c))) return bfail_ctx(b, i, r);
+        BHIPCHK(b, hipEventRecord(c->ev[EV_VAB], c->stream));
+        BHIPCHK(b, hipStreamWaitEvent(b->stream, c->ev[EV_VAB], 0));
+    }
+    return 0;
+}
+
+void drop_graph(flosic_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_sig.clear();
+}
+
+int batch_evaluate(flosic_batch *b, double *zsic) {
+    const double t_host0 = wall_ms();
+    int r = 0;
+    const int n = (int)b->ctxs.size();
+    b->st = Status();
+    BHIPCHK(b, hipSetDevice(b->device));
+    for (int i = 0; i < n; ++i) {
+        flosic_ctx *c = b->ctxs[i];
+        c->st = Status();
+        c->timed = true;
+        if ((r = eval_fo(c))) return bfail_ctx(b, i, r);
+        BHIPCHK(b, hipEventRecord(c->ev[EV_FO], c->stream));
+        BHIPCHK(b, hipStreamWaitEvent(b->stream, c->ev[EV_FO], 0));
+    }
+    const std::vector<uint64_t> sig = batch_sig(b);
+    const bool replay = b->use_graph && b->exec && sig == b->graph_sig;
+    const bool capture = b->use_graph && !replay && b->evals > 0 && sig == b->plan_sig;
+    if (!replay) drop_graph(b);
+    for (flosic_ctx *c : b->ctxs) c->timed = !(replay || capture);
+    BHIPCHK(b, hipEventRecord(b->ev_t0, b->stream));
+    if (replay) {
+        BHIPCHK(b, hipGraphLaunch(b->exec, b->stream));
+        ++b->replays;
+    } else if (capture) {
+        BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        r = issue_region(b, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (r || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            for (flosic_ctx *c : b->ctxs) c->timed = true;
+            return r ? r : bfail(b, FLOSIC_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        }
+        b->graph = g;
+        BHIPCHK(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+        b->graph_sig = sig;
+        ++b->captures;
+        BHIPCHK(b, hipGraphLaunch(b->exec, b->stream));
+    } else {
+        if ((r = issue_region(b, true))) return r;
+    }
+    BHIPCHK(b, hipEventRecord(b->ev_end, b->stream));
+    for (flosic_ctx *c : b->ctxs) BHIPCHK(b, hipStreamWaitEvent(c->stream, b->ev_end, 0));
+    for (int i = 0; i < n; ++i) {
+        flosic_ctx *c = b->ctxs[i];
+        r = eval_finish(c);
+        c->timed = true;
+        if (r) return bfail_ctx(b, i, r);
+        if (zsic) zsic[i] = c->zsic;
+    }
+    if (!replay && !capture) b->plan_sig = batch_sig(b);
+    float ms = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->ev_t0, b->ev_end));
+    b->timings[1] = ms;
+    b->timings[2] = 0.0;
+    if (!replay && !capture) {
+        BHIPCHK(b, hipEventElapsedTime(&ms, b->ev_c0, b->ev_c1));
+        b->timings[2] = ms;
+    }
+    b->timings[3] = replay ? 2 : capture ? 1 : 0;
+    ++b->evals;
+    b->timings[0] = wall_ms() - t_host0;
     return 0;
 }
 
@@ -1140,6 +1539,7 @@ const char *flosic_last_error(const flosic_ctx *c) { return c ? c->st.msg.c_str(
 int32_t flosic_set_basis(flosic_ctx *c, int32_t natom, const double *xyz, const int32_t *bare_off, const double *alpha,
                          const int32_t *ncon, const int32_t *con_off, const double *con) {
     if (!c) return FLOSIC_EINVAL;
+    ++c->cfg_gen;
     if (natom <= 0 || !xyz || !bare_off || !alpha || !ncon || !con_off || !con)
         return fail(c, FLOSIC_EINVAL, "flosic_set_basis: bad arguments");
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
@@ -1177,6 +1577,7 @@ int32_t flosic_set_basis(flosic_ctx *c, int32_t natom, const double *xyz, const 
 
 int32_t flosic_set_mesh(flosic_ctx *c, int64_t nmsh, const double *rmsh, const double *wmsh) {
     if (!c) return FLOSIC_EINVAL;
+    ++c->cfg_gen;
     if (nmsh <= 0 || !rmsh || !wmsh) return fail(c, FLOSIC_EINVAL, "flosic_set_mesh: bad arguments");
     if (nmsh > (int64_t)INT32_MAX / 2) return fail(c, FLOSIC_EINVAL, "mesh too large");
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
@@ -1207,6 +1608,7 @@ int32_t flosic_set_mesh(flosic_ctx *c, int64_t nmsh, const double *rmsh, const d
 
 int32_t flosic_set_options(flosic_ctx *c, int32_t nspn, int32_t xc, int32_t precision) {
     if (!c) return FLOSIC_EINVAL;
+    ++c->cfg_gen;
     if (nspn != 1 && nspn != 2) return fail(c, FLOSIC_EINVAL, "nspn must be 1 or 2");
     if (xc != FLOSIC_XC_LSDA && xc != FLOSIC_XC_PBE) return fail(c, FLOSIC_EINVAL, "unknown functional %d", xc);
     if (precision < FLOSIC_PREC_FP64 || precision > FLOSIC_PREC_BF16X2)
@@ -1219,8 +1621,33 @@ int32_t flosic_set_options(flosic_ctx *c, int32_t nspn, int32_t xc, int32_t prec
     return FLOSIC_OK;
 }
 
+int32_t flosic_set_mesh_block(flosic_ctx *c, int64_t block_points) {
+    if (!c) return FLOSIC_EINVAL;
+    if (block_points < 0) return fail(c, FLOSIC_EINVAL, "block_points must be >= 0");
+    const int64_t b = block_points == 0 ? 0 : (block_points + 255) / 256 * 256;
+    if (b != c->block_pts) {
+        if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_mesh_block sync");
+        // mesh-sized buffers of the other mode are released
+        c->d_planes.release();
+        c->d_lapplane.release();
+        c->d_phi.release();
+        c->d_Vtot.release();
+        c->d_U.release();
+        c->d_lap.release();
+        c->d_wop.release();
+        c->planes_ok = false;
+        c->lapplane_ok = false;
+        c->evaluated = false;
+    }
+    c->block_pts = b;
+    ++c->cfg_gen;
+    return FLOSIC_OK;
+}
+
 int32_t flosic_set_orbitals(flosic_ctx *c, int32_t ispn, int32_t nbas, int32_t nwf, int32_t nocc, const double *coef) {
     if (!c) return FLOSIC_EINVAL;
+    ++c->cfg_gen;
     const int s = spin_index(c, ispn);
     if (s < 0) return FLOSIC_EINVAL;
     if (nbas != c->nbas) return fail(c, FLOSIC_EINVAL, "nbas %d differs from the basis (%d)", nbas, c->nbas);
@@ -1340,12 +1767,32 @@ int32_t flosic_get_potential(flosic_ctx *c, int32_t ispn, int32_t iorb, double *
     if (!c) return FLOSIC_EINVAL;
     const int s = spin_index(c, ispn);
     if (s < 0) return FLOSIC_EINVAL;
-    const int r = need_eval(c);
+    int r = need_eval(c);
     if (r) return r;
     if (iorb < 1 || iorb > c->sp[s].nocc) return fail(c, FLOSIC_EINVAL, "orbital index %d out of range", iorb);
     const int f = (s == 0 ? 0 : c->sp[0].nocc) + iorb - 1;
-    if (pot) HIPCHK(c, hipMemcpy(pot, c->d_Vtot.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
-    if (vh) HIPCHK(c, hipMemcpy(vh, c->d_VH.p + (size_t)f * c->M, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    const int64_t M = c->M;
+    if (vh) HIPCHK(c, hipMemcpy(vh, c->d_VH.p + (size_t)f * M, sizeof(double) * M, hipMemcpyDeviceToHost));
+    if (!pot) return FLOSIC_OK;
+    if (c->block_pts <= 0) {
+        HIPCHK(c, hipMemcpy(pot, c->d_Vtot.p + (size_t)f * M, sizeof(double) * M, hipMemcpyDeviceToHost));
+        return FLOSIC_OK;
+    }
+    // mesh-block mode keeps no V_tot: the blocks are evaluated again (energy partials land in scratch)
+    const int nF = c->nF, nd = xc_planes(c);
+    const int64_t mb = mesh_block(c);
+    const int nblk = (int)((M + 255) / 256);
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        if ((r = block_planes(c, k0, m, nd, c->d_planes.p))) return r;
+        if ((r = orbitals_on_points(c, c->d_planes.p, m, nd, c->d_cflo.p, nF, c->d_phi.p))) return r;
+        launch_xc(c->xc, nd, c->d_phi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_Vtot.p, c->d_U.p, c->d_part.p,
+                  nblk, (int)(k0 / 256), c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(pot + k0, c->d_Vtot.p + (size_t)f * m, sizeof(double) * m, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return FLOSIC_OK;
 }
 
@@ -1355,46 +1802,60 @@ int32_t flosic_lagmult_slv(flosic_ctx *c, int32_t ispn, const double *pot, int32
     if (s < 0) return FLOSIC_EINVAL;
     int r = need_eval(c);
     if (r) return r;
-    const int n = c->sp[s].nocc, N = c->nbas, nF = c->nF;
+    const int n = c->sp[s].nocc, N = c->nbas;
     const int64_t M = c->M;
     if (ldsic < n) return fail(c, FLOSIC_EINVAL, "ldsic %d < nfrm %d", ldsic, n);
     if (n == 0) return FLOSIC_OK;
     const int off = s == 0 ? 0 : c->sp[0].nocc;
     const double one = 1.0, zero = 0.0;
-    // Laplacians of this spin's FLOs on the mesh: from the Hessian planes (PBE) or a Laplacian plane built once
-    ALLOC(c, c->d_lap, (size_t)n * M);
     const double *cf = c->d_cflo.p + (size_t)off * N;
-    if (c->planes_nd == 10) {
-        for (int d = 4; d < 7; ++d)
-            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, N, &one,
-                                     c->d_planes.p + (size_t)d * N * M, (rocblas_int)M, cf, N, d == 4 ? &zero : &one,
-                                     c->d_lap.p, (rocblas_int)M));
-    } else {
-        if (!c->lapplane_ok) {
+    const bool blocked = c->block_pts > 0;
+    const int64_t mb = mesh_block(c);
+    const int nd = xc_planes(c);
+    if ((r = upload(c, c->d_pot, pot, (size_t)M))) return r;
+    ALLOC(c, c->d_lap, (size_t)n * mb);
+    ALLOC(c, c->d_wop, (size_t)n * mb);
+    ALLOC(c, c->d_small, (size_t)n * n);
+    // Laplacians of the FLOs: from the Hessian planes (PBE) or from a Laplacian plane (kept resident unless in
+    // mesh-block mode)
+    if (nd != 10) {
+        if (blocked) {
+            ALLOC(c, c->d_lapplane, (size_t)N * mb);
+        } else if (!c->lapplane_ok) {
             ALLOC(c, c->d_lapplane, (size_t)N * M);
-            HIPCHK(c, hipMemsetAsync(c->d_lapplane.p, 0, sizeof(double) * N * M, c->stream));
-            launch_basis_planes<double>(basis_view(c), c->d_x.p, c->d_y.p, c->d_z.p, M, -1, c->d_lapplane.p, M,
-                                        c->stream);
-            HIPCHK(c, hipGetLastError());
+            if ((r = block_planes(c, 0, M, -1, c->d_lapplane.p))) return r;
             c->lapplane_ok = true;
         }
-        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, N, &one,
-                                 c->d_lapplane.p, (rocblas_int)M, cf, N, &zero, c->d_lap.p, (rocblas_int)M));
     }
-    // T_k = w (-1/2 lap phi_k + V phi_k), L = phi^T T
-    if ((r = upload(c, c->d_pot, pot, (size_t)M))) return r;
-    ALLOC(c, c->d_wop, (size_t)n * M);
-    launch_weighted_op(n, M, c->d_w.p, c->d_pot.p, c->d_phi.p + (size_t)off * M, c->d_lap.p, -0.5, c->d_wop.p,
-                       c->stream);
-    HIPCHK(c, hipGetLastError());
-    ALLOC(c, c->d_small, (size_t)n * n);
-    if ((r = gemm_tn_points(c, n, n, c->d_phi.p + (size_t)off * M, c->d_wop.p, c->d_small.p))) return r;
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        const double *planes = c->d_planes.p;
+        const double *phi = c->d_phi.p + (size_t)off * M;
+        if (blocked) {
+            if ((r = block_planes(c, k0, m, nd, c->d_planes.p))) return r;
+            if ((r = orbitals_on_points(c, planes, m, 1, cf, n, c->d_phi.p))) return r;
+            phi = c->d_phi.p;
+        }
+        if (nd == 10) {
+            for (int d = 4; d < 7; ++d)
+                BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, n, N,
+                                         &one, planes + (size_t)d * N * m, (rocblas_int)m, cf, N,
+                                         d == 4 ? &zero : &one, c->d_lap.p, (rocblas_int)m));
+        } else {
+            if (blocked && (r = block_planes(c, k0, m, -1, c->d_lapplane.p))) return r;
+            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, n, N, &one,
+                                     c->d_lapplane.p, (rocblas_int)m, cf, N, &zero, c->d_lap.p, (rocblas_int)m));
+        }
+        // T_k = w (-1/2 lap phi_k + V phi_k), L = phi^T T
+        launch_weighted_op(n, m, c->d_w.p + k0, c->d_pot.p + k0, phi, c->d_lap.p, -0.5, c->d_wop.p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        if ((r = gemm_tn_points(c, n, n, phi, c->d_wop.p, m, c->d_small.p, k0 > 0))) return r;
+    }
     std::vector<double> L((size_t)n * n);
     HIPCHK(c, hipMemcpyAsync(L.data(), c->d_small.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < n; ++k)
         for (int j = 0; j < n; ++j) sic[j + (size_t)ldsic * k] += L[j + (size_t)n * k];
-    (void)nF;
     return FLOSIC_OK;
 }
 
@@ -1409,12 +1870,23 @@ int32_t flosic_sic_hamiltonian(flosic_ctx *c, int32_t ispn, double *hao) {
     std::fill(hao, hao + (size_t)N * N, 0.0);
     if (n == 0) return FLOSIC_OK;
     const int off = s == 0 ? 0 : c->sp[0].nocc;
+    const bool blocked = c->block_pts > 0;
+    const int64_t mb = mesh_block(c);
     // B_ai = <chi_a|phi_i> and G_ai = <chi_a|V_i|phi_i> on the mesh (G from the evaluation)
-    ALLOC(c, c->d_wop, (size_t)n * M);
-    launch_weighted_op(n, M, c->d_w.p, nullptr, c->d_phi.p + (size_t)off * M, nullptr, 0.0, c->d_wop.p, c->stream);
-    HIPCHK(c, hipGetLastError());
+    ALLOC(c, c->d_wop, (size_t)n * mb);
     ALLOC(c, c->d_B, (size_t)N * n);
-    if ((r = gemm_tn_points(c, N, n, c->d_planes.p, c->d_wop.p, c->d_B.p))) return r;
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        const double *phi = c->d_phi.p + (size_t)off * M;
+        if (blocked) {
+            if ((r = block_planes(c, k0, m, 1, c->d_planes.p))) return r;
+            if ((r = orbitals_on_points(c, c->d_planes.p, m, 1, c->d_cflo.p + (size_t)off * N, n, c->d_phi.p))) return r;
+            phi = c->d_phi.p;
+        }
+        launch_weighted_op(n, m, c->d_w.p + k0, nullptr, phi, nullptr, 0.0, c->d_wop.p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        if ((r = gemm_tn_points(c, N, n, c->d_planes.p, c->d_wop.p, m, c->d_B.p, k0 > 0))) return r;
+    }
     ALLOC(c, c->d_hao, (size_t)N * N);
     const double half = 0.5, zero = 0.0;
     BLASCHK(c, rocblas_dsyr2k(c->blas, rocblas_fill_upper, rocblas_operation_none, N, n, &half,
@@ -1441,7 +1913,66 @@ int32_t flosic_get_timings(flosic_ctx *c, int32_t n, double *ms) {
 
 int32_t flosic_get_counts(flosic_ctx *c, int32_t n, int64_t *counts) {
     if (!c || !counts || n < 0) return FLOSIC_EINVAL;
-    for (int i = 0; i < n && i < 7; ++i) counts[i] = c->counts[i];
+    for (int i = 0; i < n && i < 8; ++i) counts[i] = c->counts[i];
+    return FLOSIC_OK;
+}
+
+int32_t flosic_batch_create(int32_t n, flosic_ctx *const *ctxs, int32_t use_graph, flosic_batch **out) {
+    if (!out || n <= 0 || !ctxs) return FLOSIC_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return FLOSIC_EINVAL;
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return FLOSIC_EINVAL;
+    }
+    flosic_batch *b = new (std::nothrow) flosic_batch();
+    if (!b) return FLOSIC_ENOMEM;
+    b->device = ctxs[0]->device;
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->use_graph = use_graph != 0;
+    bool ok = hipSetDevice(b->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    for (hipEvent_t *e : {&b->ev_fork, &b->ev_coul, &b->ev_end, &b->ev_t0, &b->ev_c0, &b->ev_c1})
+        ok = ok && hipEventCreate(e) == hipSuccess;
+    if (!ok) {
+        flosic_batch_destroy(b);
+        return FLOSIC_EDEVICE;
+    }
+    *out = b;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_batch_evaluate(flosic_batch *b, double *zsic) {
+    if (!b) return FLOSIC_EINVAL;
+    return batch_evaluate(b, zsic);
+}
+
+const char *flosic_batch_last_error(const flosic_batch *b) { return b ? b->st.msg.c_str() : "null batch"; }
+
+int32_t flosic_batch_get_timings(flosic_batch *b, int32_t n, double *ms) {
+    if (!b || !ms || n < 0) return FLOSIC_EINVAL;
+    for (int i = 0; i < n && i < 4; ++i) ms[i] = b->timings[i];
+    return FLOSIC_OK;
+}
+
+int32_t flosic_batch_get_counts(flosic_batch *b, int32_t n, int64_t *counts) {
+    if (!b || !counts || n < 0) return FLOSIC_EINVAL;
+    int64_t items = 0;
+    for (const auto &k : b->classes) items += k.nitems;
+    const int64_t v[5] = {b->evals, b->captures, b->replays, (int64_t)b->classes.size(), items};
+    for (int i = 0; i < n && i < 5; ++i) counts[i] = v[i];
+    return FLOSIC_OK;
+}
+
+int32_t flosic_batch_destroy(flosic_batch *b) {
+    if (!b) return FLOSIC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (hipEvent_t e : {b->ev_fork, b->ev_coul, b->ev_end, b->ev_t0, b->ev_c0, b->ev_c1})
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
     return FLOSIC_OK;
 }
 

@@ -127,12 +127,19 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
                          const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
                          const double *btab, int nF, double *V, int64_t ldv, hipStream_t s);
 
-// XC + total potential + energy partials; phi: nd planes of [nF][M] (plane stride nF*M)
-void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w,
-               double *Vtot, double *U, double *partial /*[nF][nblk][9]*/, int nblk, hipStream_t s);
+// XC + total potential + energy partials over M points: phi nd planes of [nF][M] (plane stride nF*M), Vtot and U
+// [nF][M], VH[f * ldvh + k], w[k]; 256-point block b writes partial[(f * nblk + blk0 + b) * 9 + q]
+void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, int64_t ldvh,
+               const double *w, double *Vtot, double *U, double *partial /*[nF][nblk][9]*/, int nblk, int blk0,
+               hipStream_t s);
 
+// p[0..n) = 0 (a kernel: usable inside captured graphs); dst[i] = src[i * stride], i < n
+void launch_fill_zero(double *p, size_t n, hipStream_t s);
+void launch_gather(const double *src, int64_t stride, int n, double *dst, hipStream_t s);
+
+// out[f][q] (+)= sum over b of partial[f][b][q], compensated, in block order
 void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out /*[nF][nq]*/,
-                            hipStream_t s);
+                            hipStream_t s, bool accumulate = false);
 
 // V^i_ab for one spin: phi0 (n FLOs of this spin, [n][M]), Vtot ([n][M]), w -> partial sums [slice][i][b][a]
 int vab_slices(int64_t M);

@@ -1307,9 +1307,10 @@ __device__ __forceinline__ void xc_point(const double *rg, double &vxc, double *
 // 8 sum|V_H|
 template <int GGA>
 __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int64_t M, int nF,
-                                             const double *__restrict__ VH, const double *__restrict__ w,
-                                             double *__restrict__ Vtot, double *__restrict__ U,
-                                             double *__restrict__ partial, int nblk) {
+                                             const double *__restrict__ VH, int64_t ldvh,
+                                             const double *__restrict__ w, double *__restrict__ Vtot,
+                                             double *__restrict__ U, double *__restrict__ partial, int nblk,
+                                             int blk0) {
     __shared__ double red[9][256];
     const int f = blockIdx.y;
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1337,7 +1338,7 @@ __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int6
         double vxc, e4[4];
         xc_point<GGA>(rg, vxc, e4);
         const double wk = w[k];
-        const double vh = VH[off];
+        const double vh = VH[(int64_t)f * ldvh + k];
         const double vt = vxc + vh;
         Vtot[off] = vt;
         U[off] = wk * vt * ph;
@@ -1361,24 +1362,45 @@ __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int6
         }
         __syncthreads();
     }
-    if (threadIdx.x < 9) partial[((int64_t)f * nblk + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 9)
+        partial[((int64_t)f * nblk + blk0 + blockIdx.x) * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, const double *w, double *Vtot,
-               double *U, double *partial, int nblk, hipStream_t s) {
-    const dim3 grid((unsigned)nblk, (unsigned)nF), blk(256);
-    if (xc == 1) hipLaunchKernelGGL((k_xc<1>), grid, blk, 0, s, phi, M, nF, VH, w, Vtot, U, partial, nblk);
-    else hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, w, Vtot, U, partial, nblk);
+void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, int64_t ldvh,
+               const double *w, double *Vtot, double *U, double *partial, int nblk, int blk0, hipStream_t s) {
+    const dim3 grid((unsigned)((M + 255) / 256), (unsigned)nF), blk(256);
+    if (xc == 1) hipLaunchKernelGGL((k_xc<1>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0);
+    else hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0);
     (void)nd;
 }
 
-// deterministic compensated (Neumaier) reduction of per-block partials: thread per (f, q)
+__global__ void k_fill_zero(double *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+void launch_fill_zero(double *p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, p, (int64_t)n);
+}
+
+__global__ void k_gather(const double *__restrict__ src, int64_t stride, int n, double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(int64_t)i * stride];
+}
+void launch_gather(const double *src, int64_t stride, int n, double *dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, src, stride, n, dst);
+}
+
+// deterministic compensated (Neumaier) reduction of per-block partials: thread per (f, q); with accumulate the
+// sum starts from out
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nF, int nblk, int nq,
-                                  double *__restrict__ out) {
+                                  double *__restrict__ out, int accumulate) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nF * nq) return;
     const int f = i / nq, q = i % nq;
-    double s = 0.0, c = 0.0;
+    double s = accumulate ? out[i] : 0.0, c = 0.0;
     for (int b = 0; b < nblk; ++b) {
         const double x = partial[((int64_t)f * nblk + b) * nq + q];
         const double t = s + x;
@@ -1388,9 +1410,11 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int nF, in
     out[i] = s + c;
 }
 
-void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out, hipStream_t s) {
+void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, double *out, hipStream_t s,
+                            bool accumulate) {
     const int n = nF * nq;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((n + 127) / 128), dim3(128), 0, s, partial, nF, nblk, nq, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((n + 127) / 128), dim3(128), 0, s, partial, nF, nblk, nq, out,
+                       accumulate ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------------------------------

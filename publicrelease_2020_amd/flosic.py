@@ -68,6 +68,12 @@ def library():
             "flosic_get_pairs": [vp, _ip, _ip],
             "flosic_get_timings": [vp, C.c_int32, _dp],
             "flosic_get_counts": [vp, C.c_int32, _lp],
+            "flosic_set_mesh_block": [vp, C.c_int64],
+            "flosic_batch_create": [C.c_int32, C.POINTER(vp), C.c_int32, C.POINTER(vp)],
+            "flosic_batch_evaluate": [vp, _dp],
+            "flosic_batch_get_timings": [vp, C.c_int32, _dp],
+            "flosic_batch_get_counts": [vp, C.c_int32, _lp],
+            "flosic_batch_destroy": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -75,6 +81,8 @@ def library():
             fn.restype = C.c_int32
         L.flosic_last_error.argtypes = [vp]
         L.flosic_last_error.restype = C.c_char_p
+        L.flosic_batch_last_error.argtypes = [vp]
+        L.flosic_batch_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -84,7 +92,9 @@ def exported_symbols() -> list:
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
             "flosic_evaluate", "flosic_apotnl_sic", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
-            "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts"]
+            "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block",
+            "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
+            "flosic_batch_get_counts", "flosic_batch_destroy"]
 
 
 def _d(a):
@@ -241,9 +251,16 @@ class Flosic:
         return t
 
     def counts(self) -> np.ndarray:
-        c = np.zeros(7, np.int64)
-        self._chk(self._L.flosic_get_counts(self._h, 7, c.ctypes.data_as(_lp)))
+        c = np.zeros(8, np.int64)
+        self._chk(self._L.flosic_get_counts(self._h, 8, c.ctypes.data_as(_lp)))
         return c
+
+    def peak_device_bytes(self) -> int:
+        return int(self.counts()[7])
+
+    def set_mesh_block(self, block_points: int):
+        """0: resident basis planes and whole-mesh fields; > 0: mesh-block recomputation."""
+        self._chk(self._L.flosic_set_mesh_block(self._h, int(block_points)))
 
     def results(self) -> dict:
         """Everything the evaluation produced, per spin, in natural numpy layouts."""
@@ -252,3 +269,46 @@ class Flosic:
             out["spins"].append(dict(results=self.apotnl_sic(s), sic=self.siclagm(s), vab=self.sic_vab(s),
                                      debdax=self.fod_forces(s), grad=self.fod_gradient(s)))
         return out
+
+
+class FlosicBatch:
+    """Independent systems (Flosic contexts on one device) evaluated together: one batched Coulomb launch per
+    class, optionally the whole device part as a replayed hipGraph."""
+
+    def __init__(self, engines, use_graph: bool = False):
+        self._L = library()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e._h for e in self.engines])
+        h = C.c_void_p()
+        st = self._L.flosic_batch_create(len(self.engines), arr, 1 if use_graph else 0, C.byref(h))
+        if st != OK:
+            raise FlosicError(st, "flosic_batch_create failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.flosic_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate(self) -> np.ndarray:
+        z = np.zeros(len(self.engines))
+        st = self._L.flosic_batch_evaluate(self._h, _d(z))
+        if st != OK:
+            raise FlosicError(st, self._L.flosic_batch_last_error(self._h).decode())
+        return z
+
+    def timings(self) -> np.ndarray:
+        t = np.zeros(4)
+        self._L.flosic_batch_get_timings(self._h, 4, _d(t))
+        return t
+
+    def counts(self) -> np.ndarray:
+        c = np.zeros(5, np.int64)
+        self._L.flosic_batch_get_counts(self._h, 5, c.ctypes.data_as(_lp))
+        return c
