@@ -152,6 +152,171 @@ def copy_calibration(mib: int, reps: int) -> int:
     return 0
 
 
+def bench_batch(args) -> int:
+    """The batch workload: n_sys independent systems from the batch generator (K uniform in [4,96], LSDA or PBE
+    at random, 4..128 FODs per spin), every context in mesh-block mode.  Timed: the batch in the mixed form
+    with its device part captured once and replayed (the reported value), the same systems evaluated one at a
+    time, and the batch in the plain FP64 form (the accuracy reference)."""
+    import numpy as np
+
+    from publicrelease_2020_amd import flosic as F
+    from publicrelease_2020_amd import system as S
+
+    t0 = time.perf_counter()
+    systems = []
+    for i in range(args.batch_systems):
+        systems.append(S.make_config(5, index=i))
+        log(f"system {i}: K={systems[-1].meta['K']} nbas={systems[-1].nbas} npts={systems[-1].mesh.npts} "
+            f"nocc={systems[-1].nocc} xc={'PBE' if systems[-1].xc else 'LSDA'} "
+            f"({time.perf_counter() - t0:.0f} s)")
+
+    def engines(prec):
+        out = []
+        for i, sysm in enumerate(systems):
+            e = F.Flosic(0)
+            e.load_system(sysm, prec)
+            e.set_mesh_block(args.batch_block)
+            out.append(e)
+        log(f"{len(out)} contexts loaded ({time.perf_counter() - t0:.0f} s)")
+        return out
+
+    def forces(e):
+        return np.concatenate([e.fod_forces(1), e.fod_forces(2)])
+
+    mixed = engines(F.PREC_MIXED)
+    batch = F.FlosicBatch(mixed, use_graph=True)
+    z = batch.evaluate()  # eager: allocations, plan
+    t_eager = batch.timings()
+    log(f"mixed batch eager: {t_eager[0]:.0f} ms wall, Coulomb launches {t_eager[2]:.0f} ms")
+    z = batch.evaluate()  # captured
+    log(f"mixed batch captured: {batch.timings()[0]:.0f} ms wall")
+    walls = []
+    for _ in range(args.steps):
+        z = batch.evaluate()
+        walls.append(batch.timings()[0])
+        log(f"mixed batch replay: {walls[-1]:.0f} ms wall, device part {batch.timings()[1]:.0f} ms")
+    t_batch = float(np.mean(walls)) * 1e-3
+    f_mixed = [forces(e) for e in mixed]
+    peak = sum(e.peak_device_bytes() for e in mixed)
+    # the same systems one at a time
+    t1 = time.perf_counter()
+    seq_coul = 0.0
+    for e in mixed:
+        e.evaluate()
+        seq_coul += e.timings()[9]
+    t_seq = time.perf_counter() - t1
+    log(f"mixed one at a time: {t_seq * 1e3:.0f} ms, Coulomb kernels {seq_coul:.0f} ms")
+    counts = [e.counts() for e in mixed]
+    batch.close()
+    for e in mixed:
+        e.close()
+    # plain FP64 form (eager batch): accuracy reference and speed comparison
+    fp64 = engines(F.PREC_FP64)
+    b64 = F.FlosicBatch(fp64)
+    t1 = time.perf_counter()
+    z64 = b64.evaluate()
+    t_fp64 = time.perf_counter() - t1
+    t64 = b64.timings()
+    log(f"fp64 batch: {t_fp64 * 1e3:.0f} ms wall, Coulomb launches {t64[2]:.0f} ms")
+    dE = float(np.max(np.abs(z - z64)))
+    dF = float(max(np.max(np.abs(fm - forces(e))) for fm, e in zip(f_mixed, fp64)))
+    b64.close()
+    for e in fp64:
+        e.close()
+    flops = sum(coulomb_flops(c) for c in counts)
+    achieved = flops / (t_eager[2] * 1e-3) / 1e12
+    n = len(systems)
+    out = {
+        "metric": "FLO-SIC E+FOD-force evals/s, mixed vs FP64 on one GPU at |dE|<1e-6 Ha",
+        "value": n / t_batch, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch of {n} synthetic systems, K uniform in [4,96], LSDA/PBE, 60x194 Becke "
+                               f"meshes, mesh blocks of {args.batch_block} points, one step = one evaluation of "
+                               f"every system", "systems": n,
+                   "atoms": [s.meta["K"] for s in systems], "npts_total": int(sum(s.mesh.npts for s in systems)),
+                   "nflo_total": int(sum(sum(s.nocc) for s in systems))},
+        "batch_graph_s": t_batch, "one_at_a_time_s": t_seq, "fp64_batch_s": t_fp64,
+        "speedup_vs_one_at_a_time": t_seq / t_batch, "speedup_vs_fp64": t_fp64 / t_batch,
+        "max_dE_Ha": dE, "max_dF_Ha_per_bohr": dF, "accepted": bool(dE < E_TOL and dF < F_TOL),
+        "peak_device_bytes": int(peak),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["mixed"][1], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["mixed"][1], "traffic": None,
+                     "kernel": "k_coulomb_ws_batch<float, *> (all classes)", "kernel_ms": t_eager[2],
+                     "flops_per_launch": flops, "one_at_a_time_kernel_ms": seq_coul},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """The differentiation workload: K=32, PBE, 48 FODs per spin.  FP64 evaluations with resident basis planes
+    and with mesh-block recomputation (time and peak device memory of each), the FOD gradient against central
+    differences of E_SIC (h = 1e-4 bohr) on a few coordinates, and the mixed-precision gradient against FP64."""
+    import numpy as np
+
+    from publicrelease_2020_amd import flosic as F
+    from publicrelease_2020_amd import system as S
+
+    sysm = S.make_config(4)
+    log(f"workload K={sysm.meta['K']} nbas={sysm.nbas} npts={sysm.mesh.npts} nocc={sysm.nocc} PBE")
+    modes = {}
+    for name, block in (("resident", 0), ("mesh_block", args.adjoint_block)):
+        e = F.Flosic(0)
+        e.load_system(sysm, F.PREC_FP64)
+        e.set_mesh_block(block)
+        e.evaluate()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            z = e.evaluate()
+        dt = (time.perf_counter() - t0) / args.steps
+        g = np.concatenate([e.fod_gradient(1), e.fod_gradient(2)])
+        modes[name] = dict(ms_per_eval=dt * 1e3, peak_device_bytes=e.peak_device_bytes(), esic=z, grad=g,
+                           debdax=np.concatenate([e.fod_forces(1), e.fod_forces(2)]), engine=e)
+        log(f"{name}: {dt * 1e3:.1f} ms/eval, peak device memory {e.peak_device_bytes() / 2**30:.2f} GiB")
+    e = modes["mesh_block"]["engine"]
+    ref = modes["resident"]
+    h = 1e-4
+    rng = np.random.default_rng(4)
+    coords = [(int(rng.integers(2)), int(rng.integers(sysm.nocc[0])), int(rng.integers(3)))
+              for _ in range(args.fd_coords)]
+    fd_err, deb_err = [], []
+    for (s, i, x) in coords:
+        fods = [np.array(f, dtype=np.float64) for f in sysm.fod]
+        fods[s][i, x] += h
+        e.set_fods(1, fods[0])
+        e.set_fods(2, fods[1])
+        ep = e.evaluate()
+        fods[s][i, x] -= 2 * h
+        e.set_fods(1, fods[0])
+        e.set_fods(2, fods[1])
+        em = e.evaluate()
+        fd = (ep - em) / (2 * h)
+        k = s * sysm.nocc[0] + i
+        fd_err.append(abs(fd - ref["grad"][k, x]))
+        deb_err.append(abs(fd - ref["debdax"][k, x]))
+        log(f"coordinate {(s, i, x)}: FD {fd:.9f} adjoint {ref['grad'][k, x]:.9f} FRMORB2 {ref['debdax'][k, x]:.9f}")
+    mixed = F.Flosic(0)
+    mixed.load_system(sysm, F.PREC_MIXED)
+    mixed.set_mesh_block(args.adjoint_block)
+    zm = mixed.evaluate()
+    gm = np.concatenate([mixed.fod_gradient(1), mixed.fod_gradient(2)])
+    out = {
+        "metric": "FOD gradient of E_SIC through Lowdin and the mesh contractions, K=32 PBE",
+        "config": {"workload": "synthetic K=32 ladder basis, 60x194 Becke mesh, PBE, 48 FODs per spin",
+                   "nbas": sysm.nbas, "npts": sysm.mesh.npts, "mesh_block_points": args.adjoint_block},
+        "resident": {k: modes["resident"][k] for k in ("ms_per_eval", "peak_device_bytes", "esic")},
+        "mesh_block": {k: modes["mesh_block"][k] for k in ("ms_per_eval", "peak_device_bytes", "esic")},
+        "dE_block_vs_resident": abs(modes["mesh_block"]["esic"] - ref["esic"]),
+        "max_dgrad_block_vs_resident": float(np.abs(modes["mesh_block"]["grad"] - ref["grad"]).max()),
+        "fd_h_bohr": h, "fd_coordinates": coords, "max_fd_error_adjoint": float(max(fd_err)),
+        "max_fd_error_frmorb2_formula": float(max(deb_err)),
+        "mixed_dE": abs(zm - ref["esic"]), "mixed_max_dgrad": float(np.abs(gm - ref["grad"]).max()),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +327,13 @@ def main() -> int:
     ap.add_argument("--forms", default="mixed,bf16x3", help="reduced-precision forms to time after FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256, help="mesh points in the CPU oracle sample")
+    ap.add_argument("--workload", default="precision", choices=("precision", "batch", "adjoint"),
+                    help="precision: one system, FP64 and reduced forms; batch: many systems in one batch; "
+                         "adjoint: K=32 PBE gradient, recomputation and memory")
+    ap.add_argument("--adjoint-block", type=int, default=65536, help="mesh-block points of the adjoint workload")
+    ap.add_argument("--fd-coords", type=int, default=6, help="FOD coordinates checked by central differences")
+    ap.add_argument("--batch-systems", type=int, default=48)
+    ap.add_argument("--batch-block", type=int, default=8192, help="mesh-block points of the batch contexts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run device copies of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -171,6 +343,10 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, 5)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import numpy as np
 

@@ -773,6 +773,74 @@ void oracle_xc_points(int xc, int n, const double *rhog /* 10 x n */, double *vx
     for (int i = 0; i < n; ++i) xc_point(xc, rhog + 10 * (size_t)i, vxc + i, e4 + 4 * (size_t)i);
 }
 
+/* Partial derivatives of the PBE energy density e(rho, g) = EXL+EXN+ECL+ECN of xc_point (the energy depends on
+ * the orbital density only through rho and g = |grad rho|: pbeex.f's EXUNIF*FXPBE at D = 2 rho, S = g/(kF D);
+ * pbecor.f's EC + H at zeta = 1, T = g/(2 G ks rho)), by forward-mode differentiation: every quantity carries
+ * (value, derivative). */
+typedef struct { double v, d; } fwd;
+static fwd f_c(double v) { fwd r = {v, 0.0}; return r; }
+static fwd f_add(fwd a, fwd b) { fwd r = {a.v + b.v, a.d + b.d}; return r; }
+static fwd f_sub(fwd a, fwd b) { fwd r = {a.v - b.v, a.d - b.d}; return r; }
+static fwd f_mul(fwd a, fwd b) { fwd r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static fwd f_div(fwd a, fwd b) { fwd r = {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; return r; }
+static fwd f_scl(double c, fwd a) { fwd r = {c * a.v, c * a.d}; return r; }
+static fwd f_pow(fwd a, double e) { double p = pow(a.v, e); fwd r = {p, e * pow(a.v, e - 1.0) * a.d}; return r; }
+static fwd f_log(fwd a) { fwd r = {log(a.v), a.d / a.v}; return r; }
+static fwd f_exp(fwd a) { double x = exp(a.v); fwd r = {x, x * a.d}; return r; }
+
+static fwd gcor2_f(double A, double A1, double B1, double B2, double B3, double B4, fwd rtrs) {
+    fwd q0 = f_scl(-2.0 * A, f_add(f_c(1.0), f_scl(A1, f_mul(rtrs, rtrs))));
+    fwd poly = f_add(f_c(B3), f_scl(B4, rtrs));
+    poly = f_add(f_c(B2), f_mul(rtrs, poly));
+    poly = f_add(f_c(B1), f_mul(rtrs, poly));
+    fwd q1 = f_scl(2.0 * A, f_mul(rtrs, poly));
+    return f_mul(q0, f_log(f_add(f_c(1.0), f_div(f_c(1.0), q1))));
+}
+
+static fwd pbe_energy_density(fwd rho, fwd g) {
+    const double THIRD = 1.0 / 3.0, TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627;
+    const double DKF2RS = 1.9191582926775128, AX = -0.738558766382022405884230032680836;
+    const double UM = 0.2195149727645171, UK = 0.8040, UL = UM / UK;
+    const double GAM = 0.5198420997897463295344212145565, GAMMA = 0.03109069086965489503494086371273;
+    const double BET = 0.06672455060314922, DELT = BET / GAMMA;
+    /* exchange of the up spin at D = 2 rho */
+    fwd D = f_scl(2.0, rho);
+    fwd kF = f_pow(f_scl(TRPI2, D), THIRD);
+    fwd S = f_div(g, f_mul(kF, D));
+    fwd exunif = f_scl(AX, f_pow(D, THIRD));
+    fwd fx = f_sub(f_c(1.0 + UK), f_div(f_c(UK), f_add(f_c(1.0), f_scl(UL, f_mul(S, S)))));
+    fwd ex = f_scl(0.5, f_mul(f_mul(exunif, fx), D));
+    /* correlation, zeta = 1 */
+    fwd kFc = f_pow(f_scl(TRPI2, rho), THIRD);
+    fwd rs = f_div(f_c(DKF2RS), kFc);
+    fwd rtrs = f_pow(rs, 0.5);
+    fwd eu = gcor2_f(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, rtrs);
+    fwd ep = gcor2_f(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, rtrs);
+    double F = (pow(2.0, 4.0 / 3.0) - 2.0) / GAM;
+    fwd ec = f_add(f_scl(1.0 - F, eu), f_scl(F, ep));
+    double G = 0.5 * pow(2.0, 2.0 / 3.0), G3 = G * G * G;
+    fwd ks = f_pow(f_scl(FRDPI, kFc), 0.5);
+    fwd T = f_div(g, f_mul(f_scl(2.0 * G, ks), rho));
+    fwd B = f_div(f_c(DELT), f_sub(f_exp(f_scl(-1.0 / (G3 * GAMMA), ec)), f_c(1.0)));
+    fwd T2 = f_mul(T, T);
+    fwd q4 = f_add(f_c(1.0), f_mul(B, T2));
+    fwd q5 = f_add(q4, f_mul(f_mul(B, B), f_mul(T2, T2)));
+    fwd H = f_scl(G3 * BET / DELT, f_log(f_add(f_c(1.0), f_scl(DELT, f_div(f_mul(q4, T2), q5)))));
+    return f_add(ex, f_mul(rho, f_add(ec, H)));
+}
+
+/* e, de/drho, de/dg at n points (rho, g); zero below rho = 1e-20 (xc_point's cut) */
+void oracle_pbe_edens(int n, const double *rho, const double *g, double *e, double *er, double *eg) {
+    for (int i = 0; i < n; ++i) {
+        if (!(rho[i] >= 1.0e-20)) { e[i] = er[i] = eg[i] = 0.0; continue; }
+        fwd a = pbe_energy_density((fwd){rho[i], 1.0}, (fwd){g[i], 0.0});
+        fwd b = pbe_energy_density((fwd){rho[i], 0.0}, (fwd){g[i], 1.0});
+        e[i] = a.v;
+        er[i] = a.d;
+        eg[i] = b.d;
+    }
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* compensated summation                                                                             */
 /* ------------------------------------------------------------------------------------------------ */
@@ -1050,6 +1118,8 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
     free(H); free(PL); free(PP); free(ps); free(pr);
     /* XC and energies per FLO */
     double *VX = calloc((size_t)ntot * npts, sizeof(double));
+    double *ER = xc == 1 ? calloc((size_t)ntot * npts, sizeof(double)) : NULL; /* de/drho */
+    double *EQ = xc == 1 ? calloc((size_t)ntot * npts, sizeof(double)) : NULL; /* (de/dg) / g */
     int kmax = 0;
     {
         double rmx = 0;
@@ -1089,6 +1159,12 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
                 xc_point(xc, rg, &vxc, e4);
                 double wk = w[k];
                 VX[(size_t)f * npts + k] = vxc;
+                if (ER) {  /* exact derivatives of e(rho, g) for the FOD gradient */
+                    double gg = sqrt(rg[1] * rg[1] + rg[2] * rg[2] + rg[3] * rg[3]), ee, er, eg;
+                    oracle_pbe_edens(1, &rg[0], &gg, &ee, &er, &eg);
+                    ER[(size_t)f * npts + k] = er;
+                    EQ[(size_t)f * npts + k] = gg > 1.0e-30 ? eg / gg : 0.0;
+                }
                 double vh = VH[(size_t)f * npts + k];
                 addit += fabs(vh);
                 kadd(&exl, e4[0] * wk); kadd(&exn, e4[1] * wk); kadd(&ecl, e4[2] * wk); kadd(&ecn, e4[3] * wk);
@@ -1172,14 +1248,51 @@ int oracle_eval(const ob_basis *b, int64_t npts, const double *pts, const double
                 }
         }
         free(cj);
+        /* PBE: the FOD gradient takes the exact mesh derivative of E_SIC,i along every FLO j of the spin,
+         * SIC_ex(j,i) = -sum_k w [((e_rho + V_H) phi_i + 2 q phi_i |grad phi_i|^2) phi_j
+         *                         + 2 q phi_i^2 grad phi_i . grad phi_j],  q = (de/dg)/g,
+         * in place of SIC(j,i), whose GGA potential (Laplacian/Hessian form) is the functional derivative of
+         * E_xc only in the continuum */
+        double *sic_ex = NULL;
+        if (ER && spins[s].grad) {
+            sic_ex = calloc((size_t)n * n, sizeof(double));
+#pragma omp parallel for collapse(2) schedule(dynamic)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    int fi = off[s] + i, fj = off[s] + j;
+                    ksum acc = {0, 0};
+                    for (int64_t k = 0; k < npts; ++k) {
+                        double pi = phi[(size_t)fi * npts + k], pj = phi[(size_t)fj * npts + k];
+                        double gi2 = 0, gij = 0;
+                        for (int d = 1; d <= 3; ++d) {
+                            double a = phi[((size_t)d * ntot + fi) * npts + k], bj = phi[((size_t)d * ntot + fj) * npts + k];
+                            gi2 += a * a;
+                            gij += a * bj;
+                        }
+                        double q = EQ[(size_t)fi * npts + k];
+                        double v = ER[(size_t)fi * npts + k] + VH[(size_t)fi * npts + k];
+                        kadd(&acc, -w[k] * ((v * pi + 2.0 * q * pi * gi2) * pj + 2.0 * q * pi * pi * gij));
+                    }
+                    sic_ex[j + (size_t)n * i] = kval(&acc);
+                }
+        }
         /* stage D: FOD forces from SIC(1:n,1:n) */
         t_mark = wall_s();
-        if (spins[s].sic && (spins[s].debdax || spins[s].grad))
+        if (spins[s].sic && spins[s].debdax)
             oracle_fod_forces(n, spins[s].fmat, spins[s].U, spins[s].q, sinvh[s], spins[s].tmat, spins[s].sic, nwf,
-                              spins[s].debdax, spins[s].grad);
+                              spins[s].debdax, NULL);
+        if (spins[s].grad) {
+            if (sic_ex)
+                oracle_fod_forces(n, spins[s].fmat, spins[s].U, spins[s].q, sinvh[s], spins[s].tmat, sic_ex, n, NULL,
+                                  spins[s].grad);
+            else if (spins[s].sic)
+                oracle_fod_forces(n, spins[s].fmat, spins[s].U, spins[s].q, sinvh[s], spins[s].tmat, spins[s].sic,
+                                  nwf, NULL, spins[s].grad);
+        }
+        free(sic_ex);
         t_fixed += wall_s() - t_mark;
     }
-    free(chi); free(phi); free(VH); free(VX); free(cflo);
+    free(chi); free(phi); free(VH); free(VX); free(ER); free(EQ); free(cflo);
     free(sinvh[0]); free(sinvh[1]);
     g_stage_fixed = t_fixed;
     g_stage_mesh = wall_s() - t_start - t_fixed;

@@ -63,6 +63,7 @@ def lib():
         L.oracle_flo_coef.argtypes = [C.POINTER(ObBasis), C.POINTER(ObSpin), _dp]
         L.oracle_lagmult.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_sic_hamiltonian.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_pbe_edens.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_nbas.restype = C.c_int
         L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
         _lib = L
@@ -133,6 +134,15 @@ def xc_points(xc: int, rhog) -> tuple:
     e = np.zeros((n, 4))
     lib().oracle_xc_points(xc, n, _d(rhog), _d(v), _d(e))
     return v, e
+
+
+def pbe_edens(rho, g) -> tuple:
+    """PBE energy density e(rho, |grad rho|) of a fully polarised density and its exact partial derivatives."""
+    rho = np.ascontiguousarray(rho, np.float64).ravel()
+    g = np.ascontiguousarray(g, np.float64).ravel()
+    e, er, eg = np.zeros(rho.size), np.zeros(rho.size), np.zeros(rho.size)
+    lib().oracle_pbe_edens(rho.size, _d(rho), _d(g), _d(e), _d(er), _d(eg))
+    return e, er, eg
 
 
 def xc_kernel(which: int, args) -> np.ndarray:

@@ -89,6 +89,7 @@ struct Spin {
     // results
     std::vector<double> results;  // 13 x n
     std::vector<double> sic;      // nwf x n
+    std::vector<double> sic_ex;   // n x n: exact mesh derivative form of SIC(1:n,1:n) (PBE; LSDA uses sic)
     std::vector<double> vab;      // n^3
     std::vector<double> debdax, grad;  // 3 x n
     DevBuf<double> d_ctil;        // nbas x nwf: FLOs then virtual KS orbitals
@@ -153,7 +154,7 @@ struct flosic_ctx {
     // evaluation buffers
     int nF = 0, ldh = 0;
     DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_Gpart, d_vabpart, d_vab,
-        d_lambda, d_asy;
+        d_lambda, d_asy, d_Ux, d_lamex;
     std::vector<double> h_energy, h_asy;
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
@@ -570,15 +571,15 @@ void force_formula(const Spin &s, double *deb) {
 }
 
 // adjoint: Gbar = 2 SIC^T TMAT (= dE/dTMAT), back-propagated through TMAT = S^-1/2 T and S = T T^T
-void force_adjoint(const Spin &s, double *grad) {
-    const int n = s.nocc, ld = s.nwf;
+void force_adjoint(const Spin &s, const double *sic, int ld, double *grad) {
+    const int n = s.nocc;
     const size_t nn = (size_t)n * n;
     auto F = [&](int j, int i, int K) { return s.fmat[j + (size_t)n * (i + (size_t)n * K)]; };
     std::vector<double> G(nn), B(nn), Bt(nn), W(nn), tmp(nn), Tb(nn);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             double acc = 0.0;
-            for (int k = 0; k < n; ++k) acc += s.sic[k + (size_t)ld * i] * s.tmat[k + (size_t)n * j];
+            for (int k = 0; k < n; ++k) acc += sic[k + (size_t)ld * i] * s.tmat[k + (size_t)n * j];
             G[i + (size_t)n * j] = 2.0 * acc;
         }
     for (int i = 0; i < n; ++i)  // B = G T^T
@@ -1073,6 +1074,13 @@ int eval_fields(flosic_ctx *c) {
     ALLOC(c, c->d_Vtot, (size_t)nF * mb);
     ALLOC(c, c->d_U, (size_t)nF * mb);
     if (blocked) ALLOC(c, c->d_planes, (size_t)nd * N * mb);
+    const bool exact = c->xc == FLOSIC_XC_PBE;  // exact mesh derivative for the FOD gradient
+    size_t lamex_cnt = 0;
+    for (int s = 0; s < 2; ++s) lamex_cnt += (size_t)c->sp[s].nocc * c->sp[s].nocc;
+    if (exact) {
+        ALLOC(c, c->d_Ux, (size_t)4 * nF * mb);
+        ALLOC(c, c->d_lamex, lamex_cnt);
+    }
     size_t vab_cnt = 0, vpart_cnt = 0;
     for (int s = 0; s < 2; ++s) {
         const size_t n = (size_t)c->sp[s].nocc;
@@ -1088,9 +1096,22 @@ int eval_fields(flosic_ctx *c) {
         if ((r = orbitals_on_points(c, planes, m, nd, c->d_cflo.p, nF, c->d_phi.p))) return r;
         if (!blocked) MARK(c, EV_PHI);
         launch_xc(c->xc, nd, c->d_phi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_Vtot.p, c->d_U.p, c->d_part.p,
-                  nblk, (int)(k0 / 256), c->stream);
+                  nblk, (int)(k0 / 256), c->stream, exact ? c->d_Ux.p : nullptr);
         HIPCHK(c, hipGetLastError());
         if (!blocked) MARK(c, EV_XC);
+        if (exact) {  // -SIC_ex(j,i) = sum_k [Ux0_i phi_j + sum_d Ux_d,i d_d phi_j], FLOs of one spin
+            size_t lo = 0;
+            for (int s = 0; s < 2; ++s) {
+                const int n = c->sp[s].nocc;
+                const size_t off = s == 0 ? 0 : (size_t)n0;
+                for (int d = 0; d < 4 && n > 0; ++d)
+                    if ((r = gemm_tn_points(c, n, n, c->d_phi.p + ((size_t)d * nF + off) * m,
+                                            c->d_Ux.p + ((size_t)d * nF + off) * m, m, c->d_lamex.p + lo,
+                                            k0 > 0 || d > 0)))
+                        return r;
+                lo += (size_t)n * n;
+            }
+        }
         // G = Phi_0^T U over the points (split K, fixed-order partial sums)
         if ((r = gemm_tn_points(c, N, nF, planes, c->d_U.p, m, c->d_G.p, k0 > 0))) return r;
         if (!blocked) MARK(c, EV_SIC);
@@ -1157,7 +1178,20 @@ int eval_finish(flosic_ctx *c) {
             vo += (size_t)n * n * n;
         }
     }
+    if (c->xc == FLOSIC_XC_PBE) {
+        size_t lo = 0;
+        for (int s = 0; s < 2; ++s) {
+            Spin &sp = c->sp[s];
+            const size_t nn = (size_t)sp.nocc * sp.nocc;
+            sp.sic_ex.resize(nn);
+            if (nn) HIPCHK(c, hipMemcpyAsync(sp.sic_ex.data(), c->d_lamex.p + lo, sizeof(double) * nn,
+                                             hipMemcpyDeviceToHost, c->stream));
+            lo += nn;
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < 2; ++s)
+        for (double &x : c->sp[s].sic_ex) x = -x;
     // RESULTS(1:13) per FLO (apotnl_sic.F:564-623)
     c->zsic = 0.0;
     const double e13 = (double)(1.0f / 3.0f);
@@ -1193,7 +1227,8 @@ int eval_finish(flosic_ctx *c) {
         sp.grad.assign(3 * (size_t)sp.nocc, 0.0);
         if (sp.nocc > 0) {
             force_formula(sp, sp.debdax.data());
-            force_adjoint(sp, sp.grad.data());
+            if (c->xc == FLOSIC_XC_PBE) force_adjoint(sp, sp.sic_ex.data(), sp.nocc, sp.grad.data());
+            else force_adjoint(sp, sp.sic.data(), sp.nwf, sp.grad.data());
         }
     }
     HIPCHK(c, hipEventRecord(c->ev[EV_END], c->stream));

@@ -128,10 +128,12 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
                          const double *btab, int nF, double *V, int64_t ldv, hipStream_t s);
 
 // XC + total potential + energy partials over M points: phi nd planes of [nF][M] (plane stride nF*M), Vtot and U
-// [nF][M], VH[f * ldvh + k], w[k]; 256-point block b writes partial[(f * nblk + blk0 + b) * 9 + q]
+// [nF][M], VH[f * ldvh + k], w[k]; 256-point block b writes partial[(f * nblk + blk0 + b) * 9 + q].  For PBE,
+// Ux (4 planes of [nF][M], or null) receives the exact mesh derivative of the orbital's E_xc + Hartree term:
+// plane 0 w[(e_rho + V_H) phi + 2 q phi |grad phi|^2], planes 1..3 w 2 q phi^2 d_x,y,z phi with q = e_g / g
 void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, int64_t ldvh,
                const double *w, double *Vtot, double *U, double *partial /*[nF][nblk][9]*/, int nblk, int blk0,
-               hipStream_t s);
+               hipStream_t s, double *Ux = nullptr);
 
 // p[0..n) = 0 (a kernel: usable inside captured graphs); dst[i] = src[i * stride], i < n
 void launch_fill_zero(double *p, size_t n, hipStream_t s);

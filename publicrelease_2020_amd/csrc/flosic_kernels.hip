@@ -1237,6 +1237,79 @@ __device__ __forceinline__ void pbecor_pol(double RS, double T, double UU, doubl
     DVCUP = comm + PREF;
 }
 
+// ---------------------------------------------------------------------------------------------------
+// Exact partial derivatives of the PBE energy density.  The energy that xc_point integrates depends on the
+// orbital density only through rho and g = |grad rho| (U, V, UU, VV enter the potential alone), so
+// e(rho, g) = EXL + EXN + ECL + ECN is restated below as one expression and differentiated in forward mode
+// (value + derivative pairs), once along rho and once along g.
+// ---------------------------------------------------------------------------------------------------
+struct Fwd {
+    double v, d;
+};
+__device__ __forceinline__ Fwd operator+(Fwd a, Fwd b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Fwd operator-(Fwd a, Fwd b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Fwd operator*(Fwd a, Fwd b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Fwd operator/(Fwd a, Fwd b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ Fwd operator+(double a, Fwd b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Fwd operator-(double a, Fwd b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ Fwd operator*(double a, Fwd b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Fwd operator/(double a, Fwd b) { return {a / b.v, -a * b.d / (b.v * b.v)}; }
+__device__ __forceinline__ Fwd operator*(Fwd a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Fwd fpow(Fwd a, double e) {
+    const double p = pow(a.v, e);
+    return {p, e * p / a.v * a.d};
+}
+__device__ __forceinline__ Fwd fsqrt(Fwd a) {
+    const double r = sqrt(a.v);
+    return {r, 0.5 / r * a.d};
+}
+__device__ __forceinline__ Fwd flog(Fwd a) { return {log(a.v), a.d / a.v}; }
+__device__ __forceinline__ Fwd fexp(Fwd a) {
+    const double e = exp(a.v);
+    return {e, e * a.d};
+}
+
+// the value part of gcor2 (pbecor.f) in forward mode
+__device__ __forceinline__ Fwd gcor2_fwd(double A, double A1, double B1, double B2, double B3, double B4, Fwd RTRS) {
+    const Fwd Q0 = -2.0 * A * (1.0 + A1 * (RTRS * RTRS));
+    const Fwd Q1 = 2.0 * A * (RTRS * (B1 + RTRS * (B2 + RTRS * (B3 + B4 * RTRS))));
+    return Q0 * flog(1.0 + 1.0 / Q1);
+}
+
+// e(rho, g): PBE exchange of the up spin at D = 2 rho (pbeex.f) + PBE correlation at zeta = 1 (pbecor.f), per
+// unit volume, with the same constants as xc_point
+__device__ __forceinline__ Fwd pbe_edens(Fwd rho, Fwd g) {
+    const double THIRD = 0.3333333333333333, TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627;
+    const double DKF2RS = 1.9191582926775128;
+    const double AX = -0.738558766382022405884230032680836, UM = 0.2195149727645171, UK = 0.8040, UL = UM / UK;
+    const double GAM = 0.5198420997897463295344212145565;
+    const double GAMMA = 0.03109069086965489503494086371273, BET = 0.06672455060314922, DELT = BET / GAMMA;
+    // exchange: S = 2g / (2 kF D)
+    const Fwd D = 2.0 * rho;
+    const Fwd DKF = fpow(TRPI2 * D, THIRD);
+    const Fwd S = (2.0 * g) * (0.5 / (DKF * D));
+    const Fwd EXUNIF = AX * fpow(D, THIRD);
+    const Fwd FX = (1.0 + UK) - UK / (1.0 + UL * (S * S));
+    const Fwd ex = 0.5 * ((EXUNIF * FX) * D);
+    // correlation: T = g / (2 G ks rho)
+    const Fwd DKFc = fpow(TRPI2 * rho, THIRD);
+    const Fwd RS = DKF2RS / DKFc;
+    const Fwd RTRS = fsqrt(RS);
+    const Fwd EU = gcor2_fwd(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, RTRS);
+    const Fwd EP = gcor2_fwd(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, RTRS);
+    const double F = (pow(2.0, 4.0 / 3.0) - 2.0) / GAM;
+    const Fwd ec = EU * Fwd{1.0 - F, 0.0} + EP * Fwd{F, 0.0};
+    const double G = 0.5 * pow(2.0, 0.6666666666666667), G3 = G * G * G;
+    const Fwd DKS = fsqrt(FRDPI * DKFc);
+    const Fwd T = g * (0.5 / (DKS * G)) / rho;
+    const Fwd B = DELT / (fexp((-1.0 / (G3 * GAMMA)) * ec) - Fwd{1.0, 0.0});
+    const Fwd T2 = T * T;
+    const Fwd Q4 = 1.0 + B * T2;
+    const Fwd Q5 = 1.0 + B * T2 + (B * B) * (T2 * T2);
+    const Fwd H = (G3 * (BET / DELT)) * flog(1.0 + DELT * (Q4 * T2) / Q5);
+    return ex + rho * (ec + H);
+}
+
 // rg: rho, grad(3), Hessian xx,yy,zz,xy,xz,yz of the orbital density; returns v_xc and EXL,EXN,ECL,ECN
 // densities per unit weight
 template <int GGA>
@@ -1310,7 +1383,7 @@ __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int6
                                              const double *__restrict__ VH, int64_t ldvh,
                                              const double *__restrict__ w, double *__restrict__ Vtot,
                                              double *__restrict__ U, double *__restrict__ partial, int nblk,
-                                             int blk0) {
+                                             int blk0, double *__restrict__ Ux) {
     __shared__ double red[9][256];
     const int f = blockIdx.y;
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1342,6 +1415,25 @@ __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int6
         const double vt = vxc + vh;
         Vtot[off] = vt;
         U[off] = wk * vt * ph;
+        if (GGA && Ux) {
+            // exact mesh derivative of E_xc + the Hartree term: dE/dphi = w[(e_rho + vh) phi + 2 q phi |grad phi|^2],
+            // dE/d(grad phi) = w 2 q phi^2 grad phi, q = e_g / g
+            double er = 0.0, qg = 0.0, g2phi = 0.0;
+            const double rho = rg[0];
+            if (rho >= 1.0e-20) {
+                const double g = sqrt(rg[1] * rg[1] + rg[2] * rg[2] + rg[3] * rg[3]);
+                er = pbe_edens(Fwd{rho, 1.0}, Fwd{g, 0.0}).d;
+                if (g > 1.0e-30) qg = pbe_edens(Fwd{rho, 0.0}, Fwd{g, 1.0}).d / g;
+            }
+            const int64_t plane = (int64_t)nF * M;
+            double gp[3];
+            for (int d = 0; d < 3; ++d) {
+                gp[d] = phi[(1 + d) * plane + off];
+                g2phi += gp[d] * gp[d];
+            }
+            Ux[off] = wk * ((er + vh) * ph + 2.0 * qg * ph * g2phi);
+            for (int d = 0; d < 3; ++d) Ux[(1 + d) * plane + off] = wk * 2.0 * qg * ph * ph * gp[d];
+        }
         q[0] = wk * rg[0];
         q[1] = wk * rg[0] * vh;
         q[2] = wk * pow(fabs(rg[0]), (double)(4.0f / 3.0f));
@@ -1367,10 +1459,14 @@ __global__ void __launch_bounds__(256) k_xc(const double *__restrict__ phi, int6
 }
 
 void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const double *VH, int64_t ldvh,
-               const double *w, double *Vtot, double *U, double *partial, int nblk, int blk0, hipStream_t s) {
+               const double *w, double *Vtot, double *U, double *partial, int nblk, int blk0, hipStream_t s,
+               double *Ux) {
     const dim3 grid((unsigned)((M + 255) / 256), (unsigned)nF), blk(256);
-    if (xc == 1) hipLaunchKernelGGL((k_xc<1>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0);
-    else hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0);
+    if (xc == 1)
+        hipLaunchKernelGGL((k_xc<1>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0, Ux);
+    else
+        hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0,
+                           (double *)nullptr);
     (void)nd;
 }
 

@@ -1,12 +1,10 @@
 """E_SIC(FOD positions) through torch.autograd: the backward pass is the library's FOD gradient.
 
-LSDA: central differences of the GPU energy (h = 1e-4 bohr, FP64) agree with it to 1e-5 Ha/bohr (the residual
-is the mesh quadrature of the Coulomb matrix elements; the oracle shows the same 3e-6 on this 80x302 mesh).
-PBE: the FRMORB2 force uses <psi_j|V_i|phi_i> with the Hessian-based GGA potential sampled on the mesh, whose
-quadrature error does not shrink quickly with the mesh (oracle, same system: FD 0.01244 / 0.01404 / 0.01396
-against 0.01558 / 0.00618 / 0.01192 on 40x110 / 80x302 / 120x590).  There the test pins the GPU to the
-oracle instead: the gradient to 1e-9, and central differences of the energy to 1e-6 (energy parity of 1e-10
-Ha divided by 2h)."""
+Central differences of the GPU energy (h = 1e-4 bohr, FP64) agree with it to 1e-5 Ha/bohr for LSDA and PBE
+(the residual is the mesh quadrature of the Coulomb matrix elements; the oracle shows the same 2e-6..3e-6 on this
+80x302 mesh).  For PBE the gradient takes the exact mesh derivative of the discrete E_xc (e_rho, e_g of the
+energy density), not the FRMORB2 force with the Hessian-form GGA potential, which misses the same differences by
+up to 8e-3 Ha/bohr here; the gradient is also pinned to the oracle's to 1e-9."""
 import numpy as np
 import pytest
 import torch
@@ -30,7 +28,8 @@ def test_autograd_gradient_matches_fd(xc):
     e.backward()
     np.testing.assert_allclose(up.grad.numpy(), eng.fod_gradient(1), rtol=0, atol=0)
     np.testing.assert_allclose(dn.grad.numpy(), eng.fod_gradient(2), rtol=0, atol=0)
-    np.testing.assert_allclose(up.grad.numpy(), eng.fod_forces(1), atol=1e-10)
+    if xc == S.XC_LSDA:  # LSDA: the adjoint and the FRMORB2 formula are one number
+        np.testing.assert_allclose(up.grad.numpy(), eng.fod_forces(1), atol=1e-10)
     h = 1e-4
     if xc == S.XC_PBE:
         from oracle import oracle as O
@@ -45,13 +44,5 @@ def test_autograd_gradient_matches_fd(xc):
         em = float(fod_energy(eng, *fods))
         fd = (ep - em) / (2 * h)
         g = (up.grad if s == 0 else dn.grad)[i, x].item()
-        if xc == S.XC_LSDA:
-            assert abs(fd - g) < 1e-5
-        else:
-            fp = [f.copy() for f in sy.fod]
-            fm = [f.copy() for f in sy.fod]
-            fp[s][i, x] += h
-            fm[s][i, x] -= h
-            fd_ref = (O.evaluate(sy, fods=fp, want_vab=False)["esic"] -
-                      O.evaluate(sy, fods=fm, want_vab=False)["esic"]) / (2 * h)
-            assert abs(fd - fd_ref) < 1e-6
+        print(f"xc={xc} coord {(s, i, x)}: fd {fd:.9f} adjoint {g:.9f}", flush=True)
+        assert abs(fd - g) < 1e-5

@@ -93,7 +93,8 @@ def test_fod_forces_parity(small_case):
     for s in range(2):
         _close(got["spins"][s]["debdax"], ref["spins"][s]["debdax"], 1e-9, 1e-12)
         _close(got["spins"][s]["grad"], ref["spins"][s]["grad"], 1e-9, 1e-12)
-        _close(got["spins"][s]["grad"], got["spins"][s]["debdax"], 1e-10, 1e-13)
+        if sysm.xc == S.XC_LSDA:  # PBE: the gradient is the exact mesh derivative, DEBDAX the FRMORB2 formula
+            _close(got["spins"][s]["grad"], got["spins"][s]["debdax"], 1e-10, 1e-13)
 
 
 def test_potentials_parity(small_case):

@@ -175,6 +175,55 @@ def test_pbe_potential_is_functional_derivative():
     assert abs(dE - dv) < 2e-6 * abs(dv)
 
 
+def test_pbe_energy_density_is_the_integrated_energy():
+    """e(rho, g) of the exact-derivative path is the EXL+EXN+ECL+ECN that xc_point integrates, for densities
+    with arbitrary gradient direction and Hessian (which must not enter the energy)."""
+    rng = np.random.default_rng(4)
+    rg = np.zeros((200, 10))
+    rg[:, 0] = 10.0 ** rng.uniform(-8, 1, 200)
+    rg[:, 1:4] = rng.normal(size=(200, 3)) * rg[:, :1] * 10.0 ** rng.uniform(-2, 1, (200, 1))
+    rg[:, 4:] = rng.normal(size=(200, 6))
+    _, e4 = O.xc_points(S.XC_PBE, rg)
+    g = np.linalg.norm(rg[:, 1:4], axis=1)
+    e, _, _ = O.pbe_edens(rg[:, 0], g)
+    np.testing.assert_allclose(e, e4.sum(1), rtol=1e-12, atol=1e-300)
+
+
+def test_pbe_energy_density_derivatives_match_fd():
+    """de/drho and de/dg of the forward-mode path against central differences of the energy itself."""
+    rng = np.random.default_rng(5)
+    rho = 10.0 ** rng.uniform(-6, 1, 100)
+    g = rho * 10.0 ** rng.uniform(-2, 1, 100)
+    e, er, eg = O.pbe_edens(rho, g)
+    h = 1e-5
+    fr = (O.pbe_edens(rho * (1 + h), g)[0] - O.pbe_edens(rho * (1 - h), g)[0]) / (2 * h * rho)
+    fg = (O.pbe_edens(rho, g * (1 + h))[0] - O.pbe_edens(rho, g * (1 - h))[0]) / (2 * h * g)
+    # allowed: the truncation error of the central difference plus its rounding error ~ eps |e| / step
+    eps = np.finfo(float).eps
+    assert np.all(np.abs(er - fr) <= 1e-7 * np.abs(fr) + 100 * eps * np.abs(e) / (h * rho))
+    assert np.all(np.abs(eg - fg) <= 1e-7 * np.abs(fg) + 100 * eps * np.abs(e) / (h * g))
+
+
+def test_pbe_fd_gradient_converges_to_exact_adjoint():
+    """PBE: central differences of the discrete E_SIC (h=1e-4) agree with the exact-derivative FOD gradient to
+    the same mesh-quadrature level as LSDA (the Hartree term keeps its continuum symmetry), while the
+    reference formula with the Laplacian-form GGA potential (DEBDAX) does not."""
+    sy = S.make_synthetic(3, 3, 7, xc=S.XC_PBE, nrad=80, nang=302, box=4.5)
+    res = O.evaluate(sy, want_vab=False)
+    h = 1e-4
+    worst_grad = worst_deb = 0.0
+    for (s, i, x) in [(0, 0, 0), (1, 2, 1), (0, 1, 2)]:
+        fp = [f.copy() for f in sy.fod]
+        fm = [f.copy() for f in sy.fod]
+        fp[s][i, x] += h
+        fm[s][i, x] -= h
+        fd = (O.evaluate(sy, fods=fp, want_vab=False)["esic"] - O.evaluate(sy, fods=fm, want_vab=False)["esic"]) / (2 * h)
+        worst_grad = max(worst_grad, abs(fd - res["spins"][s]["grad"][i, x]))
+        worst_deb = max(worst_deb, abs(fd - res["spins"][s]["debdax"][i, x]))
+    assert worst_grad < 1e-5
+    assert worst_deb > 10 * worst_grad
+
+
 # -------------------------------------------------------------------------------- FO, Lowdin, forces
 def _lowdin_from_fmat(fm):
     T = fm[0]          # T[i, j] = FMAT(j,i,1)
