@@ -137,6 +137,11 @@ module flosic_hip
       integer(c_int32_t), value :: n
       integer(c_int64_t) :: counts(*)
     end function
+    integer(c_int32_t) function flosic_set_coulomb_cols(ctx, ncols) bind(C, name="flosic_set_coulomb_cols")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ncols
+    end function
     integer(c_int32_t) function flosic_set_mesh_block(ctx, block_points) bind(C, name="flosic_set_mesh_block")
       import :: c_int32_t, c_ptr, c_int64_t
       type(c_ptr), value :: ctx

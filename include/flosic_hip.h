@@ -140,8 +140,17 @@ int32_t flosic_get_timings(flosic_ctx *ctx, int32_t n, double *ms);
  * rows (shell pairs that share both primitive exponents and centres are one group with one row block)
  * [2] nmsh [3] nbas [4] nflo (both spins) [5] exponent-pair groups [6] Hermite rows as the Coulomb kernel of
  * the current precision streams them (chunks zero-extended to its MFMA K step) [7] peak device bytes the
- * context has held since its creation.  n <= 8 values are written. */
+ * context has held since its creation [8] FLO columns per Coulomb block in use (0 for the VALU form).  n <= 9
+ * values are written. */
 int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
+
+/* Tile shape of the Coulomb kernels: FLO columns per block (each block contracts 64 mesh points with this many
+ * columns; a narrower block repeats the Boys/Hermite R generation for every column block but gives more blocks).
+ * 0 (default): the widest the precision allows; 64, 128, 192, 256: fixed (must divide the padded FLO count:
+ * 64, 128, 192 or a multiple of 256; the split-BF16 forms take at most 192); -1: autotuned at the next
+ * evaluation, every legal width timed on a strided sample of <= 32768 mesh points and the fastest kept (cached
+ * per process by precision, FLO count, Hermite rows and mesh size).  No reference counterpart. */
+int32_t flosic_set_coulomb_cols(flosic_ctx *ctx, int32_t ncols);
 
 /* Mesh-block recomputation.  block_points = 0 (default): the FOD-independent basis planes (values, and for PBE
  * gradients and Hessians: 10 planes of nbas x nmsh) stay resident between evaluations and the FLO fields of the

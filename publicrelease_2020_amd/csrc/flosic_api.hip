@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -148,6 +150,10 @@ struct flosic_ctx {
     bool planes_ok = false;
     int64_t block_pts = 0;
     uint64_t cfg_gen = 0;  // bumped by every setter that changes the launch sequence
+    // Coulomb column-block width: requested (0 widest, -1 autotune, else fixed) and in use
+    int coul_req = 0, coul_nfb = 0;
+    std::string tuned_key;
+    DevBuf<double> d_tx, d_ty, d_tz, d_tv;  // autotuner point sample and its potentials
     bool timed = true;     // per-phase events (off while a batch graph is captured or replayed)
     // spins
     Spin sp[2];
@@ -159,7 +165,7 @@ struct flosic_ctx {
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
     double timings[10] = {};
-    int64_t counts[8] = {};
+    int64_t counts[9] = {};
 };
 
 // Many independent systems evaluated together: the Coulomb contractions of all of them run as one launch per
@@ -173,16 +179,14 @@ struct flosic_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_coul = nullptr, ev_end = nullptr, ev_t0 = nullptr, ev_c0 = nullptr,
                ev_c1 = nullptr;
-    // launch plan: job descriptors and item offsets of every class, uploaded once per plan
+    // launch plan: per class of equal (form, pieces, column block width) the job tables of its launches
     struct Class {
         int form = 0, npieces = 0, nfb = 0;
         std::vector<int> members;
-        size_t job_off = 0, item_off = 0;  // byte offsets in d_plan
+        std::vector<CoulJobTable<double>> tables;  // the job layout does not depend on the operand type
         int64_t nitems = 0;
     };
     std::vector<Class> classes;
-    std::vector<unsigned char> h_plan;
-    DevBuf<unsigned char> d_plan;
     std::vector<uint64_t> plan_sig;  // context signatures the plan was built for
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -953,7 +957,108 @@ CoulForm coulomb_form(const flosic_ctx *c) {
     return c->prec == FLOSIC_PREC_MIXED ? COUL_FP32 : COUL_FP64;
 }
 int coulomb_nfb(const flosic_ctx *c) {
+    if (c->coul_nfb > 0) return c->coul_nfb;
     return coulomb_form(c) == COUL_BF16 ? coulomb_cols_split(c->nF) : coulomb_cols(c->nF);
+}
+
+// One single-system Coulomb launch of the context's form over M points (x, y, z) into V (ldv), nfb columns
+void coulomb_launch(flosic_ctx *c, const double *x, const double *y, const double *z, int64_t M, double *V,
+                    int64_t ldv, int nfb) {
+    const int nF = c->nF;
+    switch (coulomb_form(c)) {
+        case COUL_VALU:
+            launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, x, y, z, M, c->d_btab.p, nF, V,
+                           ldv, c->stream);
+            break;
+        case COUL_BF16:
+            launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks_bf16.p, c->nchunks_bf16,
+                                c->rows_padded_bf16 / 32, c->d_pairP.p, c->d_Hs.p, c->ldh, x, y, z, M, c->d_btab.p, nF,
+                                V, ldv, c->stream, nfb);
+            break;
+        case COUL_FP32:
+            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
+                                       x, y, z, M, c->d_btab.p, nF, V, ldv, c->stream, nfb);
+            break;
+        case COUL_FP64:
+            launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H64.p,
+                                        c->ldh, x, y, z, M, c->d_btab.p, nF, V, ldv, c->stream, nfb);
+            break;
+    }
+}
+
+// The autotuner: every legal column-block width of the context's Coulomb form is timed on a strided sample of at
+// most 32768 mesh points (every point of a small mesh) and the fastest kept.  Results are cached per process by
+// form, FLO count, Hermite rows and mesh size, so equal shapes are tuned once.
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_cache;
+
+int autotune_coulomb(flosic_ctx *c) {
+    const CoulForm form = coulomb_form(c);
+    char key[160];
+    snprintf(key, sizeof key, "f%d p%d n%d r%lld m%lld", (int)form, c->prec, c->nF,
+             (long long)(form == COUL_BF16 ? c->rows_padded_bf16 : c->rows_padded), (long long)c->M);
+    if (c->tuned_key == key && c->coul_nfb > 0) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            c->coul_nfb = it->second;
+            c->tuned_key = key;
+            return 0;
+        }
+    }
+    c->coul_nfb = 0;
+    if (form == COUL_VALU) {
+        c->tuned_key = key;
+        return 0;
+    }
+    const int64_t S = std::min<int64_t>(c->M, 32768);
+    const int64_t stride = c->M / S;
+    ALLOC(c, c->d_tx, (size_t)S);
+    ALLOC(c, c->d_ty, (size_t)S);
+    ALLOC(c, c->d_tz, (size_t)S);
+    ALLOC(c, c->d_tv, (size_t)c->nF * S);
+    launch_gather(c->d_x.p, stride, (int)S, c->d_tx.p, c->stream);
+    launch_gather(c->d_y.p, stride, (int)S, c->d_ty.p, c->stream);
+    launch_gather(c->d_z.p, stride, (int)S, c->d_tz.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    int best = 0;
+    float best_ms = 0.f;
+    for (int nfb : {256, 192, 128, 64}) {
+        if (!coulomb_cols_legal(nfb, c->ldh, form == COUL_BF16)) continue;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the second launch is timed
+            HIPCHK(c, hipEventRecord(c->ev[EV_COUL0], c->stream));
+            coulomb_launch(c, c->d_tx.p, c->d_ty.p, c->d_tz.p, S, c->d_tv.p, S, nfb);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(c->ev[EV_COUL1], c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev[EV_COUL1]));
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[EV_COUL0], c->ev[EV_COUL1]));
+        }
+        if (best == 0 || ms < best_ms) {
+            best = nfb;
+            best_ms = ms;
+        }
+    }
+    c->coul_nfb = best;
+    c->tuned_key = key;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+    return 0;
+}
+
+// Column-block width of this evaluation: fixed, autotuned, or the widest the form allows
+int resolve_coulomb_cols(flosic_ctx *c) {
+    if (c->coul_req == -1) return autotune_coulomb(c);
+    c->coul_nfb = 0;
+    if (c->coul_req > 0) {
+        if (coulomb_form(c) == COUL_VALU) return 0;
+        if (!coulomb_cols_legal(c->coul_req, c->ldh, coulomb_form(c) == COUL_BF16))
+            return fail(c, FLOSIC_EINVAL, "Coulomb column block %d does not divide ldh %d for this precision",
+                        c->coul_req, c->ldh);
+        c->coul_nfb = c->coul_req;
+    }
+    return 0;
 }
 
 // Phase A1: Fermi orbitals, Lowdin, FLO coefficients (FP64; n x n algebra on the host), resident planes
@@ -1021,34 +1126,13 @@ int eval_hermite(flosic_ctx *c) {
     }
     HIPCHK(c, hipGetLastError());
     ALLOC(c, c->d_VH, (size_t)nF * c->M);
-    return 0;
+    return resolve_coulomb_cols(c);
 }
 
 // Phase B (one system): Coulomb potentials of all FLOs, V_H[f][k]
 int eval_coulomb(flosic_ctx *c) {
-    const int nF = c->nF;
-    const int64_t M = c->M;
     MARK(c, EV_COUL0);
-    switch (coulomb_form(c)) {
-        case COUL_VALU:
-            launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M,
-                           c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-            break;
-        case COUL_BF16:
-            launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks_bf16.p, c->nchunks_bf16,
-                                c->rows_padded_bf16 / 32, c->d_pairP.p, c->d_Hs.p, c->ldh, c->d_x.p, c->d_y.p,
-                                c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-            break;
-        case COUL_FP32:
-            launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
-                                       c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M, c->stream);
-            break;
-        case COUL_FP64:
-            launch_coulomb_mfma<double>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H64.p,
-                                        c->ldh, c->d_x.p, c->d_y.p, c->d_z.p, M, c->d_btab.p, nF, c->d_VH.p, M,
-                                        c->stream);
-            break;
-    }
+    coulomb_launch(c, c->d_x.p, c->d_y.p, c->d_z.p, c->M, c->d_VH.p, c->M, coulomb_nfb(c));
     HIPCHK(c, hipGetLastError());
     MARK(c, EV_COUL1);
     return 0;
@@ -1256,6 +1340,7 @@ int eval_finish(flosic_ctx *c) {
     c->counts[5] = c->ngroup;
     c->counts[6] = coulomb_form(c) == COUL_BF16 ? c->rows_padded_bf16 : c->rows_padded;
     c->counts[7] = (int64_t)c->mem.peak;
+    c->counts[8] = coulomb_form(c) == COUL_VALU ? 0 : coulomb_nfb(c);
     c->evaluated = true;
     return 0;
 }
@@ -1325,10 +1410,12 @@ CoulJob<double> coulomb_job(const flosic_ctx *c) {
 }
 
 // Classes of equal (form, pieces, column block width); members heaviest first (work per item ~ K steps), so
-// that the longest items are dispatched first; job descriptors and item offsets uploaded in one buffer
+// that the longest items are dispatched first; kBatchJobs members per launch
 int build_plan(flosic_batch *b) {
-    static_assert(sizeof(CoulJob<double>) == sizeof(CoulJob<float>) && sizeof(CoulJob<double>) == sizeof(CoulJob<__bf16>),
+    static_assert(sizeof(CoulJobTable<double>) == sizeof(CoulJobTable<float>) &&
+                      sizeof(CoulJobTable<double>) == sizeof(CoulJobTable<__bf16>),
                   "job layout is independent of the operand type");
+    static_assert(sizeof(CoulJobTable<double>) <= 4096, "job table must fit the kernel argument space");
     b->classes.clear();
     for (int i = 0; i < (int)b->ctxs.size(); ++i) {
         const flosic_ctx *c = b->ctxs[i];
@@ -1348,34 +1435,26 @@ int build_plan(flosic_batch *b) {
         }
         it->members.push_back(i);
     }
-    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
-    b->h_plan.clear();
     for (auto &k : b->classes) {
         std::vector<CoulJob<double>> jobs;
         for (int i : k.members) jobs.push_back(coulomb_job(b->ctxs[i]));
-        std::vector<int> order(jobs.size());
-        for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].nsteps > jobs[y].nsteps; });
-        std::vector<CoulJob<double>> sorted;
-        std::vector<int64_t> off;
-        int64_t items = 0;
-        for (int j : order) {
-            sorted.push_back(jobs[j]);
-            off.push_back(items);
-            items += (jobs[j].M + 63) / 64 * jobs[j].ncb;
+        std::stable_sort(jobs.begin(), jobs.end(),
+                         [](const CoulJob<double> &x, const CoulJob<double> &y) { return x.nsteps > y.nsteps; });
+        k.nitems = 0;
+        for (size_t j0 = 0; j0 < jobs.size(); j0 += kBatchJobs) {
+            CoulJobTable<double> t{};
+            int64_t items = 0;
+            for (size_t j = j0; j < std::min(jobs.size(), j0 + kBatchJobs); ++j) {
+                t.job[t.njobs] = jobs[j];
+                t.off[t.njobs++] = items;
+                items += (jobs[j].M + 63) / 64 * jobs[j].ncb;
+            }
+            t.off[t.njobs] = items;
+            if (items >= (int64_t)INT32_MAX)
+                return bfail(b, FLOSIC_EINVAL, "batch too large: %lld Coulomb items in one launch", (long long)items);
+            k.tables.push_back(t);
+            k.nitems += items;
         }
-        if (items >= (int64_t)INT32_MAX) return bfail(b, FLOSIC_EINVAL, "batch too large: %lld Coulomb items", (long long)items);
-        k.nitems = items;
-        k.job_off = align(b->h_plan.size());
-        b->h_plan.resize(k.job_off + sorted.size() * sizeof(CoulJob<double>));
-        std::memcpy(b->h_plan.data() + k.job_off, sorted.data(), sorted.size() * sizeof(CoulJob<double>));
-        k.item_off = align(b->h_plan.size());
-        b->h_plan.resize(k.item_off + off.size() * sizeof(int64_t));
-        std::memcpy(b->h_plan.data() + k.item_off, off.data(), off.size() * sizeof(int64_t));
-    }
-    if (!b->h_plan.empty()) {
-        if (!b->d_plan.ensure(b->h_plan.size(), &b->mem)) return bfail(b, FLOSIC_ENOMEM, "batch plan allocation");
-        BHIPCHK(b, hipMemcpy(b->d_plan.p, b->h_plan.data(), b->h_plan.size(), hipMemcpyHostToDevice));
     }
     b->plan_sig = batch_sig(b);
     return 0;
@@ -1383,20 +1462,17 @@ int build_plan(flosic_batch *b) {
 
 int launch_classes(flosic_batch *b) {
     for (const auto &k : b->classes) {
-        const auto *jobs = b->d_plan.p + k.job_off;
-        const auto *off = reinterpret_cast<const int64_t *>(b->d_plan.p + k.item_off);
-        const int nj = (int)k.members.size();
         const double *btab = b->ctxs[k.members[0]]->d_btab.p;
-        if (k.form == COUL_FP64)
-            launch_coulomb_batch<double>(reinterpret_cast<const CoulJob<double> *>(jobs), off, nj, k.nitems, k.nfb,
-                                         btab, b->stream);
-        else if (k.form == COUL_FP32)
-            launch_coulomb_batch<float>(reinterpret_cast<const CoulJob<float> *>(jobs), off, nj, k.nitems, k.nfb, btab,
-                                        b->stream);
-        else
-            launch_coulomb_bf16_batch(k.npieces, reinterpret_cast<const CoulJob<__bf16> *>(jobs), off, nj, k.nitems,
-                                      k.nfb, btab, b->stream);
-        BHIPCHK(b, hipGetLastError());
+        for (const auto &t : k.tables) {
+            if (k.form == COUL_FP64)
+                launch_coulomb_batch<double>(t, k.nfb, btab, b->stream);
+            else if (k.form == COUL_FP32)
+                launch_coulomb_batch<float>(reinterpret_cast<const CoulJobTable<float> &>(t), k.nfb, btab, b->stream);
+            else
+                launch_coulomb_bf16_batch(k.npieces, reinterpret_cast<const CoulJobTable<__bf16> &>(t), k.nfb, btab,
+                                          b->stream);
+            BHIPCHK(b, hipGetLastError());
+        }
     }
     return 0;
 }
@@ -1680,6 +1756,18 @@ int32_t flosic_set_mesh_block(flosic_ctx *c, int64_t block_points) {
     return FLOSIC_OK;
 }
 
+int32_t flosic_set_coulomb_cols(flosic_ctx *c, int32_t ncols) {
+    if (!c) return FLOSIC_EINVAL;
+    if (ncols != -1 && ncols != 0 && ncols != 64 && ncols != 128 && ncols != 192 && ncols != 256)
+        return fail(c, FLOSIC_EINVAL, "Coulomb column block must be -1, 0, 64, 128, 192 or 256 (got %d)", ncols);
+    c->coul_req = ncols;
+    c->coul_nfb = 0;
+    c->tuned_key.clear();
+    c->evaluated = false;
+    ++c->cfg_gen;
+    return FLOSIC_OK;
+}
+
 int32_t flosic_set_orbitals(flosic_ctx *c, int32_t ispn, int32_t nbas, int32_t nwf, int32_t nocc, const double *coef) {
     if (!c) return FLOSIC_EINVAL;
     ++c->cfg_gen;
@@ -1948,7 +2036,7 @@ int32_t flosic_get_timings(flosic_ctx *c, int32_t n, double *ms) {
 
 int32_t flosic_get_counts(flosic_ctx *c, int32_t n, int64_t *counts) {
     if (!c || !counts || n < 0) return FLOSIC_EINVAL;
-    for (int i = 0; i < n && i < 8; ++i) counts[i] = c->counts[i];
+    for (int i = 0; i < n && i < 9; ++i) counts[i] = c->counts[i];
     return FLOSIC_OK;
 }
 

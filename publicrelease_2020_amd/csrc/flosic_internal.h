@@ -68,6 +68,9 @@ struct CoulJob {
     int ncb;
 };
 
+// Column-block widths the Coulomb kernels are built for: nfb must divide ldh; the split-BF16 form takes at most 192
+bool coulomb_cols_legal(int nfb, int ldh, bool split);
+
 // FLO columns handled by one Coulomb block: all of them up to 256; the H leading dimension ldh is nF rounded
 // up to a multiple of this
 int coulomb_cols(int nF);
@@ -106,16 +109,22 @@ void launch_frag_h(const ChunkDev *chunks, int nchunks, const double *H, int ldh
 template <typename T>
 void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP, const T *Hf,
                          int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
-                         int nF, double *V, int64_t ldv, hipStream_t s);
+                         int nF, double *V, int64_t ldv, hipStream_t s, int nfb = 0);
 
-// Batched form: one launch over the (system, point tile, column block) items of jobs of one Coulomb form and
-// one column block width nfb (= coulomb_cols of every job's nF; split-BF16: coulomb_cols_split).  item_off[j]
-// is the first item of job j (a job has ceil(M/64) * ncb items, tile-major); nitems < 2^31.
+// Batched form: one launch over the (system, point tile, column block) items of up to kBatchJobs jobs of one
+// Coulomb form and one column block width nfb.  off[j] is the first item of job j (a job has ceil(M/64) * ncb
+// items, tile-major), off[njobs] the total (< 2^31).  The table is passed by value as the kernel argument.
+constexpr int kBatchJobs = 24;
 template <typename T>
-void launch_coulomb_batch(const CoulJob<T> *jobs, const int64_t *item_off, int njobs, int64_t nitems, int nfb,
-                          const double *btab, hipStream_t s);
-void launch_coulomb_bf16_batch(int npieces, const CoulJob<__bf16> *jobs, const int64_t *item_off, int njobs,
-                               int64_t nitems, int nfb, const double *btab, hipStream_t s);
+struct CoulJobTable {
+    CoulJob<T> job[kBatchJobs];
+    int64_t off[kBatchJobs + 1];
+    int njobs;
+};
+template <typename T>
+void launch_coulomb_batch(const CoulJobTable<T> &jt, int nfb, const double *btab, hipStream_t s);
+void launch_coulomb_bf16_batch(int npieces, const CoulJobTable<__bf16> &jt, int nfb, const double *btab,
+                               hipStream_t s);
 int coulomb_cols_split(int nF);
 
 // split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,
@@ -125,7 +134,7 @@ void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const doub
                     int64_t nsteps, __bf16 *Hs, hipStream_t s);
 void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
                          const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
-                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s);
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb = 0);
 
 // XC + total potential + energy partials over M points: phi nd planes of [nF][M] (plane stride nF*M), Vtot and U
 // [nF][M], VH[f * ldvh + k], w[k]; 256-point block b writes partial[(f * nblk + blk0 + b) * 9 + q].  For PBE,

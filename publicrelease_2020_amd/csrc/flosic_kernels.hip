@@ -952,47 +952,43 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(CoulJob<__bf16
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Batched Coulomb: one launch over the (system, point tile, column block) items of many systems.  Block b
-// runs item b with the single-system tile code; the host orders the items (largest systems first) so that the
-// dispatcher, which hands blocks to CUs in index order as CUs free up, balances the systems like a work queue.
-// One block fills a CU (16 waves at 128 VGPRs), so the grid is the queue and no block loops.
+// Batched Coulomb: one launch over the (system, point tile, column block) items of up to kBatchJobs systems.
+// Block b runs item b with the single-system tile code; the host orders the systems (largest first) so that the
+// dispatcher, which hands blocks to CUs in index order as CUs free up, balances them like a work queue.  One
+// block fills a CU (16 waves at 128 VGPRs), so the grid is the queue and no block loops.  The job table travels
+// in the kernel arguments: loaded from there, every job field stays a scalar (uniform) value in the tile code,
+// exactly as in the single-system kernel (loaded from global memory, the compiler treats the tile's loop
+// bounds as divergent and the kernel runs 20% slower).
 // ---------------------------------------------------------------------------------------------------
-// item -> job with item_off[j] the first item of job j (item_off[0] = 0, increasing)
-__device__ __forceinline__ int find_job(const int64_t *__restrict__ item_off, int njobs, int64_t it) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (item_off[mid] <= it) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <typename T, int NFB>
-__global__ void __launch_bounds__(kCoulThreads)
-    k_coulomb_ws_batch(const CoulJob<T> *__restrict__ jobs, const int64_t *__restrict__ item_off, int njobs,
-                       const double *__restrict__ tab) {
-    const int64_t it = blockIdx.x;
-    const int j = find_job(item_off, njobs, it);
-    const CoulJob<T> J = jobs[j];
-    const int64_t local = it - item_off[j];
-    coulomb_tile_fp<T, NFB>(J, tab, local / J.ncb, (int)(local % J.ncb));
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_batch(const CoulJobTable<T> jt,
+                                                                   const double *__restrict__ tab) {
+    const unsigned it = blockIdx.x;
+    int j = 0;
+    while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
+    const unsigned local = it - (unsigned)jt.off[j];  // items of one launch: < 2^31
+    coulomb_tile_fp<T, NFB>(jt.job[j], tab, local / (unsigned)jt.job[j].ncb, (int)(local % (unsigned)jt.job[j].ncb));
 }
 
 template <int NP, int NFB>
-__global__ void __launch_bounds__(kCoulThreads)
-    k_coulomb_ws_bf16_batch(const CoulJob<__bf16> *__restrict__ jobs, const int64_t *__restrict__ item_off,
-                            int njobs, const double *__restrict__ tab) {
-    const int64_t it = blockIdx.x;
-    const int j = find_job(item_off, njobs, it);
-    const CoulJob<__bf16> J = jobs[j];
-    const int64_t local = it - item_off[j];
-    coulomb_tile_bf16<NP, NFB>(J, tab, local / J.ncb, (int)(local % J.ncb));
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16_batch(const CoulJobTable<__bf16> jt,
+                                                                        const double *__restrict__ tab) {
+    const unsigned it = blockIdx.x;
+    int j = 0;
+    while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
+    const unsigned local = it - (unsigned)jt.off[j];
+    coulomb_tile_bf16<NP, NFB>(jt.job[j], tab, local / (unsigned)jt.job[j].ncb,
+                               (int)(local % (unsigned)jt.job[j].ncb));
 }
 
 // FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
 // split-BF16 kernel takes at most 192 (its register budget), 128 beyond that
 int coulomb_cols(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 256; }
+bool coulomb_cols_legal(int nfb, int ldh, bool split) {
+    if (nfb != 64 && nfb != 128 && nfb != 192 && nfb != 256) return false;
+    if (split && nfb > 192) return false;
+    return ldh % nfb == 0;
+}
 int coulomb_cols_split(int nF) { return nF <= 64 ? 64 : nF <= 128 ? 128 : nF <= 192 ? 192 : 128; }
 
 template <typename T>
@@ -1006,9 +1002,9 @@ template void launch_frag_h<float>(const ChunkDev *, int, const double *, int, i
 template <typename T>
 void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP, const T *Hf,
                          int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
-                         int nF, double *V, int64_t ldv, hipStream_t s) {
+                         int nF, double *V, int64_t ldv, hipStream_t s, int nfb) {
     if (nchunks <= 0 || M <= 0) return;
-    const int nfb = coulomb_cols(nF);
+    if (nfb <= 0) nfb = coulomb_cols(nF);
     const CoulJob<T> J{chunks, nchunks, pairP, Hf, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb};
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
     switch (nfb) {
@@ -1020,10 +1016,10 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, co
 }
 template void launch_coulomb_mfma<double>(const ChunkDev *, int, int64_t, const double4 *, const double *, int,
                                           const double *, const double *, const double *, int64_t, const double *, int,
-                                          double *, int64_t, hipStream_t);
+                                          double *, int64_t, hipStream_t, int);
 template void launch_coulomb_mfma<float>(const ChunkDev *, int, int64_t, const double4 *, const float *, int,
                                          const double *, const double *, const double *, int64_t, const double *, int,
-                                         double *, int64_t, hipStream_t);
+                                         double *, int64_t, hipStream_t, int);
 
 void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const double *H, int ldh, int nF,
                     int64_t nsteps, __bf16 *Hs, hipStream_t s) {
@@ -1036,9 +1032,9 @@ void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const doub
 
 void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
                          const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
-                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s) {
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb) {
     if (nchunks <= 0 || M <= 0) return;
-    const int nfb = coulomb_cols_split(nF);
+    if (nfb <= 0) nfb = coulomb_cols_split(nF);
     const CoulJob<__bf16> J{chunks, nchunks, pairP, Hs, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb};
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
 #define FLOSIC_WSB(NP, NFB) hipLaunchKernelGGL((k_coulomb_ws_bf16<NP, NFB>), grid, blk, 0, s, J, btab)
@@ -1058,28 +1054,24 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
 }
 
 template <typename T>
-void launch_coulomb_batch(const CoulJob<T> *jobs, const int64_t *item_off, int njobs, int64_t nitems, int nfb,
-                          const double *btab, hipStream_t s) {
-    if (njobs <= 0 || nitems <= 0) return;
-    const dim3 grid((unsigned)nitems), blk(kCoulThreads);
+void launch_coulomb_batch(const CoulJobTable<T> &jt, int nfb, const double *btab, hipStream_t s) {
+    if (jt.njobs <= 0 || jt.off[jt.njobs] <= 0) return;
+    const dim3 grid((unsigned)jt.off[jt.njobs]), blk(kCoulThreads);
     switch (nfb) {
-        case 64: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 64>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
-        case 128: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 128>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
-        case 192: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 192>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
-        default: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 256>), grid, blk, 0, s, jobs, item_off, njobs, btab); break;
+        case 64: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 64>), grid, blk, 0, s, jt, btab); break;
+        case 128: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 128>), grid, blk, 0, s, jt, btab); break;
+        case 192: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 192>), grid, blk, 0, s, jt, btab); break;
+        default: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 256>), grid, blk, 0, s, jt, btab); break;
     }
 }
-template void launch_coulomb_batch<double>(const CoulJob<double> *, const int64_t *, int, int64_t, int,
-                                           const double *, hipStream_t);
-template void launch_coulomb_batch<float>(const CoulJob<float> *, const int64_t *, int, int64_t, int, const double *,
-                                          hipStream_t);
+template void launch_coulomb_batch<double>(const CoulJobTable<double> &, int, const double *, hipStream_t);
+template void launch_coulomb_batch<float>(const CoulJobTable<float> &, int, const double *, hipStream_t);
 
-void launch_coulomb_bf16_batch(int npieces, const CoulJob<__bf16> *jobs, const int64_t *item_off, int njobs,
-                               int64_t nitems, int nfb, const double *btab, hipStream_t s) {
-    if (njobs <= 0 || nitems <= 0) return;
-    const dim3 grid((unsigned)nitems), blk(kCoulThreads);
-#define FLOSIC_WSBB(NP, NFB)                                                                                      \
-    hipLaunchKernelGGL((k_coulomb_ws_bf16_batch<NP, NFB>), grid, blk, 0, s, jobs, item_off, njobs, btab)
+void launch_coulomb_bf16_batch(int npieces, const CoulJobTable<__bf16> &jt, int nfb, const double *btab,
+                               hipStream_t s) {
+    if (jt.njobs <= 0 || jt.off[jt.njobs] <= 0) return;
+    const dim3 grid((unsigned)jt.off[jt.njobs]), blk(kCoulThreads);
+#define FLOSIC_WSBB(NP, NFB) hipLaunchKernelGGL((k_coulomb_ws_bf16_batch<NP, NFB>), grid, blk, 0, s, jt, btab)
 #define FLOSIC_WSBB_NFB(NP)                        \
     switch (nfb) {                                 \
         case 64: FLOSIC_WSBB(NP, 64); break;       \

@@ -69,6 +69,7 @@ def library():
             "flosic_get_timings": [vp, C.c_int32, _dp],
             "flosic_get_counts": [vp, C.c_int32, _lp],
             "flosic_set_mesh_block": [vp, C.c_int64],
+            "flosic_set_coulomb_cols": [vp, C.c_int32],
             "flosic_batch_create": [C.c_int32, C.POINTER(vp), C.c_int32, C.POINTER(vp)],
             "flosic_batch_evaluate": [vp, _dp],
             "flosic_batch_get_timings": [vp, C.c_int32, _dp],
@@ -92,7 +93,7 @@ def exported_symbols() -> list:
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
             "flosic_evaluate", "flosic_apotnl_sic", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
-            "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block",
+            "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
             "flosic_batch_get_counts", "flosic_batch_destroy"]
 
@@ -251,9 +252,16 @@ class Flosic:
         return t
 
     def counts(self) -> np.ndarray:
-        c = np.zeros(8, np.int64)
-        self._chk(self._L.flosic_get_counts(self._h, 8, c.ctypes.data_as(_lp)))
+        c = np.zeros(9, np.int64)
+        self._chk(self._L.flosic_get_counts(self._h, 9, c.ctypes.data_as(_lp)))
         return c
+
+    def set_coulomb_cols(self, ncols: int):
+        """FLO columns per Coulomb block: 0 widest, -1 autotune, or 64/128/192/256."""
+        self._chk(self._L.flosic_set_coulomb_cols(self._h, int(ncols)))
+
+    def coulomb_cols(self) -> int:
+        return int(self.counts()[8])
 
     def peak_device_bytes(self) -> int:
         return int(self.counts()[7])

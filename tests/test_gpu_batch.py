@@ -153,3 +153,31 @@ def test_batch_graph_replays_match_eager():
     assert modes == [0, 1, 2, 2, 2]
     c = batch.counts()
     assert c[1] == 1 and c[2] == 3
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "mixed", "bf16x3"])
+def test_coulomb_column_blocks_and_autotuner(prec):
+    """Every legal Coulomb tile width gives the same potentials (the columns are independent); the autotuner
+    picks one of them and reports it."""
+    F = _F()
+    sysm = S.make_synthetic(8, 40, 21, S.XC_LSDA, nrad=24, nang=50, box=5.0)  # 80 FLOs: widths 64 and 128
+    e = F.Flosic(0)
+    e.load_system(sysm, prec)
+    z0 = e.evaluate()
+    assert e.coulomb_cols() == 128
+    v0 = e.potential(1, 3)[1]
+    r0 = _all(e)
+    e.set_coulomb_cols(64)
+    z1 = e.evaluate()
+    assert e.coulomb_cols() == 64
+    _close(e.potential(1, 3)[1], v0, 1e-13)
+    _close([z1], [z0], 1e-13)
+    _compare(_all(e), r0, 1e-12)
+    e.set_coulomb_cols(-1)
+    z2 = e.evaluate()
+    print(f"precision {prec}: autotuned Coulomb columns {e.coulomb_cols()}", flush=True)
+    assert e.coulomb_cols() in (64, 128)
+    _close([z2], [z0], 1e-13)
+    with pytest.raises(F.FlosicError):
+        e.set_coulomb_cols(192)  # does not divide the padded FLO count (128)
+        e.evaluate()
