@@ -1623,7 +1623,7 @@ int32_t flosic_create(int32_t device, flosic_ctx **out) {
         delete c;
         return FLOSIC_EDEVICE;
     }
-    std::vector<double> tab((size_t)kBoysRows * kBoysCols);
+    std::vector<double> tab((size_t)kBoysTableDoubles);
     boys_table(tab.data());
     if (upload(c, c->d_btab, tab) || hipStreamSynchronize(c->stream) != hipSuccess) {
         delete c;

@@ -76,7 +76,9 @@ bool coulomb_cols_legal(int nfb, int ldh, bool split);
 int coulomb_cols(int nF);
 
 // per-phase launch helpers (stream-ordered)
-void boys_table(double *h_tab);  // host: fills kBoysRows x kBoysCols
+// host: fills kBoysRows x kBoysCols doubles followed by the same table in FP32 (kBoysTableDoubles in all)
+constexpr int kBoysTableDoubles = kBoysRows * kBoysCols * 3 / 2;
+void boys_table(double *h_tab);
 
 // nd = 1 (values), 4 (+ gradient), 10 (+ Hessian xx,yy,zz,xy,xz,yz) planes, or -1 (one Laplacian plane)
 template <typename T>

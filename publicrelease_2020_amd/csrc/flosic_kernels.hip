@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "flosic_internal.h"
 
@@ -292,77 +293,84 @@ void launch_hermite(int ngroup, const PairDev *pairs, const double *dps, int nF,
 // Boys functions F_0..F_TOP (fffmt3.F:66-88): T > 16 one-term asymptote + upward recursion; otherwise
 // cubic Taylor of F_TOP about the nearest 0.01 node and downward recursion
 // ---------------------------------------------------------------------------------------------------
-template <int TOP>
-__device__ __forceinline__ void boys(double T, double eT, const double *__restrict__ tab, double *F) {
-    if (T > 16.0) {
-        const double rect1 = 0.5 / T;
-        F[0] = rect1 * (sqrt(T) * 1.77245385090551602728 - eT);
+// Boys F_0..F_TOP of the reference scheme (fffmt3.F): T > 16 one-term asymptote and upward recursion, else the
+// cubic Taylor expansion of F_TOP about the nearest table node and downward recursion.  C = double for the FP64
+// forms; C = float (table rounded to FP32) where R feeds FP32 or split-BF16 MFMA operands anyway.
+template <int TOP, typename C>
+__device__ __forceinline__ void boys(C T, C eT, const C *__restrict__ tab, C *F) {
+    if (T > C(16.0)) {
+        const C rect1 = C(0.5) / T;
+        F[0] = rect1 * (sqrt(T) * C(1.77245385090551602728) - eT);
 #pragma unroll
         for (int m = 1; m <= TOP; ++m) F[m] = rect1 * ((2 * m - 1) * F[m - 1] - eT);
     } else {
-        const int it = (int)(100.0 * T + 1.5);
-        const double dt = 1.0e-2 * (it - 1) - T;
-        const double *f = tab + (it - 1) * kBoysCols;
-        F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] / 2.0 + dt * (f[TOP + 3] / 6.0)));
-        const double rect1 = 2.0 * T;
+        const int it = (int)(C(100.0) * T + C(1.5));
+        const C dt = C(1.0e-2) * (it - 1) - T;
+        const C *f = tab + (it - 1) * kBoysCols;
+        F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] / C(2.0) + dt * (f[TOP + 3] / C(6.0))));
+        const C rect1 = C(2.0) * T;
 #pragma unroll
         for (int m = TOP - 1; m >= 0; --m) {
-            const double r = F[m + 1] * rect1 + eT;
-            F[m] = (m == 0) ? r : r * (1.0 / (2 * m + 1));
+            const C r = F[m + 1] * rect1 + eT;
+            F[m] = (m == 0) ? r : r * (C(1.0) / (2 * m + 1));
         }
     }
 }
 
 // one Hermite step R^{(n)}_I from R^{(n+1)} in place (entries of lower order still hold level n+1)
-template <int I>
-__device__ __forceinline__ void rstep(double *R, double X, double Y, double Z) {
+template <int I, typename C>
+__device__ __forceinline__ void rstep(C *R, C X, C Y, C Z) {
     constexpr int t = tuv_comp(I, 0), u = tuv_comp(I, 1), v = tuv_comp(I, 2);
     if constexpr (t > 0) {
-        double r = X * R[tuv_idx(t - 1, u, v)];
+        C r = X * R[tuv_idx(t - 1, u, v)];
         if constexpr (t > 1) r += (t - 1) * R[tuv_idx(t - 2, u, v)];
         R[I] = r;
     } else if constexpr (u > 0) {
-        double r = Y * R[tuv_idx(t, u - 1, v)];
+        C r = Y * R[tuv_idx(t, u - 1, v)];
         if constexpr (u > 1) r += (u - 1) * R[tuv_idx(t, u - 2, v)];
         R[I] = r;
     } else {
-        double r = Z * R[tuv_idx(t, u, v - 1)];
+        C r = Z * R[tuv_idx(t, u, v - 1)];
         if constexpr (v > 1) r += (v - 1) * R[tuv_idx(t, u, v - 2)];
         R[I] = r;
     }
 }
-template <int I>
-__device__ __forceinline__ void rdown(double *R, double X, double Y, double Z) {
+template <int I, typename C>
+__device__ __forceinline__ void rdown(C *R, C X, C Y, C Z) {
     if constexpr (I >= 1) {
-        rstep<I>(R, X, Y, Z);
-        rdown<I - 1>(R, X, Y, Z);
+        rstep<I, C>(R, X, Y, Z);
+        rdown<I - 1, C>(R, X, Y, Z);
     }
 }
-template <int L, int N>
-__device__ __forceinline__ void rlevels(double *R, double X, double Y, double Z, const double *F, const double *pw) {
+template <int L, int N, typename C>
+__device__ __forceinline__ void rlevels(C *R, C X, C Y, C Z, const C *F, const C *pw) {
     if constexpr (N >= 0) {
-        rdown<ntuv(L - N) - 1>(R, X, Y, Z);
+        rdown<ntuv(L - N) - 1, C>(R, X, Y, Z);
         R[0] = pw[N] * F[N];
-        rlevels<L, N - 1>(R, X, Y, Z, F, pw);
+        rlevels<L, N - 1, C>(R, X, Y, Z, F, pw);
     }
 }
 
 // R_tuv(p, P - C) for all t+u+v <= L, TUV order
-template <int L>
-__device__ __forceinline__ void hermite_R(double p, double X, double Y, double Z, const double *__restrict__ tab,
-                                          double *R) {
+template <int L, typename C = double>
+__device__ __forceinline__ void hermite_R(C p, C X, C Y, C Z, const C *__restrict__ tab, C *R) {
     constexpr int TOP = L < 2 ? 2 : L;
-    const double T = p * (X * X + Y * Y + Z * Z);
-    double F[TOP + 1];
-    // beyond T = 60, e^-T (< 9e-27) is below half an ulp of every term it enters (F_4(60) ~ 6e-8), so
-    // dropping it leaves each F_m bitwise unchanged and saves the exponential for most (point, group) pairs
-    boys<TOP>(T, T > 60.0 ? 0.0 : exp(-T), tab, F);
-    double pw[L + 1];
-    pw[0] = 1.0;
+    const C T = p * (X * X + Y * Y + Z * Z);
+    C F[TOP + 1];
+    C eT;
+    if constexpr (sizeof(C) == 8)
+        // beyond T = 60, e^-T (< 9e-27) is below half an ulp of every term it enters (F_4(60) ~ 6e-8), so
+        // dropping it leaves each F_m bitwise unchanged and saves the exponential for most (point, group) pairs
+        eT = T > 60.0 ? 0.0 : exp(-T);
+    else
+        eT = expf(-T);
+    boys<TOP, C>(T, eT, tab, F);
+    C pw[L + 1];
+    pw[0] = C(1.0);
 #pragma unroll
-    for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * (-2.0 * p);
+    for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * (C(-2.0) * p);
     R[0] = pw[L] * F[L];
-    rlevels<L, L - 1>(R, X, Y, Z, F, pw);
+    rlevels<L, L - 1, C>(R, X, Y, Z, F, pw);
 }
 
 template <int L, int CH>
@@ -541,16 +549,19 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
                                         int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
                                         const double *__restrict__ tab, T *Rl) {
     constexpr int nt = ntuv(L);
+    // R in FP64 for the FP64 form; in FP32 (table copy after the FP64 one) for FP32 and split-BF16 operands
+    using C = typename std::conditional<sizeof(T) == 8, double, float>::type;
+    const C *ctab = sizeof(T) == 8 ? (const C *)tab : (const C *)(tab + kBoysRows * kBoysCols);
     const int np = ch.pe - ch.pb;
     const int glo = np * q / nq, ghi = np * (q + 1) / nq;
     for (int g = glo + rw; g < ghi; g += nrw) {
         const double4 P = pP[ch.pb + g];
-        double R[nt];
+        C R[nt];
         if (valid) {
-            hermite_R<L>(P.w, P.x - px, P.y - py, P.z - pz, tab, R);
+            hermite_R<L, C>((C)P.w, (C)(P.x - px), (C)(P.y - py), (C)(P.z - pz), ctab, R);
         } else {
 #pragma unroll
-            for (int t = 0; t < nt; ++t) R[t] = 0.0;
+            for (int t = 0; t < nt; ++t) R[t] = C(0.0);
         }
 #pragma unroll
         for (int t = 0; t < nt; ++t) {
@@ -606,18 +617,12 @@ __device__ __forceinline__ int stage_h_fp(const ChunkDev *__restrict__ chunks, S
 // One (point tile, column block) of one system's Coulomb contraction, executed by a whole block.  All waves
 // return from here together; the persistent batch kernel calls it once per work item.
 template <typename T, int NFB>
-__device__ __forceinline__ void coulomb_tile_fp(const CoulJob<T> &J, const double *__restrict__ tab, int64_t tile,
-                                                int cb) {
-    const ChunkDev *__restrict__ chunks = J.chunks;
-    const int nchunks = J.nchunks;
-    const double4 *__restrict__ pP = J.pP;
-    const T *__restrict__ Hf = (const T *)J.H;
-    const int nct = J.nct;
-    const double *__restrict__ X = J.X, *__restrict__ Y = J.Y, *__restrict__ Z = J.Z;
-    const int64_t M = J.M;
-    const int nF = J.nF;
-    double *__restrict__ V = J.V;
-    const int64_t ldv = J.ldv;
+__device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chunks, int nchunks,
+                                                const double4 *__restrict__ pP, const T *__restrict__ Hf, int nct,
+                                                const double *__restrict__ X, const double *__restrict__ Y,
+                                                const double *__restrict__ Z, int64_t M, int nF,
+                                                double *__restrict__ V, int64_t ldv, const double *__restrict__ tab,
+                                                int64_t tile, int cb) {
     constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns (TP = wave size)
     constexpr int SR = sizeof(T) == 4 ? 32 : (NFB <= 192 ? 16 : 8);  // Hermite rows per sub-step
     constexpr int NRW = 7, NCPW = 1;                                 // R waves, copy waves
@@ -744,7 +749,8 @@ __device__ __forceinline__ void coulomb_tile_fp(const CoulJob<T> &J, const doubl
 
 template <typename T, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(CoulJob<T> J, const double *__restrict__ tab) {
-    coulomb_tile_fp<T, NFB>(J, tab, blockIdx.x, blockIdx.y);
+    coulomb_tile_fp<T, NFB>(J.chunks, J.nchunks, J.pP, (const T *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab,
+                            blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -802,19 +808,12 @@ __device__ __forceinline__ int stage_h_bf16(const ChunkDev *__restrict__ chunks,
 }
 
 template <int NP, int NFB>
-__device__ __forceinline__ void coulomb_tile_bf16(const CoulJob<__bf16> &J, const double *__restrict__ tab,
-                                                  int64_t tile, int cb) {
-    const ChunkDev *__restrict__ chunks = J.chunks;
-    const int nchunks = J.nchunks;
-    const int64_t nsteps = J.nsteps;
-    const double4 *__restrict__ pP = J.pP;
-    const __bf16 *__restrict__ Hs = (const __bf16 *)J.H;
-    const int nct = J.nct;
-    const double *__restrict__ X = J.X, *__restrict__ Y = J.Y, *__restrict__ Z = J.Z;
-    const int64_t M = J.M;
-    const int nF = J.nF;
-    double *__restrict__ V = J.V;
-    const int64_t ldv = J.ldv;
+__device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ chunks, int nchunks, int64_t nsteps,
+                                                  const double4 *__restrict__ pP, const __bf16 *__restrict__ Hs,
+                                                  int nct, const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int64_t M, int nF,
+                                                  double *__restrict__ V, int64_t ldv,
+                                                  const double *__restrict__ tab, int64_t tile, int cb) {
     constexpr int TP = 64, NCW = NFB / 32, SR = 32;  // consumer wave: 16 points x NFB/2 columns
     constexpr int NCPW = NP == 3 ? 2 : 1, NRW = kConsumerWave0 - NCPW;
     constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
@@ -948,7 +947,8 @@ __device__ __forceinline__ void coulomb_tile_bf16(const CoulJob<__bf16> &J, cons
 
 template <int NP, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(CoulJob<__bf16> J, const double *__restrict__ tab) {
-    coulomb_tile_bf16<NP, NFB>(J, tab, blockIdx.x, blockIdx.y);
+    coulomb_tile_bf16<NP, NFB>(J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
+                               J.nF, J.V, J.ldv, tab, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -967,7 +967,9 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_batch(const CoulJob
     int j = 0;
     while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
     const unsigned local = it - (unsigned)jt.off[j];  // items of one launch: < 2^31
-    coulomb_tile_fp<T, NFB>(jt.job[j], tab, local / (unsigned)jt.job[j].ncb, (int)(local % (unsigned)jt.job[j].ncb));
+    const CoulJob<T> &J = jt.job[j];
+    coulomb_tile_fp<T, NFB>(J.chunks, J.nchunks, J.pP, (const T *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab,
+                            local / (unsigned)J.ncb, (int)(local % (unsigned)J.ncb));
 }
 
 template <int NP, int NFB>
@@ -977,8 +979,9 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16_batch(const Co
     int j = 0;
     while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
     const unsigned local = it - (unsigned)jt.off[j];
-    coulomb_tile_bf16<NP, NFB>(jt.job[j], tab, local / (unsigned)jt.job[j].ncb,
-                               (int)(local % (unsigned)jt.job[j].ncb));
+    const CoulJob<__bf16> &J = jt.job[j];
+    coulomb_tile_bf16<NP, NFB>(J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
+                               J.nF, J.V, J.ldv, tab, local / (unsigned)J.ncb, (int)(local % (unsigned)J.ncb));
 }
 
 // FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
@@ -1612,6 +1615,8 @@ void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t
 // host: Boys table (fffmt3.F:24-65): F_19 by its series, downward recursion, F_0..F_7 kept
 // ---------------------------------------------------------------------------------------------------
 void boys_table(double *tab) {
+    // rows 0 .. kBoysRows-1 in FP64, followed by the same table rounded to FP32
+    float *tab32 = (float *)(tab + kBoysRows * kBoysCols);
     for (int i = 0; i < kBoysCols; ++i) tab[i] = 1.0 / (2 * (i + 1) - 1);
     for (int L = 2; L <= kBoysRows; ++L) {
         const double T = 1.0e-2 * (L - 1);
@@ -1632,6 +1637,7 @@ void boys_table(double *tab) {
         }
         for (int i = 0; i < kBoysCols; ++i) tab[(L - 1) * kBoysCols + i] = G[i + 1];
     }
+    for (int i = 0; i < kBoysRows * kBoysCols; ++i) tab32[i] = (float)tab[i];
 }
 
 }  // namespace flosic
