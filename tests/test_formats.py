@@ -1,0 +1,149 @@
+"""On-disk formats of the reference (publicrelease_2020_amd/formats.py) on the reference's own example inputs
+(tests/golden/reference_inputs: the SYMBOL, ISYMGEN and FRMIDT files of the hydrogen and neon examples) and by
+round trips through the writers.
+
+Pins of the basis reader: the NRLMOL basis files store their first contraction normalised (the .basis entries
+are coef * alpha^(0.75+(l-1)/2) of a normalised contraction), so with the exponents sorted and the coefficients
+carried along as READINP does, the first s contraction of every atom has self-overlap 1 (and neon's first p
+contraction 1/3 per Cartesian component); the bare functions (one coefficient 1) have the closed-form
+self-overlap (pi/2a)^(3/2) / (4a)^l of x^l e^{-a r^2}.  A wrong permutation of the coefficients breaks them.
+"""
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from publicrelease_2020_amd import formats as Fm
+from publicrelease_2020_amd import system as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EX = os.path.join(HERE, "golden", "reference_inputs")
+
+
+def _load(atom):
+    d = os.path.join(EX, atom)
+    sym = Fm.read_symbol(open(os.path.join(d, "SYMBOL")).read())
+    inp = Fm.read_input(Fm.expand_symbolic(open(os.path.join(d, "ISYMGEN")).read(), sym.symbols))
+    return sym, inp, d
+
+
+def test_symbol_file():
+    sym, _, _ = _load("hydrogen")
+    assert sym.mode == "SCF-ONLY" and sym.functional == "LDA-PW91*LDA-PW91" and sym.mesh == "OLDMESH"
+    assert sym.files == [("ISYMGEN", "INPUT")]
+    assert sym.symbols[0][0] == "ALL-HYD001" and sym.symbols[1] == ("ELECTRONS", " 1.000000 0.0")
+    assert Fm.functional_xc(sym.functional) == S.XC_LSDA
+
+
+def test_symbol_line_rules():
+    # blanks collapse to one, the name loses its blanks, '!' ends the line, names keep at most 16 characters
+    assert Fm._split_symbol_line("AB C =   1.0    2.0  ! note") == ("ABC", " 1.0 2.0 ")
+    assert Fm._split_symbol_line("A" * 20 + " = 3")[0] == "A" * 16
+    # the last symbol whose name starts the line wins
+    out = Fm.expand_symbolic("ELECTRONS\nX\n", [("ELEC", " 1"), ("ELECTRONS", " 2 3")])
+    assert out.splitlines() == [" 2 3", "X"]
+
+
+@pytest.mark.parametrize("atom,Z,nbare,ncon,electrons", [
+    ("hydrogen", 1, 6, (4, 3, 1), (1.0, 0.0)),
+    ("neon", 10, 14, (5, 4, 3), (5.0, -5.0)),
+])
+def test_input_basis(atom, Z, nbare, ncon, electrons):
+    _, inp, _ = _load(atom)
+    b = inp.basis
+    assert b.natom == 1 and int(b.Z[0]) == Z
+    assert b.nbare(0) == nbare and tuple(int(x) for x in b.ncon[0]) == ncon
+    assert np.all(np.diff(b.alpha) < 0)  # sorted from large to small
+    assert inp.electrons == electrons and inp.wffile == "WFOUT"
+    assert b.nbas == ncon[0] + 3 * ncon[1] + 6 * ncon[2]
+    Smat = S.overlap_matrix(b)
+    d = np.diag(Smat)
+    assert abs(d[0] - 1.0) < 1e-5                     # first s contraction normalised
+    if atom == "neon":
+        assert abs(d[ncon[0]] - 1.0 / 3.0) < 1e-5     # first p contraction: 1/3 per component
+    # single bare Gaussians with coefficient 1: closed-form self-overlaps of x^l e^{-a r^2}
+    nbare_fn = 0
+    for l, first in ((0, 0), (1, ncon[0])):
+        blk = b.con_block(0, l)
+        for c in range(ncon[l]):
+            nz = np.flatnonzero(blk[c])
+            if nz.size == 1 and blk[c, nz[0]] == 1.0:
+                a = b.alpha[nz[0]]
+                ref = (math.pi / (2 * a)) ** 1.5 / (4 * a) ** l
+                assert abs(d[first + c] - ref) < 1e-12 * ref
+                nbare_fn += 1
+    assert nbare_fn >= 2
+
+
+def test_frmidt():
+    up, dn = Fm.read_frmorb(open(os.path.join(EX, "hydrogen", "FRMIDT")).read())
+    assert up.shape == (1, 3) and dn.shape == (0, 3)
+    np.testing.assert_allclose(up[0], [0.2999987484, -7.880652e-4, -3.604181e-4])
+    up, dn = Fm.read_frmorb(open(os.path.join(EX, "neon", "FRMIDT")).read())
+    assert up.shape == (5, 3) and dn.shape == (5, 3)
+    np.testing.assert_allclose(dn[-1], [-3.52133765e-2, 0.827737999, -0.6923088207])
+    u2, d2 = Fm.read_frmorb(Fm.write_frmorb(up, dn))
+    np.testing.assert_array_equal(u2, up)
+    np.testing.assert_array_equal(d2, dn)
+
+
+def test_fforce_format():
+    deb = np.array([[1.0, -2.5, 3.25e-6], [0.123456789012345, -1e-13, 12.0]])
+    txt = Fm.write_fforce(deb[:1], deb[1:])
+    lines = txt.splitlines()
+    assert len(lines) == 2 and all(len(l) == 61 for l in lines)  # ' ' + 3F20.12
+    assert lines[0] == " " + "      1.000000000000     -2.500000000000      0.000003250000"
+    np.testing.assert_allclose(Fm.read_fforce(txt), deb, atol=5e-13)
+
+
+def test_wfout_round_trip_and_layout():
+    rng = np.random.default_rng(3)
+    wf = Fm.Wavefunctions(nspn=2, nwf=7, nwfs=[4, 3], evlocc=rng.normal(size=7),
+                          occupancy=[np.ones(4), np.array([1.0, 1.0, 0.0])],
+                          coef=[rng.normal(size=(9, 4)), rng.normal(size=(9, 3))])
+    data = Fm.write_wfout(wf)
+    # first record: NSPN framed by its byte length (4) on both sides
+    assert struct.unpack_from("<3i", data, 0) == (4, 2, 4)
+    back = Fm.read_wfout(data)
+    assert back.nspn == 2 and back.nwf == 7 and back.nwfs == [4, 3]
+    for s in range(2):
+        np.testing.assert_array_equal(back.coef[s], wf.coef[s])
+        np.testing.assert_array_equal(back.occupancy[s], wf.occupancy[s])
+    with pytest.raises(ValueError):
+        Fm.read_wfout(data[:-1] + b"\x01")
+
+
+def test_mesh_round_trip():
+    rng = np.random.default_rng(4)
+    xyz, w = rng.normal(size=(11, 3)), rng.random(11)
+    data = Fm.write_mesh(xyz, w, icalc=3)
+    assert struct.unpack_from("<4i", data, 0) == (8, 11, 3, 8)
+    x2, w2, ic = Fm.read_mesh(data)
+    assert ic == 3
+    np.testing.assert_array_equal(x2, xyz)
+    np.testing.assert_array_equal(w2, w)
+
+
+def test_load_run_directory(tmp_path):
+    """A run directory in the reference's layout becomes one evaluation input: the example's SYMBOL/ISYMGEN and
+    FRMIDT, with a WFOUT and OLDMESH written here (orthonormal orbitals of the neon basis, a small mesh)."""
+    src = os.path.join(EX, "neon")
+    for f in ("SYMBOL", "ISYMGEN", "FRMIDT"):
+        (tmp_path / f).write_text(open(os.path.join(src, f)).read())
+    _, inp, _ = _load("neon")
+    b = inp.basis
+    Smat = S.overlap_matrix(b)
+    rng = np.random.default_rng(5)
+    C = S.random_orbitals(Smat, 5, 8, rng)
+    wf = Fm.Wavefunctions(nspn=2, nwf=16, nwfs=[8, 8], evlocc=np.zeros(16),
+                          occupancy=[np.r_[np.ones(5), np.zeros(3)]] * 2, coef=[C, C])
+    (tmp_path / "WFOUT").write_bytes(Fm.write_wfout(wf))
+    mesh = S.becke_mesh(b.xyz, np.random.default_rng(6), nrad=10, nang=14)
+    (tmp_path / "OLDMESH").write_bytes(Fm.write_mesh(mesh.xyz, mesh.w))
+    sysm = Fm.load_run(str(tmp_path))
+    assert sysm.nocc == [5, 5] and sysm.xc == S.XC_LSDA and sysm.nspn == 2
+    assert sysm.basis.nbas == b.nbas and sysm.mesh.npts == mesh.npts
+    np.testing.assert_array_equal(sysm.coef[0], C)
+    assert sysm.fod[0].shape == (5, 3)
