@@ -33,15 +33,17 @@ PEAK = {"fp64": ("f64", 78.6), "mixed": ("f32", 157.3), "bf16x3": ("bf16", 2500.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "coulomb_k64_traffic.txt")
 
 
-def coulomb_kernel_name(prec: str, nF: int) -> str:
-    """Name of the Coulomb kernel instantiation the library launches for this form and FLO count."""
-    cols = 64 if nF <= 64 else 128 if nF <= 128 else 192 if nF <= 192 else 256
+def coulomb_kernel_name(prec: str, nF: int, cols: int = 0) -> str:
+    """Name of the Coulomb kernel instantiation the library launches for this form, FLO count and tile width
+    (0: the widest, the library's default)."""
+    if cols <= 0:
+        cols = 64 if nF <= 64 else 128 if nF <= 128 else 192 if nF <= 192 else 256
     if prec == "fp64":
         return f"k_coulomb_ws<double, {cols}>"
     if prec == "mixed":
         return f"k_coulomb_ws<float, {cols}>"
     npc = 3 if prec == "bf16x3" else 2
-    return f"k_coulomb_ws_bf16<{npc}, {cols if nF <= 192 else 128}>"
+    return f"k_coulomb_ws_bf16<{npc}, {min(cols, 192) if cols != 256 else 128}>"
 
 
 def measured_traffic(kernel: str, atoms: int):
@@ -72,7 +74,7 @@ def coulomb_flops(counts) -> float:
     return 2.0 * M * rows * nF
 
 
-def run_form(sysm, prec_name: str, steps: int, warmup: int) -> dict:
+def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 0) -> dict:
     import numpy as np
 
     from publicrelease_2020_amd import flosic as F
@@ -82,6 +84,7 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int) -> dict:
     try:
         t0 = time.perf_counter()
         ctx.load_system(sysm, prec)
+        ctx.set_coulomb_cols(coulomb_cols)
         for _ in range(max(1, warmup)):
             e = ctx.evaluate()
         log(f"{prec_name}: set-up + {max(1, warmup)} warmup eval(s) {time.perf_counter() - t0:.2f} s, "
@@ -98,7 +101,8 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int) -> dict:
         out = dict(prec=prec_name, evals_s=steps / dt, ms_per_step=1000.0 * dt / steps, esic=e, forces=forces,
                    kernel_ms=float(np.mean(kern_ms)), phases=np.mean(np.array(phases), axis=0),
                    counts=ctx.counts())
-        log(f"{prec_name}: {out['ms_per_step']:.1f} ms/eval, Coulomb kernel {out['kernel_ms']:.1f} ms")
+        log(f"{prec_name}: {out['ms_per_step']:.1f} ms/eval, Coulomb kernel {out['kernel_ms']:.1f} ms "
+            f"({ctx.coulomb_cols()} FLO columns per block)")
         return out
     finally:
         ctx.close()
@@ -327,6 +331,8 @@ def main() -> int:
     ap.add_argument("--atoms", type=int, default=64, choices=(16, 32, 64),
                     help="K of the synthetic workload: 64 (precision workload), 16 (plain FP64), 32 (PBE)")
     ap.add_argument("--forms", default="mixed,bf16x3", help="reduced-precision forms to time after FP64")
+    ap.add_argument("--coulomb-cols", type=int, default=0,
+                    help="FLO columns per Coulomb block: 0 widest, -1 autotuned, or 64/128/192/256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256, help="mesh points in the CPU oracle sample")
     ap.add_argument("--workload", default="precision", choices=("precision", "batch", "adjoint"),
@@ -359,10 +365,10 @@ def main() -> int:
     log(f"workload K={sysm.meta['K']} nbas={sysm.nbas} npts={sysm.mesh.npts} nocc={sysm.nocc} "
         f"xc={'PBE' if sysm.xc else 'LSDA'} (generated in {time.perf_counter() - t0:.1f} s)")
 
-    ref = run_form(sysm, "fp64", args.steps, args.warmup)
+    ref = run_form(sysm, "fp64", args.steps, args.warmup, args.coulomb_cols)
     forms = [ref]
     for name in [f for f in args.forms.split(",") if f]:
-        r = run_form(sysm, name, args.steps, args.warmup)
+        r = run_form(sysm, name, args.steps, args.warmup, args.coulomb_cols)
         r["dE"] = abs(r["esic"] - ref["esic"])
         r["dF"] = float(np.max(np.abs(r["forces"] - ref["forces"])))
         r["accepted"] = bool(r["dE"] < E_TOL and r["dF"] < F_TOL)
@@ -373,7 +379,7 @@ def main() -> int:
     best = max(ok, key=lambda r: r["evals_s"]) if ok else ref
     flops = coulomb_flops(best["counts"])
     arith, peak = PEAK[best["prec"]]
-    kname = coulomb_kernel_name(best["prec"], int(best["counts"][4]))
+    kname = coulomb_kernel_name(best["prec"], int(best["counts"][4]), int(best["counts"][8]))
     # compulsory HBM bytes of one launch: H once in the form's operand layout, V written once, mesh coordinates
     c = best["counts"]
     hbytes = {"fp64": 8, "mixed": 4, "bf16x3": 6, "bf16x2": 4}[best["prec"]]

@@ -15,9 +15,13 @@ Engines are duck-typed: set_fods(ispn, fods), evaluate() -> E_SIC, fod_gradient(
 from __future__ import annotations
 
 import dataclasses
+import math
+import re
 from typing import Callable, List, Optional, Sequence
 
 import numpy as np
+
+ELOWDIN = 2  # status of a singular Fermi-orbital overlap (include/flosic_hip.h FLOSIC_ELOWDIN)
 
 
 @dataclasses.dataclass
@@ -110,9 +114,16 @@ class _Problem:
                                for s in range(2)])
 
     def value_grad(self, x: np.ndarray):
+        """E_SIC and gradient at x; a trial point whose Fermi-orbital overlap is singular (the library's
+        FLOSIC_ELOWDIN, the reference's STOPIT in LOWSIC) counts as an infinite energy."""
         self.set(x)
-        e = float(self.engine.evaluate())
         self.evals += 1
+        try:
+            e = float(self.engine.evaluate())
+        except Exception as err:  # noqa: BLE001 - only the Lowdin failure is a property of the point
+            if getattr(err, "status", None) == ELOWDIN:
+                return math.inf, None
+            raise
         return e, self.grad()
 
 
@@ -197,7 +208,19 @@ def optimize_batch(batch, engines, fods: Sequence[tuple], fmax_tol: float = 1e-4
                 dmax = np.sqrt((d[i].reshape(-1, 3) ** 2).sum(axis=1).max())
                 t[i] = min(1.0, max_step / dmax) if dmax > 0 else 1.0
             P[i].set(x[i] + t[i] * d[i])
-        z = batch.evaluate()
+        try:
+            z = batch.evaluate()
+        except Exception as err:  # noqa: BLE001
+            m = re.search(r"system (\d+):", str(err))
+            if getattr(err, "status", None) != ELOWDIN or not m:
+                raise
+            j = int(m.group(1))  # that system's trial point is singular: shorten its step and evaluate again
+            t[j] *= 0.5
+            tries[j] += 1
+            evals += 1
+            if tries[j] >= 30:
+                active[j] = False
+            continue
         evals += 1
         for i in range(n):
             if not active[i]:

@@ -93,3 +93,31 @@ def test_batch_matches_single_systems():
         assert a.converged
         assert abs(a.energy - b.energy) < 1e-8
         np.testing.assert_allclose(a.fod_up, b.fod_up, atol=1e-4)
+
+
+class SingularRegion(FakeEngine):
+    """Raises the library's Lowdin failure (status 2) when two FODs of a spin come within 0.3 bohr."""
+
+    def evaluate(self):
+        for f in self.f:
+            for i in range(len(f)):
+                for j in range(i + 1, len(f)):
+                    if np.linalg.norm(f[i] - f[j]) < 0.3:
+                        err = RuntimeError("flosic status 2: Lowdin eigenvalue below 1e-8")
+                        err.status = 2
+                        raise err
+        return super().evaluate()
+
+
+def test_singular_trial_points_are_backtracked():
+    # wells at the same point pull the FODs together; the optimiser must stop short of the singular region
+    cu = np.zeros((2, 3))
+    cd = np.zeros((1, 3))
+    fu = np.array([[0.6, 0.0, 0.0], [-0.6, 0.0, 0.0]])
+    fd = np.array([[0.1, 0.2, 0.0]])
+    eng = SingularRegion(cu, cd)
+    res = fodopt.optimize(eng, fu, fd, max_steps=50, max_step=1.0)
+    assert np.linalg.norm(res.fod_up[0] - res.fod_up[1]) >= 0.3
+    energies = [h[1] for h in res.history]
+    assert all(b <= a + 1e-14 for a, b in zip(energies, energies[1:]))
+
