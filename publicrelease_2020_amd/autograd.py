@@ -3,7 +3,9 @@
 `fod_energy(ctx, fod_up, fod_dn)` returns E_SIC (a 0-d float64 tensor) evaluated by the HIP library for the
 given FOD positions (n_up x 3 and n_dn x 3 float64 CPU tensors); its backward pass returns the library's
 adjoint gradient dE_SIC/da (flosic_fod_gradient: dE/dTMAT = 2 SIC^T TMAT back-propagated through the Lowdin
-step and the Fermi-orbital construction), which equals the FRMORB2 FOD force DEBDAX (frmorb2.F:589-729).
+step and the Fermi-orbital construction, with SIC the exact mesh derivative of the discrete energy), which for
+LSDA equals the FRMORB2 FOD force DEBDAX (frmorb2.F:589-729); for PBE DEBDAX uses the Hessian-form GGA potential
+and differs from the derivative of the energy.
 The evaluation runs on the GPU inside the library; the tensors exchanged with torch are small host arrays,
 so the function composes with any torch optimiser over FOD positions (the role of FOD_OPT/FOD_LBFGS,
 electronic_geometry_lbfgs.f:143-146).

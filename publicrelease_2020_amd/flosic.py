@@ -6,7 +6,8 @@ The methods carry the names of the reference subroutines they replace (FLOSIC/Pu
   siclagm()        SICLAGM                     SIC(j,i), j over all orbitals               frmorb_mpi.F:2-178
   sic_vab()        AHAMMAT in FLO space        V^i_ab                                       sicmat.F:380-411
   fod_forces()     FRMORB2(-ISPN,-1)           DEBDAX = +dE_SIC/da                          frmorb2.F:589-729
-  fod_gradient()   adjoint dE_SIC/da           (no reference entry point; equals DEBDAX)
+  fod_gradient()   adjoint dE_SIC/da           (no reference entry point; equals DEBDAX for LSDA, the exact
+                                               derivative of the discrete energy for PBE)
   potential()      ZPOTnnnn                    V_i = V_xc + V_H on the mesh                 apotnl_sic.F:497-522
 Errors are raised as FlosicError carrying the library status, the way the reference stops (STOPIT).
 """
