@@ -183,6 +183,10 @@ class Flosic:
         return dict(fmat=fm.reshape(4, n, n), over=ov.reshape(n, n).T, q=q, U=U.reshape(n, n).T,
                     tmat=tm.reshape(n, n).T)
 
+    def lowdin_q(self, ispn: int) -> np.ndarray:
+        """Eigenvalues of the Fermi-orbital overlap of spin ispn (EVAL before inversion, lowden_files.f:71-99)."""
+        return self.fermi_orbitals(ispn)["q"]
+
     def evaluate(self) -> float:
         z = np.zeros(1)
         self._chk(self._L.flosic_evaluate(self._h, _d(z)))

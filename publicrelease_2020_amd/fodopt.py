@@ -113,9 +113,10 @@ class _Problem:
         return np.concatenate([np.asarray(self.engine.fod_gradient(s + 1)).reshape(-1, 3)[self.free[s]].ravel()
                                for s in range(2)])
 
-    def value_grad(self, x: np.ndarray):
-        """E_SIC and gradient at x; a trial point whose Fermi-orbital overlap is singular (the library's
-        FLOSIC_ELOWDIN, the reference's STOPIT in LOWSIC) counts as an infinite energy."""
+    def value_grad(self, x: np.ndarray, qmin_floor: float = 0.0):
+        """E_SIC and gradient at x.  A trial point whose Fermi-orbital overlap is singular (the library's
+        FLOSIC_ELOWDIN, the reference's STOPIT in LOWSIC) or has an eigenvalue below qmin_floor counts as an
+        infinite energy."""
         self.set(x)
         self.evals += 1
         try:
@@ -124,7 +125,13 @@ class _Problem:
             if getattr(err, "status", None) == ELOWDIN:
                 return math.inf, None
             raise
+        if qmin_floor > 0.0 and self.qmin() < qmin_floor:
+            return math.inf, None
         return e, self.grad()
+
+    def qmin(self) -> float:
+        """Smallest eigenvalue of the Fermi-orbital overlaps of both spins (engine.lowdin_q(ispn))."""
+        return min(float(np.min(self.engine.lowdin_q(s))) for s in (1, 2) if len(self.base[s - 1]))
 
 
 def _fmax(g: np.ndarray) -> float:
@@ -133,8 +140,11 @@ def _fmax(g: np.ndarray) -> float:
 
 def optimize(engine, fod_up, fod_dn, fmax_tol: float = 1e-4, max_steps: int = 200, memory: int = 8,
              dguess: float = 0.1, max_step: float = 0.3, frozen_up=None, frozen_dn=None,
-             callback: Optional[Callable[[int, float, float], None]] = None) -> OptResult:
-    """Minimise E_SIC over the FOD positions of one system; returns the final FODs and the FOD_OPT history."""
+             qmin_floor: float = 0.0, callback: Optional[Callable[[int, float, float], None]] = None) -> OptResult:
+    """Minimise E_SIC over the FOD positions of one system; returns the final FODs and the FOD_OPT history.
+    qmin_floor > 0 keeps the Fermi-orbital overlap eigenvalues above it (trial points below count as infinite
+    energy; FRMORB2 flags eigenvalues below 0.01 as bothersome states, frmorb2.F:539-550), for energy surfaces
+    that fall towards linearly dependent Fermi orbitals; the engine then provides lowdin_q(ispn)."""
     fu = np.asarray(fod_up, dtype=np.float64).reshape(-1, 3)
     fd = np.asarray(fod_dn, dtype=np.float64).reshape(-1, 3)
     P = _Problem(engine, fu, fd, np.zeros(len(fu), bool) if frozen_up is None else frozen_up,
@@ -154,13 +164,15 @@ def optimize(engine, fod_up, fod_dn, fmax_tol: float = 1e-4, max_steps: int = 20
         accepted = False
         for _ in range(30):  # backtracking on the Armijo condition
             xn = x + t * d
-            en, gn = P.value_grad(xn)
+            en, gn = P.value_grad(xn, qmin_floor)
             if en <= e + 1e-4 * t * float(d @ g):
                 accepted = True
                 break
             t *= 0.5
         if not accepted:
             P.value_grad(x)  # leave the engine at the best point
+            break
+        if gn is None:
             break
         opt.update(xn - x, gn - g)
         x, e, g = xn, en, gn

@@ -1,13 +1,20 @@
 """In-process FOD optimisation on the library (the role of FOD_OPT): L-BFGS on E_SIC with the library's FOD
-gradient lowers the energy at every accepted step and reaches the FOD_OPT criterion (largest FOD force below
-1e-4 Ha/bohr); several systems optimised together through one batch (replayed graph) reach the same minima as
-one at a time.  Mesh 80x302 so that the gradient is the energy's derivative well below the tolerance."""
+gradient lowers the energy at every accepted step, and several systems optimised together through one batch
+(replayed graph) follow exactly the same trajectories as one at a time.
+
+The synthetic systems carry random orthonormal orbitals, not SCF orbitals; their E_SIC keeps falling towards
+linearly dependent Fermi orbitals (the oracle shows the same: the Fermi-orbital overlap eigenvalue runs to the
+1e-8 limit while the energy flattens), so a fixed number of steps is taken with the eigenvalue floor of the
+optimiser at 0.01 instead of asserting the FOD_OPT criterion, which tests/test_fodopt_host.py checks on an
+analytic energy.  Mesh 80x302 so that the gradient is the energy's derivative to a few 1e-6."""
 import numpy as np
 import pytest
 
 from publicrelease_2020_amd import system as S
 
 pytestmark = pytest.mark.gpu
+
+STEPS = 8
 
 
 def _engine(sysm):
@@ -22,30 +29,35 @@ def _start(sysm, seed):
     return [np.asarray(f) + rng.normal(scale=0.05, size=np.shape(f)) for f in sysm.fod]
 
 
-def test_fod_optimisation_converges():
+def test_fod_optimisation_descends():
     from publicrelease_2020_amd import fodopt
     sysm = S.make_synthetic(3, 3, 7, nrad=80, nang=302, box=4.5)
     eng = _engine(sysm)
     up, dn = _start(sysm, 1)
-    res = fodopt.optimize(eng, up, dn, max_steps=150)
+    res = fodopt.optimize(eng, up, dn, max_steps=STEPS, qmin_floor=0.01)
     print(f"steps {res.steps} evaluations {res.evaluations} E {res.energy:.10f} fmax {res.fmax:.2e}", flush=True)
     energies = [h[1] for h in res.history]
-    assert all(b <= a + 1e-12 for a, b in zip(energies, energies[1:]))
-    assert res.converged and res.fmax < 1e-4
-    assert energies[-1] < energies[0]
+    assert res.steps >= 5
+    assert all(b < a for a, b in zip(energies, energies[1:]))
+    assert res.history[-1][3] < res.history[0][3]  # the largest FOD force went down
+    eng.set_fods(1, res.fod_up)
+    eng.set_fods(2, res.fod_dn)
+    assert abs(eng.evaluate() - res.energy) < 1e-12
+    assert min(np.min(eng.lowdin_q(s)) for s in (1, 2)) >= 0.01
 
 
-def test_fod_optimisation_in_a_batch():
+def test_fod_optimisation_in_a_batch_follows_single_systems():
     from publicrelease_2020_amd import flosic as F
     from publicrelease_2020_amd import fodopt
     systems = [S.make_synthetic(3, 3, s, nrad=80, nang=302, box=4.5) for s in (7, 9)]
     starts = [_start(sy, 2 + i) for i, sy in enumerate(systems)]
-    single = [fodopt.optimize(_engine(sy), *st, max_steps=150) for sy, st in zip(systems, starts)]
+    single = [fodopt.optimize(_engine(sy), *st, max_steps=STEPS) for sy, st in zip(systems, starts)]
     engines = [_engine(sy) for sy in systems]
     batch = F.FlosicBatch(engines, use_graph=True)
-    res = fodopt.optimize_batch(batch, engines, starts, max_steps=150)
+    res = fodopt.optimize_batch(batch, engines, starts, max_steps=STEPS)
     print("batch counts", batch.counts(), flush=True)
     for a, b in zip(res, single):
-        assert a.converged
-        assert abs(a.energy - b.energy) < 1e-8
+        assert a.steps == b.steps
+        np.testing.assert_allclose([h[1] for h in a.history], [h[1] for h in b.history], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(a.fod_up, b.fod_up, atol=1e-12)
     assert batch.counts()[2] > 0  # replayed graph evaluations
