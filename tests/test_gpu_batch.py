@@ -181,3 +181,33 @@ def test_coulomb_column_blocks_and_autotuner(prec):
     with pytest.raises(F.FlosicError):
         e.set_coulomb_cols(192)  # does not divide the padded FLO count (128)
         e.evaluate()
+
+
+def test_batch_edge_cases():
+    """A batch of one, a mesh block larger than the mesh, and a spin without FODs (one-electron system)."""
+    F = _F()
+    sysm = S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=24, nang=50, box=5.0)
+    ref = F.Flosic(0)
+    ref.load_system(sysm)
+    e_ref = ref.evaluate()
+    one = F.Flosic(0)
+    one.load_system(sysm)
+    one.set_mesh_block(10 * sysm.mesh.w.size)  # a single block holding the whole mesh
+    b = F.FlosicBatch([one], use_graph=True)
+    for _ in range(3):
+        _close(b.evaluate(), [e_ref], 1e-11)
+    _compare(_all(one), _all(ref), 1e-11, 1e-13)
+    # spin 2 without FODs: its outputs are empty and the batch path handles it
+    h = S.make_synthetic(2, 1, 5, S.XC_LSDA, nrad=24, nang=50, box=5.0)
+    e1 = F.Flosic(0)
+    e1.load_system(h)
+    e1.set_orbitals(2, h.coef[1], 0)
+    e1.set_fods(2, np.zeros((0, 3)))
+    z_single = e1.evaluate()
+    e2 = F.Flosic(0)
+    e2.load_system(h)
+    e2.set_orbitals(2, h.coef[1], 0)
+    e2.set_fods(2, np.zeros((0, 3)))
+    z_batch = F.FlosicBatch([e2]).evaluate()
+    _close(z_batch, [z_single], 1e-13)
+    assert e2.fod_forces(2).shape == (0, 3)
