@@ -98,7 +98,8 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 
             phases.append(t)
         dt = time.perf_counter() - t0
         forces = np.concatenate([ctx.fod_forces(1), ctx.fod_forces(2)])
-        out = dict(prec=prec_name, evals_s=steps / dt, ms_per_step=1000.0 * dt / steps, esic=e, forces=forces,
+        vab = [ctx.sic_vab(1), ctx.sic_vab(2)]
+        out = dict(prec=prec_name, evals_s=steps / dt, ms_per_step=1000.0 * dt / steps, esic=e, forces=forces, vab=vab,
                    kernel_ms=float(np.mean(kern_ms)), phases=np.mean(np.array(phases), axis=0),
                    counts=ctx.counts())
         log(f"{prec_name}: {out['ms_per_step']:.1f} ms/eval, Coulomb kernel {out['kernel_ms']:.1f} ms "
@@ -371,8 +372,10 @@ def main() -> int:
         r = run_form(sysm, name, args.steps, args.warmup, args.coulomb_cols)
         r["dE"] = abs(r["esic"] - ref["esic"])
         r["dF"] = float(np.max(np.abs(r["forces"] - ref["forces"])))
+        r["dVab"] = float(max(np.max(np.abs(a - b)) for a, b in zip(r["vab"], ref["vab"])))
         r["accepted"] = bool(r["dE"] < E_TOL and r["dF"] < F_TOL)
-        log(f"{name}: |dE| = {r['dE']:.2e} Ha, max|dF| = {r['dF']:.2e} Ha/bohr, accepted={r['accepted']}, "
+        log(f"{name}: |dE| = {r['dE']:.2e} Ha, max|dF| = {r['dF']:.2e} Ha/bohr, max|dVab| = {r['dVab']:.2e} Ha, "
+            f"accepted={r['accepted']}, "
             f"speed-up {r['evals_s'] / ref['evals_s']:.2f}x")
         forms.append(r)
     ok = [r for r in forms[1:] if r["accepted"]]
@@ -412,7 +415,8 @@ def main() -> int:
                  "coulomb_kernel_ms": ref["kernel_ms"],
                  "coulomb_tflops": coulomb_flops(ref["counts"]) / (ref["kernel_ms"] * 1e-3) / 1e12},
         "forms": {r["prec"]: {"evals_s": r["evals_s"], "speedup_vs_fp64": r["evals_s"] / ref["evals_s"],
-                              "dE_Ha": r["dE"], "max_dF_Ha_per_bohr": r["dF"], "accepted": r["accepted"],
+                              "dE_Ha": r["dE"], "max_dF_Ha_per_bohr": r["dF"], "max_dVab_Ha": r["dVab"],
+                              "accepted": r["accepted"],
                               "coulomb_kernel_ms": r["kernel_ms"]} for r in forms[1:]},
         "speedup_vs_fp64": best["evals_s"] / ref["evals_s"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",

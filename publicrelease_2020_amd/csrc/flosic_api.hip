@@ -1206,7 +1206,7 @@ int eval_fields(flosic_ctx *c) {
             const int off = s == 0 ? 0 : n0;
             const int nsl = vab_slices(m);
             launch_vab(c->d_phi.p + (size_t)off * m, c->d_Vtot.p + (size_t)off * m, c->d_w.p + k0, m, n, nsl,
-                       c->d_vabpart.p, c->stream);
+                       c->d_vabpart.p, c->stream, c->prec != FLOSIC_PREC_FP64);
             HIPCHK(c, hipGetLastError());
             launch_reduce_partials(c->d_vabpart.p, 1, nsl, n * n * n, c->d_vab.p + vo, c->stream, k0 > 0);
             vo += (size_t)n * n * n;

@@ -157,6 +157,6 @@ void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, dou
 // V^i_ab for one spin: phi0 (n FLOs of this spin, [n][M]), Vtot ([n][M]), w -> partial sums [slice][i][b][a]
 int vab_slices(int64_t M);
 void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nslice,
-                double *partial /*[nslice][n^3]*/, hipStream_t s);
+                double *partial /*[nslice][n^3]*/, hipStream_t s, bool fp32 = false);
 
 }  // namespace flosic

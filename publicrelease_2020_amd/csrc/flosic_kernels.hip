@@ -1512,17 +1512,19 @@ void launch_reduce_partials(const double *partial, int nF, int nblk, int nq, dou
 // V^i_ab = sum_k Vtot_i(k) [w phi_a phi_b](k): a GEMM over the points with the pair-product operand generated in
 // registers.  Block (pair block, point slice, i block): pair block = 8 a x 8 b with a-block <= b-block (V is
 // symmetric in a,b; the mirror is stored too), 4 wavefronts, wave w takes a = 8A + 2w + {0,1} and all 8 b, i.e.
-// 16 pair columns, against up to 16*NIT i rows.  Points are staged 32 at a time; 16x16x4 FP64 MFMAs.  Partial
-// sums per slice: part[slice][i][b][a].
+// 16 pair columns, against up to 16*NIT i rows.  Points are staged 32 at a time; 16x16x4 FP64 MFMAs, or FP32
+// MFMAs (operands rounded to FP32, FP32 accumulation over each 32 staged points, FP64 sums).  Partial sums per
+// slice: part[slice][i][b][a].
 // ---------------------------------------------------------------------------------------------------
 constexpr int kVabKS = 32;
 
-template <int NIT>
+template <typename T, int NIT>
 __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi, const double *__restrict__ Vt,
                                                   const double *__restrict__ w, int64_t M, int n, int nb8,
                                                   int64_t kslice, double *__restrict__ part) {
     constexpr int NI = 16 * NIT, LDV = kVabKS + 1;
-    __shared__ double Vl[NI * LDV];
+    using Tr = MfmaTraits<T>;
+    __shared__ T Vl[NI * LDV];
     __shared__ double Pa[8 * LDV], Pb[8 * LDV], Wl[kVabKS];
     // pair block index -> (A, B), A <= B, row-major over the upper triangle
     int A = 0, rem = blockIdx.x;
@@ -1536,14 +1538,20 @@ __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi
     const int lr = lane >> 4, lc = lane & 15;
     const int la = 2 * wave + (lc >> 3), lb = lc & 7;  // this lane's pair inside the 8x8 block
     const int64_t kb = (int64_t)s * kslice, ke = min(M, kb + kslice);
-    v4d acc[NIT];
+    typename Tr::acc acc[NIT];
+    double sum[NIT][4];  // FP32 form: the FP32 accumulators of every 32 staged points are added here
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) acc[it] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int it = 0; it < NIT; ++it)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            acc[it][r] = (T)0;
+            sum[it][r] = 0.0;
+        }
     for (int64_t t0 = kb; t0 < ke; t0 += kVabKS) {
         for (int e = threadIdx.x; e < NI * kVabKS; e += 256) {
             const int i = e / kVabKS, kk = e - i * kVabKS;
             const int64_t k = t0 + kk;
-            Vl[i * LDV + kk] = (i0 + i < n && k < ke) ? Vt[(int64_t)(i0 + i) * M + k] : 0.0;
+            Vl[i * LDV + kk] = (T)((i0 + i < n && k < ke) ? Vt[(int64_t)(i0 + i) * M + k] : 0.0);
         }
         for (int e = threadIdx.x; e < 16 * kVabKS; e += 256) {
             const int r = e / kVabKS, kk = e - r * kVabKS;
@@ -1560,14 +1568,21 @@ __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi
 #pragma unroll
         for (int ks = 0; ks < kVabKS; ks += 4) {
             const int kk = ks + lr;
-            const double bv = Wl[kk] * Pa[la * LDV + kk] * Pb[lb * LDV + kk];
+            const T bv = (T)(Wl[kk] * Pa[la * LDV + kk] * Pb[lb * LDV + kk]);
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) acc[it] = Tr::mma(Vl[(it * 16 + lc) * LDV + kk], bv, acc[it]);
+        }
+        if (sizeof(T) == 4) {
 #pragma unroll
             for (int it = 0; it < NIT; ++it)
-                acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(Vl[(it * 16 + lc) * LDV + kk], bv, acc[it], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sum[it][r] += (double)acc[it][r];
+                    acc[it][r] = (T)0;
+                }
         }
         __syncthreads();
     }
-    // FP64 accumulator layout: register r of lane l holds C[l/16 + 4r][l%16]
     const int a = 8 * A + 2 * wave + (lc >> 3), b = 8 * B + (lc & 7);
     if (a >= n || b >= n) return;
     const int64_t nn = (int64_t)n * n, base = (int64_t)s * n * nn;
@@ -1575,10 +1590,11 @@ __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi
     for (int it = 0; it < NIT; ++it)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = i0 + it * 16 + lr + 4 * r;
+            const int i = i0 + it * 16 + Tr::row(lr, r);
             if (i >= n) continue;
-            part[base + i * nn + (int64_t)b * n + a] = acc[it][r];
-            if (A != B) part[base + i * nn + (int64_t)a * n + b] = acc[it][r];
+            const double v = sizeof(T) == 4 ? sum[it][r] : (double)acc[it][r];
+            part[base + i * nn + (int64_t)b * n + a] = v;
+            if (A != B) part[base + i * nn + (int64_t)a * n + b] = v;
         }
 }
 
@@ -1590,14 +1606,17 @@ int vab_slices(int64_t M) {
 }
 
 void launch_vab(const double *phi0, const double *Vtot, const double *w, int64_t M, int n, int nslice, double *part,
-                hipStream_t s) {
+                hipStream_t s, bool fp32) {
     if (n <= 0) return;
     const int nb8 = (n + 7) / 8;
     const int64_t kslice = ((M + nslice - 1) / nslice + kVabKS - 1) / kVabKS * kVabKS;
     const int nit = std::min(8, (n + 15) / 16);
     const dim3 grid((unsigned)(nb8 * (nb8 + 1) / 2), (unsigned)nslice, (unsigned)((n + 16 * nit - 1) / (16 * nit)));
-#define FLOSIC_VAB(NIT) \
-    hipLaunchKernelGGL((k_vab_mfma<NIT>), grid, dim3(256), 0, s, phi0, Vtot, w, M, n, nb8, kslice, part)
+#define FLOSIC_VAB(NIT)                                                                                         \
+    if (fp32)                                                                                                   \
+        hipLaunchKernelGGL((k_vab_mfma<float, NIT>), grid, dim3(256), 0, s, phi0, Vtot, w, M, n, nb8, kslice, part); \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_vab_mfma<double, NIT>), grid, dim3(256), 0, s, phi0, Vtot, w, M, n, nb8, kslice, part)
     switch (nit) {
         case 1: FLOSIC_VAB(1); break;
         case 2: FLOSIC_VAB(2); break;
