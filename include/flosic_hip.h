@@ -62,7 +62,9 @@ int32_t flosic_set_basis(flosic_ctx *ctx, int32_t natom, const double *xyz, cons
 /* Mesh: module MESH1 (RMSH(3,NMSH), WMSH(NMSH)). */
 int32_t flosic_set_mesh(flosic_ctx *ctx, int64_t nmsh, const double *rmsh, const double *wmsh);
 
-/* NSPN, functional and precision. */
+/* NSPN, functional and precision.  NSPN = 1 (spin-restricted): one spin channel whose SIC energies count twice
+ * (RESULTS(9) = XENG * 2/NSPN, apotnl_sic.F:581); spin 2 is empty and flosic_set_orbitals / flosic_set_fods reject
+ * it.  Switching back to NSPN = 2 requires spin 2 to be set again. */
 int32_t flosic_set_options(flosic_ctx *ctx, int32_t nspn, int32_t xc, int32_t precision);
 
 /* KS orbitals of spin ispn (1 or 2): COMMON5 PSI_COEF as unravelled by UNRAVEL2 (unravel2.f:90-225);
@@ -103,8 +105,10 @@ int32_t flosic_sic_vab(flosic_ctx *ctx, int32_t ispn, double *vab);
 int32_t flosic_fod_forces(flosic_ctx *ctx, int32_t ispn, double *debdax);
 
 /* Adjoint FOD gradient dE_SIC/da(3,nfrm): dE/dTMAT = 2 SIC^T TMAT back-propagated through the Lowdin
- * step and the Fermi-orbital construction; no counterpart entry point in the reference (it equals the
- * FRMORB2 formula). */
+ * step and the Fermi-orbital construction, with SIC the exact mesh derivative of the discrete energy (for PBE
+ * the e_rho / e_g form; LSDA's SIC is already that, and there the gradient equals the FRMORB2 formula); the
+ * Hartree term keeps the continuum symmetry <psi|V_H|phi>.  Per spin channel and without the 2/NSPN factor,
+ * as DEBDAX.  No counterpart entry point in the reference. */
 int32_t flosic_fod_gradient(flosic_ctx *ctx, int32_t ispn, double *grad);
 
 /* ZPOTnnnn analogue (apotnl_sic.F:497-522): V_i = V_xc + V_H of FLO iorb (1-based) of spin ispn on the

@@ -1725,6 +1725,16 @@ int32_t flosic_set_options(flosic_ctx *c, int32_t nspn, int32_t xc, int32_t prec
     if (precision < FLOSIC_PREC_FP64 || precision > FLOSIC_PREC_BF16X2)
         return fail(c, FLOSIC_EINVAL, "unknown precision %d", precision);
     if (xc != c->xc) c->planes_ok = false;
+    if (nspn != c->nspn) {
+        // NSPN = 1 (spin-restricted): one spin channel, its SIC energies counted twice (RESULTS(9) = XENG * 2 / NSPN,
+        // apotnl_sic.F:581); spin 2 is empty.  Back to NSPN = 2, spin 2 must be set again.
+        Spin &s2 = c->sp[1];
+        s2.nwf = s2.nocc = 0;
+        s2.coef.clear();
+        s2.fod.clear();
+        s2.have_coef = s2.have_fod = nspn == 1;
+        c->fo_done = false;
+    }
     c->nspn = nspn;
     c->xc = xc;
     c->prec = precision;
@@ -1773,6 +1783,7 @@ int32_t flosic_set_orbitals(flosic_ctx *c, int32_t ispn, int32_t nbas, int32_t n
     ++c->cfg_gen;
     const int s = spin_index(c, ispn);
     if (s < 0) return FLOSIC_EINVAL;
+    if (s == 1 && c->nspn == 1) return fail(c, FLOSIC_EINVAL, "NSPN = 1: spin 2 is not used");
     if (nbas != c->nbas) return fail(c, FLOSIC_EINVAL, "nbas %d differs from the basis (%d)", nbas, c->nbas);
     if (nocc < 0 || nwf < nocc || (nwf > 0 && !coef)) return fail(c, FLOSIC_EINVAL, "bad orbital sizes");
     Spin &sp = c->sp[s];
@@ -1789,6 +1800,7 @@ int32_t flosic_set_fods(flosic_ctx *c, int32_t ispn, int32_t nfrm, const double 
     if (!c) return FLOSIC_EINVAL;
     const int s = spin_index(c, ispn);
     if (s < 0) return FLOSIC_EINVAL;
+    if (s == 1 && c->nspn == 1) return fail(c, FLOSIC_EINVAL, "NSPN = 1: spin 2 is not used");
     if (nfrm < 0 || (nfrm > 0 && !bfrm)) return fail(c, FLOSIC_EINVAL, "bad FOD arguments");
     for (int i = 0; i < 3 * nfrm; ++i)
         if (!std::isfinite(bfrm[i])) return fail(c, FLOSIC_EINVAL, "non-finite FOD coordinate");

@@ -153,6 +153,9 @@ class Flosic:
 
     def set_options(self, nspn: int = 2, xc: int = XC_LSDA, precision: int = PREC_FP64):
         self._chk(self._L.flosic_set_options(self._h, nspn, xc, precision))
+        if nspn == 1:  # spin-restricted: spin 2 is empty
+            self._n[1] = self._nwf[1] = 0
+        self._nspn = nspn
 
     def set_orbitals(self, ispn: int, coef, nocc: int):
         c = np.asfortranarray(coef, np.float64)
@@ -162,13 +165,15 @@ class Flosic:
 
     def set_fods(self, ispn: int, fod):
         f = np.ascontiguousarray(fod, np.float64).reshape(-1, 3)
+        if ispn == 2 and getattr(self, "_nspn", 2) == 1 and f.shape[0] == 0:
+            return  # NSPN = 1 has no second spin channel
         self._chk(self._L.flosic_set_fods(self._h, ispn, f.shape[0], _d(f)))
 
     def load_system(self, sysm, precision: int = PREC_FP64):
         self.set_basis(sysm.basis)
         self.set_mesh(sysm.mesh.xyz, sysm.mesh.w)
         self.set_options(sysm.nspn, sysm.xc, precision)
-        for s in (1, 2):
+        for s in range(1, sysm.nspn + 1):
             self.set_orbitals(s, sysm.coef[s - 1], sysm.nocc[s - 1])
             self.set_fods(s, sysm.fod[s - 1])
 

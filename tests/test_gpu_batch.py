@@ -211,3 +211,24 @@ def test_batch_edge_cases():
     z_batch = F.FlosicBatch([e2]).evaluate()
     _close(z_batch, [z_single], 1e-13)
     assert e2.fod_forces(2).shape == (0, 3)
+
+
+def test_spin_restricted_counts_one_channel_twice():
+    """NSPN = 1: one spin channel whose SIC energies count twice (RESULTS(9) = XENG * 2 / NSPN) equals NSPN = 2
+    with identical spins."""
+    import dataclasses
+    F = _F()
+    sysm = S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=24, nang=50, box=5.0)
+    same = dataclasses.replace(sysm, coef=[sysm.coef[0], sysm.coef[0]], fod=[sysm.fod[0], sysm.fod[0]],
+                               nocc=[sysm.nocc[0], sysm.nocc[0]])
+    two = F.Flosic(0)
+    two.load_system(same)
+    z2 = two.evaluate()
+    one = F.Flosic(0)
+    one.load_system(dataclasses.replace(same, nspn=1))
+    z1 = one.evaluate()
+    assert abs(z1 - z2) < 1e-12 * abs(z2)
+    _close(one.fod_forces(1), two.fod_forces(1), 1e-12)
+    assert one.fod_forces(2).shape == (0, 3)
+    with pytest.raises(F.FlosicError):
+        one.set_orbitals(2, sysm.coef[1], 1)
