@@ -449,7 +449,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kCoulThreads = 1024;  // 16 wavefronts: R waves, copy waves, then 8 consumer waves
 constexpr int kConsumerWave0 = 8;
-constexpr int kSlots = 3;
+constexpr int kSlots = 3;     // FP operand stream: ring of LDS slots filled by asynchronous global->LDS loads
+constexpr int kSlotsReg = 2;  // split-BF16: H staged through the copy waves' registers, the sub-step in use and the next
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -614,6 +615,9 @@ __device__ __forceinline__ int stage_h_fp(const ChunkDev *__restrict__ chunks, S
     return n;
 }
 
+// s_waitcnt lgkmcnt(0) alone (vmcnt 63, expcnt 7)
+__device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14)); }
+
 // One (point tile, column block) of one system's Coulomb contraction, executed by a whole block.  All waves
 // return from here together; the persistent batch kernel calls it once per work item.
 template <typename T, int NFB>
@@ -754,12 +758,14 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(CoulJob<T> J, const
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces (each the rounded remainder of the previous), so the
-// product x*y is carried by x1y1 + x1y2 + x2y1 (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error,
-// every term on the 16x16x32 bf16 MFMA (lane l holds A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]; FP32
-// register r holds D[4*(l/16) + r][l%16]).  Same block organisation as k_coulomb_ws with one 32-row K step per
-// sub-step and consumer waves of 16 points x half the columns.  R is kept in LDS as FP32 [point][row]; each
-// consumer splits its 8-row A operand when it loads it; H pieces come from k_split_h in operand order.
+// Split-BF16 Coulomb.  x = x1 + x2 (+ x3) in bf16 pieces, so the product x*y is carried by x1y1 + x1y2 + x2y1
+// (+ x1y3 + x2y2 + x3y1) at ~2^-16 (2^-24) relative error, every term on the 16x16x32 bf16 MFMA (lane l holds
+// A[l%16][8*(l/16) .. +7], B[8*(l/16) .. +7][l%16]; FP32 register r holds D[4*(l/16) + r][l%16]).  Same block
+// organisation as k_coulomb_ws with one 32-row K step per sub-step and consumer waves of 16 points x half the
+// columns.  R is kept in LDS as FP32 [point][row]; each consumer splits its 8-row A operand when it loads it; H
+// pieces come from k_split_h in operand order.  The copy waves stage H through their registers (ordinary 16-byte
+// loads one sub-step ahead, 16-byte LDS stores into two slots): on the 36 KiB split-BF16 sub-steps this moves H
+// faster than the asynchronous global->LDS loads of the FP kernels, which remain faster for their 24 KiB steps.
 // ---------------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -791,21 +797,73 @@ __global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__r
     }
 }
 
-// K step of sub-step p (32 rows) of every piece for block columns [fb, fb + NFB) into an LDS slot
-// [piece][NFB/16][64][8]; returns this copy wave's load count
-template <int NP, int NFB>
-__device__ __forceinline__ int stage_h_bf16(const ChunkDev *__restrict__ chunks, SubStep p,
-                                            const __bf16 *__restrict__ Hs, int64_t nsteps, int nct, int fb,
-                                            __bf16 *slot, int cw, int ncw, int lane) {
-    constexpr int UPP = NFB / 16;  // KiB per piece
-    const int64_t g = chunks[p.c].prow0 / 32 + p.q;
-    int n = 0;
-    for (int pc = 0; pc < NP; ++pc) {
-        const char *src = (const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512);
-        n += lds_copy_async(src, (char *)(slot + (size_t)pc * UPP * 512), UPP, cw, ncw, lane, pc * UPP);
+// A operand of one K step: the 8 FP32 rows of this lane's point split into NP bf16 pieces.  NP = 3: exact split
+// by truncation, x = t(x) + t(x - t(x)) + rest with t keeping the upper 16 bits (sign, exponent, 7 mantissa bits);
+// each difference is exact and the rest has at most 8 significant bits, so it is itself a bf16.  NP = 2: rounded
+// pieces, x ~ b1 + b2 with b2 the rounded remainder.
+template <int NP>
+__device__ __forceinline__ void split_a(const float *src, v8bf (&a)[NP]) {
+    const v4f x0 = *(const v4f *)src, x1 = *(const v4f *)(src + 4);
+    const float xs[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+    if constexpr (NP == 3) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        v4u w[3];
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            unsigned u0 = __float_as_uint(xs[e]), u1 = __float_as_uint(xs[e + 1]);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                w[p][e / 2] = (u0 >> 16) | (u1 & 0xffff0000u);  // upper halves of elements e, e + 1
+                if (p < 2) {
+                    u0 = __float_as_uint(__uint_as_float(u0) - __uint_as_float(u0 & 0xffff0000u));
+                    u1 = __float_as_uint(__uint_as_float(u1) - __uint_as_float(u1 & 0xffff0000u));
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 3; ++p) a[p] = __builtin_bit_cast(v8bf, w[p]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float r = xs[e];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const __bf16 hb = (__bf16)r;
+                a[p][e] = hb;
+                r -= (float)hb;
+            }
+        }
     }
-    return n;
 }
+
+// The 1 KiB units (64 lanes x 16 bytes) of one staged K step that copy wave cw of NCPW moves, through registers:
+// unit u = pc * NFB/16 + v is KiB v of piece pc's run for the block's columns, at LDS slot offset u KiB.
+template <int NP, int NFB, int NCPW>
+struct HUnits {
+    static constexpr int UPP = NFB / 16, NU = NP * UPP, UPW = (NU + NCPW - 1) / NCPW;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    v4u r[UPW];
+    __device__ __forceinline__ void load(const ChunkDev *__restrict__ chunks, SubStep p, const __bf16 *__restrict__ Hs,
+                                         int64_t nsteps, int nct, int fb, int cw, int lane) {
+        const int64_t g = chunks[p.c].prow0 / 32 + p.q;
+#pragma unroll
+        for (int k = 0; k < UPW; ++k) {
+            const int u = cw + k * NCPW;
+            if (u < NU) {
+                const int pc = u / UPP, v = u - pc * UPP;
+                r[k] = *(const v4u *)((const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512) + v * 1024 +
+                                      lane * 16);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(__bf16 *slot, int cw, int lane) const {
+#pragma unroll
+        for (int k = 0; k < UPW; ++k) {
+            const int u = cw + k * NCPW;
+            if (u < NU) *(v4u *)((char *)slot + u * 1024 + lane * 16) = r[k];
+        }
+    }
+};
 
 template <int NP, int NFB>
 __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ chunks, int nchunks, int64_t nsteps,
@@ -819,7 +877,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
     constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
     constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
     __shared__ __attribute__((aligned(16))) float Rf[2][TP * LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlots][SLOT];
+    __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlotsReg][SLOT];
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -838,33 +896,32 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
                 __syncthreads();
             }
         }
-    } else if (wave < kConsumerWave0) {  // copy waves
+    } else if (wave < kConsumerWave0) {  // copy waves: H of sub-step gs + 2 is loaded in iteration gs, stored in gs + 1
         const int cw = wave - NRW;
+        HUnits<NP, NFB, NCPW> h;
         SubStep nxt{0, 0};
-        int gs_ahead = 0, pend = 0;
-        pend = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[0], cw, NCPW, lane);
+        h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
+        h.store(Hl[0], cw, lane);
         advance(nxt, chunks, SR);
-        ++gs_ahead;
-        if (nxt.c < nchunks) {
-            pend = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[1], cw, NCPW, lane);
+        bool held = nxt.c < nchunks;  // h holds the next sub-step
+        if (held) {
+            h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
             advance(nxt, chunks, SR);
-            ++gs_ahead;
-        } else {
-            pend = 0;
         }
-        wait_vm(pend);
+        wait_lgkm0();
         __builtin_amdgcn_s_barrier();
+        int gs = 0;
         for (int c = 0; c < nchunks; ++c) {
             const int nsub = chunks[c].kr / SR;
-            for (int q = 0; q < nsub; ++q) {
-                int issued = 0;
-                if (nxt.c < nchunks) {
-                    issued = stage_h_bf16<NP, NFB>(chunks, nxt, Hs, nsteps, nct, fb, Hl[gs_ahead % kSlots], cw, NCPW,
-                                                   lane);
+            for (int q = 0; q < nsub; ++q, ++gs) {
+                if (held) h.store(Hl[(gs + 1) % kSlotsReg], cw, lane);
+                held = nxt.c < nchunks;
+                if (held) {
+                    h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
                     advance(nxt, chunks, SR);
-                    ++gs_ahead;
                 }
-                wait_vm(issued);
+                // the stores must have landed; the loads just issued stay in flight across the barrier
+                wait_lgkm0();
                 __builtin_amdgcn_s_barrier();
             }
         }
@@ -888,28 +945,15 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
         const int nsub = chunks[c].kr / SR;
         const float *R = Rf[c & 1];
         for (int q = 0; q < nsub; ++q, ++gs) {
-            const __bf16 *Hs_ = Hl[gs % kSlots];
+            const __bf16 *Hs_ = Hl[gs % kSlotsReg];
             v8bf a[NP];
-            {  // 8 consecutive rows of this lane's point, split into NP pieces (each the rounded remainder)
-                const float *src = R + (pw + lc) * LDK + SR * q + 8 * lr;
-                const v4f x0 = *(const v4f *)src, x1 = *(const v4f *)(src + 4);
-                const float xs[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    float r = xs[e];
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        const __bf16 hb = (__bf16)r;
-                        a[p][e] = hb;
-                        r -= (float)hb;
-                    }
-                }
-            }
+            split_a<NP>(R + (pw + lc) * LDK + SR * q + 8 * lr, a);
 #pragma unroll
             for (int j = 0; j < NCW; ++j) {
                 v8bf b[NP];
 #pragma unroll
-                for (int p = 0; p < NP; ++p) b[p] = *(const v8bf *)(Hs_ + ((p * (NFB / 16) + cl0 + j) * 64 + lane) * 8);
+                for (int p = 0; p < NP; ++p)
+                    b[p] = *(const v8bf *)(Hs_ + ((p * (NFB / 16) + cl0 + j) * 64 + lane) * 8);
                 v4f cc = acc[j];
                 cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], cc, 0, 0, 0);
                 cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], cc, 0, 0, 0);
