@@ -159,10 +159,10 @@ def copy_calibration(mib: int, reps: int) -> int:
 
 def bench_batch(args) -> int:
     """The batch workload: n_sys independent systems from the batch generator (K uniform in [4,96], LSDA or PBE
-    at random, 4..128 FODs per spin), every context in mesh-block mode.  Timed: the batch in the mixed form
-    with its device part captured once and replayed (the reported value), the same systems evaluated one at a
-    time, and each system in the plain FP64 form (the accuracy reference; first evaluation of a fresh context,
-    so it includes its buffer allocations)."""
+    at random, 4..128 FODs per spin), every context in mesh-block mode.  Timed: the batch in the reduced form
+    `--batch-prec` with its device part captured once and replayed (the reported value), the same systems
+    evaluated one at a time, and each system in the plain FP64 form (the accuracy reference; first evaluation of
+    a fresh context, so it includes its buffer allocations)."""
     import numpy as np
 
     from publicrelease_2020_amd import flosic as F
@@ -189,18 +189,19 @@ def bench_batch(args) -> int:
     def forces(e):
         return np.concatenate([e.fod_forces(1), e.fod_forces(2)])
 
-    mixed = engines(F.PREC_MIXED)
+    form = args.batch_prec
+    mixed = engines({"mixed": F.PREC_MIXED, "bf16x3": F.PREC_BF16X3}[form])
     batch = F.FlosicBatch(mixed, use_graph=True)
     z = batch.evaluate()  # eager: allocations, plan
     t_eager = batch.timings()
-    log(f"mixed batch eager: {t_eager[0]:.0f} ms wall, Coulomb launches {t_eager[2]:.0f} ms")
+    log(f"{form} batch eager: {t_eager[0]:.0f} ms wall, Coulomb launches {t_eager[2]:.0f} ms")
     z = batch.evaluate()  # captured
-    log(f"mixed batch captured: {batch.timings()[0]:.0f} ms wall")
+    log(f"{form} batch captured: {batch.timings()[0]:.0f} ms wall")
     walls = []
     for _ in range(args.steps):
         z = batch.evaluate()
         walls.append(batch.timings()[0])
-        log(f"mixed batch replay: {walls[-1]:.0f} ms wall, device part {batch.timings()[1]:.0f} ms")
+        log(f"{form} batch replay: {walls[-1]:.0f} ms wall, device part {batch.timings()[1]:.0f} ms")
     t_batch = float(np.mean(walls)) * 1e-3
     f_mixed = [forces(e) for e in mixed]
     peak = sum(e.peak_device_bytes() for e in mixed)
@@ -211,7 +212,7 @@ def bench_batch(args) -> int:
         e.evaluate()
         seq_coul += e.timings()[9]
     t_seq = time.perf_counter() - t1
-    log(f"mixed one at a time: {t_seq * 1e3:.0f} ms, Coulomb kernels {seq_coul:.0f} ms")
+    log(f"{form} one at a time: {t_seq * 1e3:.0f} ms, Coulomb kernels {seq_coul:.0f} ms")
     counts = [e.counts() for e in mixed]
     batch.close()
     for e in mixed:
@@ -237,19 +238,20 @@ def bench_batch(args) -> int:
         "metric": "FLO-SIC E+FOD-force evals/s, mixed vs FP64 on one GPU at |dE|<1e-6 Ha",
         "value": n / t_batch, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
         "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "dtype": form, "data": "synthetic",
         "config": {"workload": f"batch of {n} synthetic systems, K uniform in [4,96], LSDA/PBE, 60x194 Becke "
                                f"meshes, mesh blocks of {args.batch_block} points, one step = one evaluation of "
-                               f"every system", "systems": n,
+                               f"every system", "systems": n, "precision": form,
                    "atoms": [s.meta["K"] for s in systems], "npts_total": int(sum(s.mesh.npts for s in systems)),
                    "nflo_total": int(sum(sum(s.nocc) for s in systems))},
         "batch_graph_s": t_batch, "one_at_a_time_s": t_seq, "fp64_one_at_a_time_s": t_fp64,
         "speedup_vs_one_at_a_time": t_seq / t_batch, "speedup_vs_fp64": t_fp64 / t_batch,
         "max_dE_Ha": dE, "max_dF_Ha_per_bohr": dF, "accepted": bool(dE < E_TOL and dF < F_TOL),
         "peak_device_bytes": int(peak),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["mixed"][1], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["mixed"][1], "traffic": None,
-                     "kernel": "k_coulomb_ws_batch<float, *> (all classes)", "kernel_ms": t_eager[2],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[form][1], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[form][1], "traffic": None,
+                     "kernel": ("k_coulomb_ws_batch<float, *>" if form == "mixed" else "k_coulomb_ws_bf16_batch<*>")
+                     + " (all classes)", "kernel_ms": t_eager[2],
                      "flops_per_launch": flops, "one_at_a_time_kernel_ms": seq_coul},
     }
     print(json.dumps(out), flush=True)
@@ -343,6 +345,7 @@ def main() -> int:
     ap.add_argument("--fd-coords", type=int, default=6, help="FOD coordinates checked by central differences")
     ap.add_argument("--batch-systems", type=int, default=48)
     ap.add_argument("--batch-block", type=int, default=8192, help="mesh-block points of the batch contexts")
+    ap.add_argument("--batch-prec", default="mixed", choices=("mixed", "bf16x3"), help="form of the batch workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run device copies of this size (counter calibration) and exit")
     args = ap.parse_args()
