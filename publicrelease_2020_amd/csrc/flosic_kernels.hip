@@ -545,14 +545,17 @@ __device__ __forceinline__ void put_bf16(__bf16 *Rb, int lane, int r, double v) 
     }
 }
 
+// R is computed in FP64 for the FP64 form and in FP32 for the FP32 and split-BF16 operands, with the Boys table of
+// that precision (the FP32 copy follows the FP64 table in device memory; the split-BF16 kernel reads it from LDS)
+template <typename T>
+using RType = typename std::conditional<sizeof(T) == 8, double, float>::type;
+
 template <int L, typename T, int LD, int LAYOUT, int NP>
 __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, double py, double pz, bool valid, int q,
                                         int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                        const double *__restrict__ tab, T *Rl) {
+                                        const RType<T> *__restrict__ ctab, T *Rl) {
     constexpr int nt = ntuv(L);
-    // R in FP64 for the FP64 form; in FP32 (table copy after the FP64 one) for FP32 and split-BF16 operands
-    using C = typename std::conditional<sizeof(T) == 8, double, float>::type;
-    const C *ctab = sizeof(T) == 8 ? (const C *)tab : (const C *)(tab + kBoysRows * kBoysCols);
+    using C = RType<T>;
     const int np = ch.pe - ch.pb;
     const int glo = np * q / nq, ghi = np * (q + 1) / nq;
     for (int g = glo + rw; g < ghi; g += nrw) {
@@ -588,7 +591,7 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
 template <typename T, int LD, int LAYOUT, int NP>
 __device__ __forceinline__ void produce_R(int rw, int nrw, int lane, double px, double py, double pz, bool valid,
                                           int q, int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
-                                          const double *__restrict__ tab, T *Rl) {
+                                          const RType<T> *__restrict__ tab, T *Rl) {
     switch (ch.L) {
         case 0: chunk_R<0, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
         case 1: chunk_R<1, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
@@ -643,13 +646,14 @@ __device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chu
         const int64_t k = k0 + lane;
         const bool valid = k < M;
         const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
-        produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, tab, Rl[0]);
+        const RType<T> *ctab = sizeof(T) == 8 ? (const RType<T> *)tab : (const RType<T> *)(tab + kBoysRows * kBoysCols);
+        produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, ctab, Rl[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
             const int nsub = (chunks[c].kr + SR - 1) / SR;
             for (int q = 0; q < nsub; ++q) {
                 if (c + 1 < nchunks)
-                    produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, tab,
+                    produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, ctab,
                                             Rl[(c + 1) & 1]);
                 __syncthreads();
             }
@@ -859,20 +863,26 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
     constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
     __shared__ __attribute__((aligned(16))) float Rf[2][TP * LDK];
     __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlotsReg][SLOT];
+    __shared__ __attribute__((aligned(16))) float Tb[kBoysRows * kBoysCols];  // FP32 Boys table for the R waves
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    {  // the table in LDS keeps the R waves' per-lane table lookups off the load path of the H copies
+        const v4f *src = (const v4f *)(tab + kBoysRows * kBoysCols);
+        for (int i = threadIdx.x; i < kBoysRows * kBoysCols / 4; i += kCoulThreads) ((v4f *)Tb)[i] = src[i];
+        __syncthreads();
+    }
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
         const bool valid = k < M;
         const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
-        produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, tab, Rf[0]);
+        produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, Tb, Rf[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
             const int nsub = chunks[c].kr / SR;
             for (int q = 0; q < nsub; ++q) {
                 if (c + 1 < nchunks)
-                    produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, tab,
+                    produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, Tb,
                                                 Rf[(c + 1) & 1]);
                 __syncthreads();
             }
