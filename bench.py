@@ -74,6 +74,45 @@ def coulomb_flops(counts) -> float:
     return 2.0 * M * rows * nF
 
 
+def phase_roofline(sysm, counts, phases_ms, prec_name: str) -> dict:
+    """Per-phase algorithmic work of one evaluation against the bound of each phase.  nd = field planes (1 LSDA,
+    10 PBE: value, gradient, Hessian), n_s = FLOs of spin s.
+      orbitals   phi = Phi_d C~ on the mesh (rocBLAS dgemm): 2 M N nF nd flops; reads the basis planes and writes
+                 the FLO fields, 8 M nd (N + nF) bytes
+      xc_energy  XC, V_tot, U and the energy partial sums: reads nd field planes and V_H, writes V_tot and U
+                 (PBE: and 4 planes of the exact derivative), 8 M nF (nd + 3 [+ 4]) bytes
+      sic_matrix G = Phi^T U split over point slices (dgemm): 2 M N nF flops (PBE: + 4 x 2 M n_s^2 for SIC_ex),
+                 8 M (N + nF) bytes
+      vab        V^i_ab, symmetric pair blocks: sum_s M n_s^2 (n_s + 1) flops, 8 M 2 nF bytes
+    Peaks: FP64 MFMA 78.6 TFLOP/s for the dgemms, FP64 or FP32 MFMA for V^i_ab by form, HBM 8 TB/s."""
+    M, N, nF = float(counts[2]), float(counts[3]), float(counts[4])
+    pbe = bool(sysm.xc)
+    nd = 10.0 if pbe else 1.0
+    ns = [float(n) for n in sysm.nocc]
+    vab_peak = PEAK["fp64"][1] if prec_name == "fp64" else PEAK["mixed"][1]
+    work = {
+        "orbitals": (1, 2 * M * N * nF * nd, 8 * M * nd * (N + nF), PEAK["fp64"][1]),
+        "xc_energy": (4, None, 8 * M * nF * (nd + 3 + (4 if pbe else 0)), None),
+        "sic_matrix": (5, 2 * M * N * nF + (sum(8 * M * n * n for n in ns) if pbe else 0.0), 8 * M * (N + nF),
+                       PEAK["fp64"][1]),
+        "vab": (6, sum(M * n * n * (n + 1) for n in ns), 8 * M * 2 * nF, vab_peak),
+    }
+    out = {}
+    for name, (slot, flops, nbytes, peak) in work.items():
+        sec = float(phases_ms[slot]) * 1e-3
+        if sec <= 0:
+            continue
+        tbps = nbytes / sec / 1e12
+        d = {"ms": float(phases_ms[slot]), "hbm_TBps": tbps, "hbm_frac": tbps / 8.0}
+        if flops is not None:
+            tf = flops / sec / 1e12
+            d.update({"bound": "mfma", "achieved_TFLOPs": tf, "peak_TFLOPs": peak, "frac": tf / peak})
+        else:
+            d.update({"bound": "hbm", "frac": tbps / 8.0})
+        out[name] = d
+    return out
+
+
 def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 0) -> dict:
     import numpy as np
 
@@ -430,6 +469,7 @@ def main() -> int:
         "phases_ms": {k: float(v) for k, v in zip(
             ["fo_lowdin", "orbitals", "hermite", "coulomb", "xc_energy", "sic_matrix", "vab", "forces", "total",
              "coulomb_kernel"], best["phases"])},
+        "phase_roofline": phase_roofline(sysm, best["counts"], best["phases"], best["prec"]),
         "esic": best["esic"],
     }
     if not args.no_cpu_baseline:
