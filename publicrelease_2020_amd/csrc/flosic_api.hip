@@ -164,6 +164,7 @@ struct flosic_ctx {
     std::vector<double> h_energy, h_asy;
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
+    bool vab_on_host = false;  // V^i_ab of the last evaluation copied to sp[].vab (on the first request)
     double timings[10] = {};
     int64_t counts[9] = {};
 };
@@ -1246,21 +1247,17 @@ int eval_finish(flosic_ctx *c) {
     HIPCHK(c, hipMemcpyAsync(c->h_energy.data(), c->d_energy.p, sizeof(double) * nF * 9, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_asy.data(), c->d_asy.p, sizeof(double) * nF, hipMemcpyDeviceToHost, c->stream));
     {
-        size_t lo = 0, vo = 0;
+        size_t lo = 0;
         for (int s = 0; s < 2; ++s) {
             Spin &sp = c->sp[s];
             const int n = sp.nocc;
             sp.sic.resize((size_t)sp.nwf * n);
-            sp.vab.resize((size_t)n * n * n);
-            if (n > 0) {
+            if (n > 0)
                 HIPCHK(c, hipMemcpyAsync(sp.sic.data(), c->d_lambda.p + lo, sizeof(double) * sp.nwf * n,
                                          hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipMemcpyAsync(sp.vab.data(), c->d_vab.p + vo, sizeof(double) * n * n * n, hipMemcpyDeviceToHost,
-                                         c->stream));
-            }
             lo += (size_t)sp.nwf * n;
-            vo += (size_t)n * n * n;
         }
+        c->vab_on_host = false;  // V^i_ab (n^3 per spin) stays on the device until it is asked for
     }
     if (c->xc == FLOSIC_XC_PBE) {
         size_t lo = 0;
@@ -1874,6 +1871,19 @@ int32_t flosic_sic_vab(flosic_ctx *c, int32_t ispn, double *vab) {
     if (s < 0) return FLOSIC_EINVAL;
     const int r = need_eval(c);
     if (r) return r;
+    if (!c->vab_on_host) {
+        size_t vo = 0;
+        for (int t = 0; t < 2; ++t) {
+            Spin &sp = c->sp[t];
+            const size_t cnt = (size_t)sp.nocc * sp.nocc * sp.nocc;
+            sp.vab.resize(cnt);
+            if (cnt) HIPCHK(c, hipMemcpyAsync(sp.vab.data(), c->d_vab.p + vo, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                              c->stream));
+            vo += cnt;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->vab_on_host = true;
+    }
     if (vab) std::copy(c->sp[s].vab.begin(), c->sp[s].vab.end(), vab);
     return FLOSIC_OK;
 }
