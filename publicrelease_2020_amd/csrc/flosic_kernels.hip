@@ -530,21 +530,7 @@ template <> struct MfmaTraits<float> {
 // R of the groups [np*q/nq, np*(q+1)/nq) of chunk `ch` (share q of nq) for the block's 64 points: R wave rw of
 // nrw takes every nrw-th group (wave-uniform, so the group's centre and exponent are scalar loads) and lane l
 // the point k0 + l, whose coordinates (px, py, pz) stay in registers.  LAYOUT 0: Rl[row * LD + point] in T;
-// LAYOUT 2: Rl[point * LD + row] in T; LAYOUT 1 (split-BF16): NP bf16 pieces, piece p at Rl + p * 4096, row r of
-// point l at l*64 + 8*((r/8) ^ (l%8)) + r%8 (16-byte units XOR-swizzled so the 16-byte operand reads of 8 points
-// hit distinct banks).  Share 0 also zeroes the rows beyond the groups.
-template <int NP>
-__device__ __forceinline__ void put_bf16(__bf16 *Rb, int lane, int r, double v) {
-    const int idx = lane * 64 + ((((r >> 3) ^ (lane & 7)) << 3) | (r & 7));
-    double x = v;
-#pragma unroll
-    for (int pc = 0; pc < NP; ++pc) {
-        const __bf16 b = (__bf16)(float)x;
-        Rb[pc * 4096 + idx] = b;
-        x -= (double)(float)b;
-    }
-}
-
+// LAYOUT 2: Rl[point * LD + row] in T.  Share 0 also zeroes the rows beyond the groups.
 // R is computed in FP64 for the FP64 form and in FP32 for the FP32 and split-BF16 operands, with the Boys table of
 // that precision (the FP32 copy follows the FP64 table in device memory; the split-BF16 kernel reads it from LDS)
 template <typename T>
@@ -571,20 +557,16 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
         for (int t = 0; t < nt; ++t) {
             if constexpr (LAYOUT == 0)
                 Rl[(g * nt + t) * LD + lane] = (T)R[t];
-            else if constexpr (LAYOUT == 2)
-                Rl[lane * LD + g * nt + t] = (T)R[t];
             else
-                put_bf16<NP>((__bf16 *)Rl, lane, g * nt + t, R[t]);
+                Rl[lane * LD + g * nt + t] = (T)R[t];
         }
     }
     if (q == 0)
         for (int r = np * nt + rw; r < ch.kr; r += nrw) {
             if constexpr (LAYOUT == 0)
                 Rl[r * LD + lane] = (T)0;
-            else if constexpr (LAYOUT == 2)
-                Rl[lane * LD + r] = (T)0;
             else
-                put_bf16<NP>((__bf16 *)Rl, lane, r, 0.0);
+                Rl[lane * LD + r] = (T)0;
         }
 }
 
