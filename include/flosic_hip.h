@@ -97,7 +97,9 @@ int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double
 int32_t flosic_siclagm(flosic_ctx *ctx, int32_t ispn, double *sic);
 
 /* V^i_ab = sum_k w_k phi_a V_i phi_b over the FLOs of spin ispn: nfrm^3, element (a,b,i) at
- * a + nfrm*(b + nfrm*i) (the FLO-space restriction of AHAMMAT, sicmat.F:380-411). */
+ * a + nfrm*(b + nfrm*i) (the FLO-space restriction of AHAMMAT, sicmat.F:380-411).  Computed by every
+ * evaluation; copied to the host on the first call after it (both spins), so evaluations that do not ask for it
+ * skip the n^3 transfer. */
 int32_t flosic_sic_vab(flosic_ctx *ctx, int32_t ispn, double *vab);
 
 /* FRMORB2(-ISPN,-1): DEBDAX(3,nfrm) = +dE_SIC/da (frmorb2.F:589-729; consumed as the gradient by
