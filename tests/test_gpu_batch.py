@@ -183,6 +183,31 @@ def test_coulomb_column_blocks_and_autotuner(prec):
         e.evaluate()
 
 
+@pytest.mark.parametrize("prec", [2, 3], ids=["bf16x3", "bf16x2"])
+def test_split_bf16_widest_tile(prec):
+    """The split-BF16 kernel at its widest tile (192 FLO columns: 96 + 96 FLOs) gives the same potentials as with
+    64-column tiles (the columns are independent) and stays close to the FP64 form."""
+    F = _F()
+    sysm = S.make_synthetic(16, 96, 31, S.XC_LSDA, nrad=16, nang=26, box=7.0)  # Lowdin q_min ~2e-6
+    e = F.Flosic(0)
+    e.load_system(sysm, prec)
+    z0 = e.evaluate()
+    assert e.coulomb_cols() == 192
+    v0 = e.potential(1, 5)[1]
+    r0 = _all(e)
+    e.set_coulomb_cols(64)
+    z1 = e.evaluate()
+    assert e.coulomb_cols() == 64
+    _close(e.potential(1, 5)[1], v0, 1e-13)
+    _close([z1], [z0], 1e-13)
+    _compare(_all(e), r0, 1e-12)
+    ref = F.Flosic(0)
+    ref.load_system(sysm, F.PREC_FP64)
+    z64 = ref.evaluate()
+    print(f"precision {prec}: E_SIC {z0:.10f}, FP64 {z64:.10f}", flush=True)
+    assert abs(z0 - z64) < (1e-5 if prec == 2 else 1e-3)
+
+
 def test_batch_edge_cases():
     """A batch of one, a mesh block larger than the mesh, and a spin without FODs (one-electron system)."""
     F = _F()
