@@ -78,6 +78,11 @@ module flosic_hip
       integer(c_int32_t), value :: ispn
       real(c_double) :: results(13, *), zsic
     end function
+    integer(c_int32_t) function flosic_apotnl_dft(ctx, pot, energies) bind(C, name="flosic_apotnl_dft")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: pot(*), energies(5)
+    end function
     integer(c_int32_t) function flosic_siclagm(ctx, ispn, sic) bind(C, name="flosic_siclagm")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

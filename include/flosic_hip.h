@@ -92,6 +92,15 @@ int32_t flosic_evaluate(flosic_ctx *ctx, double *zsic);
 /* APOTNL_SIC for all FLOs: RESULTS(13,nfrm) of spin ispn (apotnl_sic.F:595-623) and ZSIC. */
 int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double *zsic);
 
+/* The DFT-side potential of the last evaluation's densities (APOTNL_DFT with NSPN = 2: getvlxc.F ->
+ * subvlxc_libxc.F90:290-470 on the LDA path, Hartree part as COUPOT1 computes it): rho_s = sum of the FLO densities
+ * of spin s (an orthonormal rotation of the occupied orbitals, occupations 1), V_H = the exact potential of
+ * rho_up + rho_dn (the sum of the FLO Hartree potentials), exchange per spin at 2 rho_s (PW91EX, s = u = v = 0) and
+ * PW92 correlation (PW91LC) at the point's zeta.  pot (nmsh x 2, may be NULL): V_H + V_xc of spin up, then spin
+ * down (the POTOUT layout); energies (may be NULL): {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.
+ * LSDA, NSPN = 2 and resident fields (no mesh-block mode) only; otherwise FLOSIC_EINVAL. */
+int32_t flosic_apotnl_dft(flosic_ctx *ctx, double *pot, double *energies);
+
 /* SICLAGM: SIC(j,i) = -sum_k w_k psi~_j V_i phi_i, j over all nwf orbitals (FLOs first), i over FLOs;
  * nwf x nfrm column-major (frmorb_mpi.F:456-517). */
 int32_t flosic_siclagm(flosic_ctx *ctx, int32_t ispn, double *sic);

@@ -604,8 +604,8 @@ static void pw91ex(double DKF, double S, double U, double V, double *EXL, double
     *VX = FAC * (THRD4 * F - (U - THRD4 * S3) * FSS - V * FS);
 }
 
-/* pw91lc.f (ZET passed; SIC densities use ZET = 1) */
-static void pw91lc(double RS, double ZET, double *EC, double *VCUP) {
+/* pw91lc.f (ZET passed; SIC densities use ZET = 1); VCDN may be NULL */
+static void pw91lc3(double RS, double ZET, double *EC, double *VCUP, double *VCDN) {
     const double GAMR = 1.92366105, FZZR = 0.58482234;
     const double THRD = 0.333333333333, THRD4 = 1.333333333333;
     double ZETPR = pow(1.0 + ZET, THRD);
@@ -622,7 +622,9 @@ static void pw91lc(double RS, double ZET, double *EC, double *VCUP) {
     double ECZET = 4.0 * Z3 * F * (EP - EU + ALFM * FZZR) + FZ * (Z4 * EP - Z4 * EU - (1.0 - Z4) * ALFM * FZZR);
     double COMM = *EC - RS * ECRS * THRD - ZET * ECZET;
     *VCUP = COMM + ECZET;
+    if (VCDN) *VCDN = COMM - ECZET;
 }
+static void pw91lc(double RS, double ZET, double *EC, double *VCUP) { pw91lc3(RS, ZET, EC, VCUP, NULL); }
 
 /* pbeex.f, MODE=1 (PBE), LGGA=LPOT=1 */
 static void pbeex(double RHO, double S, double U, double V, double *EXL, double *EXN, double *VX) {
@@ -1389,10 +1391,51 @@ void oracle_sic_hamiltonian(const ob_basis *b, int64_t npts, const double *pts, 
 
 /* kernel-level access for tests: which 0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v)        */
 /* 3 PBECOR(rs,zet,t,uu,vv,ww).  out: 0: exl,exn,vx  1: ec,vcup  2: exl,exn,vx  3: ec,vcup,h,dvcup      */
+/* The DFT-side LSDA potential of the total spin densities (getvlxc.F -> subvlxc_libxc.F90:290-470 with NSPN = 2
+ * and the LDA path: exchange per spin at 2 rho_s with PW91EX(s = u = v = 0), weight ISPFAC * 0.5 * D; PW92
+ * correlation PW91LC at the point's zeta; PW91NC vanishes for t = uu = vv = ww = 0), plus the Hartree potential
+ * vh of the total density (APOTNL_DFT: POT = V_H + V_xc per spin).  pot[0..n) spin up, pot[n..2n) spin down;
+ * e = {E_x, E_c, E_H = 1/2 sum w rho vh, N_up, N_dn}. */
+void oracle_dft_lsda(int n, const double *rup, const double *rdn, const double *w, const double *vh, double *pot,
+                     double *e) {
+    const double THIRD = 0.3333333333333333;
+    const double TRPI2 = 29.6088132032680759, DKF2RS = 1.9191582926775128;
+    const double EPS = 1.0e-20;
+    for (int q = 0; q < 5; ++q) e[q] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double ru = rup[k] < 0 ? 0.0 : rup[k], rd = rdn[k] < 0 ? 0.0 : rdn[k];
+        const double D = ru + rd;
+        double vx[2] = {0.0, 0.0}, vc[2] = {0.0, 0.0}, ex = 0.0, ec = 0.0;
+        if (D >= EPS) {
+            for (int sp = 0; sp < 2; ++sp) {
+                const double Ds = 2.0 * (sp == 0 ? ru : rd);
+                if (Ds > EPS) {
+                    double EX, DEX, VX;
+                    pw91ex(pow(TRPI2 * Ds, THIRD), 0.0, 0.0, 0.0, &EX, &DEX, &VX);
+                    vx[sp] = VX;
+                    ex += 0.5 * EX * Ds;
+                }
+            }
+            const double ZET = (ru - rd) / D;
+            const double RS = DKF2RS / pow(TRPI2 * D, THIRD);
+            double EC;
+            pw91lc3(RS, ZET, &EC, &vc[0], &vc[1]);
+            ec = EC * D;
+        }
+        pot[k] = vh[k] + vx[0] + vc[0];
+        pot[n + k] = vh[k] + vx[1] + vc[1];
+        e[0] += w[k] * ex;
+        e[1] += w[k] * ec;
+        e[2] += 0.5 * w[k] * D * vh[k];
+        e[3] += w[k] * ru;
+        e[4] += w[k] * rd;
+    }
+}
+
 void oracle_xc_kernel(int which, const double *in, double *out) {
     switch (which) {
     case 0: pw91ex(in[0], in[1], in[2], in[3], out, out + 1, out + 2); break;
-    case 1: pw91lc(in[0], in[1], out, out + 1); break;
+    case 1: pw91lc3(in[0], in[1], out, out + 1, out + 2); break;
     case 2: pbeex(in[0], in[1], in[2], in[3], out, out + 1, out + 2); break;
     default: pbecor(in[0], in[1], in[2], in[3], in[4], in[5], out, out + 1, out + 2, out + 3); break;
     }

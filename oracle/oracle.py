@@ -64,6 +64,7 @@ def lib():
         L.oracle_lagmult.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_sic_hamiltonian.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_pbe_edens.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_dft_lsda.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_nbas.restype = C.c_int
         L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
         _lib = L
@@ -151,6 +152,17 @@ def xc_kernel(which: int, args) -> np.ndarray:
     out = np.zeros(4)
     lib().oracle_xc_kernel(which, _d(a), _d(out))
     return out
+
+
+def dft_lsda(rho_up, rho_dn, w, vh):
+    """DFT-side LSDA potential of the total spin densities plus the Hartree potential vh of the total density
+    (APOTNL_DFT with NSPN = 2): returns pot (2, n) and {E_x, E_c, E_H, N_up, N_dn}."""
+    ru, rd, ww, v = (np.ascontiguousarray(a, np.float64) for a in (rho_up, rho_dn, w, vh))
+    n = ru.size
+    pot = np.zeros(2 * n)
+    e = np.zeros(5)
+    lib().oracle_dft_lsda(n, _d(ru), _d(rd), _d(ww), _d(v), _d(pot), _d(e))
+    return pot.reshape(2, n), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), e))
 
 
 def eigh(A):

@@ -160,7 +160,7 @@ struct flosic_ctx {
     // evaluation buffers
     int nF = 0, ldh = 0;
     DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_Gpart, d_vabpart, d_vab,
-        d_lambda, d_asy, d_Ux, d_lamex;
+        d_lambda, d_asy, d_Ux, d_lamex, d_dft, d_dftpart;
     std::vector<double> h_energy, h_asy;
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
@@ -1852,6 +1852,35 @@ int32_t flosic_apotnl_sic(flosic_ctx *c, int32_t ispn, double *results, double *
     if (r) return r;
     if (results) std::copy(c->sp[s].results.begin(), c->sp[s].results.end(), results);
     if (zsic) *zsic = c->zsic;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
+    if (!c) return FLOSIC_EINVAL;
+    if (c->xc != FLOSIC_XC_LSDA) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft: LSDA only");
+    if (c->nspn != 2) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft: NSPN = 2 only");
+    if (c->block_pts > 0)
+        return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft needs the resident FLO fields (mesh-block mode is set)");
+    const int r = need_eval(c);
+    if (r) return r;
+    const int64_t M = c->M;
+    const int nblk = (int)((M + 255) / 256);
+    ALLOC(c, c->d_dft, (size_t)2 * M);
+    ALLOC(c, c->d_dftpart, (size_t)5 * nblk);
+    launch_dft_lsda(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
+                    c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> part((size_t)5 * nblk);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->d_dftpart.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (pot)
+        HIPCHK(c, hipMemcpyAsync(pot, c->d_dft.p, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (energies) {  // block partials summed in block order (deterministic)
+        for (int q = 0; q < 5; ++q) energies[q] = 0.0;
+        for (int b = 0; b < nblk; ++b)
+            for (int q = 0; q < 5; ++q) energies[q] += part[(size_t)5 * b + q];
+    }
     return FLOSIC_OK;
 }
 

@@ -1178,6 +1178,25 @@ __device__ __forceinline__ void pw91lc_pol(double RS, double &EC, double &VCUP) 
     VCUP = COMM + ECZET;
 }
 
+// PW92 local correlation at general zeta (pw91lc.f:4-43): EC and both spin potentials
+__device__ __forceinline__ void pw91lc_gen(double RS, double ZET, double &EC, double &VCUP, double &VCDN) {
+    const double GAMR = 1.92366105, FZZR = 0.58482234, THRD = 0.333333333333, THRD4 = 1.333333333333;
+    const double ZETPR = pow(1.0 + ZET, THRD), ZETMR = pow(1.0 - ZET, THRD);
+    const double F = (ZETPR * ZETPR * ZETPR * ZETPR + ZETMR * ZETMR * ZETMR * ZETMR - 2.0) * GAMR;
+    double EU, EURS, EP, EPRS, ALFM, ALFRSM;
+    gcor(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, RS, EU, EURS);
+    gcor(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, RS, EP, EPRS);
+    gcor(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, RS, ALFM, ALFRSM);
+    const double Z3 = ZET * ZET * ZET, Z4 = Z3 * ZET;
+    EC = EU * (1.0 - F * Z4) + EP * F * Z4 - ALFM * F * (1.0 - Z4) * FZZR;
+    const double ECRS = EURS * (1.0 - F * Z4) + EPRS * F * Z4 - ALFRSM * F * (1.0 - Z4) * FZZR;
+    const double FZ = THRD4 * (ZETPR - ZETMR) * GAMR;
+    const double ECZET = 4.0 * Z3 * F * (EP - EU + ALFM * FZZR) + FZ * (Z4 * EP - Z4 * EU - (1.0 - Z4) * ALFM * FZZR);
+    const double COMM = EC - RS * ECRS * THRD - ZET * ECZET;
+    VCUP = COMM + ECZET;
+    VCDN = COMM - ECZET;
+}
+
 // PBE exchange (pbeex.f, MODE=1)
 __device__ __forceinline__ void pbeex(double RHO, double S, double U, double V, double &EXL, double &EXN,
                                       double &VX) {
@@ -1480,6 +1499,75 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
         hipLaunchKernelGGL((k_xc<0>), grid, blk, 0, s, phi, M, nF, VH, ldvh, w, Vtot, U, partial, nblk, blk0,
                            (double *)nullptr);
     (void)nd;
+}
+
+// DFT-side LSDA potential of the total spin densities (APOTNL_DFT with NSPN = 2, getvlxc.F ->
+// subvlxc_libxc.F90:290-470 on the LDA path): rho_s = sum of the FLO densities of spin s (the FLOs of a spin are an
+// orthonormal rotation of its occupied orbitals), V_H = sum of the FLO Hartree potentials (the potential of the total
+// density), exchange per spin at 2 rho_s (PW91EX with s = u = v = 0), PW92 correlation at the point's zeta.
+// pot[0, M) spin up, pot[M, 2M) spin down = V_H + V_xc,s; partial[block * 5 + q] = block sums of
+// {E_x, E_c, E_H = 1/2 w rho V_H, N_up, N_dn} in a fixed order.
+__global__ void __launch_bounds__(256) k_dft_lsda(const double *__restrict__ phi, int64_t M, int nup, int nF,
+                                                  const double *__restrict__ VH, int64_t ldvh,
+                                                  const double *__restrict__ w, double *__restrict__ pot,
+                                                  double *__restrict__ partial) {
+    const double THIRD = 0.3333333333333333, TRPI2 = 29.6088132032680759, DKF2RS = 1.9191582926775128;
+    const double EPS = 1.0e-20;
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double q[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (k < M) {
+        double ru = 0.0, rd = 0.0, vh = 0.0;
+        for (int f = 0; f < nF; ++f) {
+            const double p = phi[(int64_t)f * M + k];
+            if (f < nup)
+                ru += p * p;
+            else
+                rd += p * p;
+            vh += VH[(int64_t)f * ldvh + k];
+        }
+        const double D = ru + rd;
+        double vxu = 0.0, vxd = 0.0, vcu = 0.0, vcd = 0.0, ex = 0.0, ec = 0.0;
+        if (D >= EPS) {
+            for (int sp = 0; sp < 2; ++sp) {
+                const double Ds = 2.0 * (sp == 0 ? ru : rd);
+                if (Ds > EPS) {
+                    double EX, DEX, VX;
+                    pw91ex(pow(TRPI2 * Ds, THIRD), 0.0, 0.0, 0.0, EX, DEX, VX);
+                    (sp == 0 ? vxu : vxd) = VX;
+                    ex += 0.5 * EX * Ds;
+                }
+            }
+            double EC;
+            pw91lc_gen(DKF2RS / pow(TRPI2 * D, THIRD), (ru - rd) / D, EC, vcu, vcd);
+            ec = EC * D;
+        }
+        pot[k] = vh + vxu + vcu;
+        pot[M + k] = vh + vxd + vcd;
+        const double wk = w[k];
+        q[0] = wk * ex;
+        q[1] = wk * ec;
+        q[2] = 0.5 * wk * D * vh;
+        q[3] = wk * ru;
+        q[4] = wk * rd;
+    }
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) red[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[(int64_t)blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
+                     double *pot, double *partial, hipStream_t s) {
+    if (M <= 0) return;
+    hipLaunchKernelGGL(k_dft_lsda, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, phi, M, nup, nF, VH, ldvh, w,
+                       pot, partial);
 }
 
 __global__ void k_fill_zero(double *__restrict__ p, int64_t n) {

@@ -59,6 +59,7 @@ def library():
             "flosic_get_fo": [vp, C.c_int32, _dp, _dp, _dp, _dp, _dp],
             "flosic_evaluate": [vp, _dp],
             "flosic_apotnl_sic": [vp, C.c_int32, _dp, _dp],
+            "flosic_apotnl_dft": [vp, _dp, _dp],
             "flosic_siclagm": [vp, C.c_int32, _dp],
             "flosic_sic_vab": [vp, C.c_int32, _dp],
             "flosic_fod_forces": [vp, C.c_int32, _dp],
@@ -92,7 +93,7 @@ def library():
 def exported_symbols() -> list:
     return ["flosic_create", "flosic_destroy", "flosic_last_error", "flosic_set_basis", "flosic_set_mesh",
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
-            "flosic_evaluate", "flosic_apotnl_sic", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
+            "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
             "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
@@ -202,6 +203,15 @@ class Flosic:
         r = np.zeros(13 * n)
         self._chk(self._L.flosic_apotnl_sic(self._h, ispn, _d(r), None))
         return r.reshape(n, 13)
+
+    def apotnl_dft(self):
+        """DFT-side LSDA potential of the total spin densities (APOTNL_DFT): pot (2, nmsh) = V_H + V_xc per spin and
+        {ex, ec, eh, n_up, n_dn}."""
+        M = int(self.counts()[2])
+        pot = np.zeros(2 * M)
+        e = np.zeros(5)
+        self._chk(self._L.flosic_apotnl_dft(self._h, _d(pot), _d(e)))
+        return pot.reshape(2, M), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), (float(x) for x in e)))
 
     def siclagm(self, ispn: int) -> np.ndarray:
         n, nwf = self._n[ispn - 1], self._nwf[ispn - 1]

@@ -122,7 +122,7 @@ def test_xc_kernels_match_reference(golden):
         np.testing.assert_allclose(O.xc_kernel(2, xin[i])[:3], golden["xc_pbeex"][i], rtol=1e-13, atol=1e-15)
     for i in range(cin.shape[0]):
         ref = golden["xc_pw91lc"][i]
-        np.testing.assert_allclose(O.xc_kernel(1, cin[i, :2])[:2], ref[:2], rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(O.xc_kernel(1, cin[i, :2])[:3], ref[:3], rtol=1e-13, atol=1e-15)  # EC, VCUP, VCDN
         ref = golden["xc_pbecor"][i]
         got = O.xc_kernel(3, cin[i])
         np.testing.assert_allclose(got[[0, 1, 2, 3]], ref[[0, 1, 3, 4]], rtol=1e-12, atol=1e-14)
