@@ -101,6 +101,14 @@ int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double
  * LSDA, NSPN = 2 and resident fields (no mesh-block mode) only; otherwise FLOSIC_EINVAL. */
 int32_t flosic_apotnl_dft(flosic_ctx *ctx, double *pot, double *energies);
 
+/* SICLAG_DER (siclag_der.F:2-181, SICLM_DERSLV siclag_der.F:186-640, without point-group symmetry and without the
+ * libxc/meta-GGA terms): DERSIC(x, i, A) = sum_k w_k dphi_i^A/dR_A,x V_i phi_i for the FLOs of both spins (spin 1
+ * first, nF = nfrm_up + nfrm_dn) and every atom A, phi_i^A the part of FLO i on atom A's basis functions and V_i the
+ * ZPOT potential of the last evaluation; element (x, i, A) at dersic[x + 3*(i + nF*A)], 3 x nF x natom.  NSPN = 1
+ * doubles it (siclag_der.F:158-166).  The SIC part of the atomic Pulay force: FPULAY subtracts the sum over i from
+ * FRC1 (mfpulay.F:88-96). */
+int32_t flosic_siclag_der(flosic_ctx *ctx, double *dersic);
+
 /* SICLAGM: SIC(j,i) = -sum_k w_k psi~_j V_i phi_i, j over all nwf orbitals (FLOs first), i over FLOs;
  * nwf x nfrm column-major (frmorb_mpi.F:456-517). */
 int32_t flosic_siclagm(flosic_ctx *ctx, int32_t ispn, double *sic);

@@ -1389,6 +1389,41 @@ void oracle_sic_hamiltonian(const ob_basis *b, int64_t npts, const double *pts, 
     free(chi); free(phi); free(B); free(G);
 }
 
+/* SICLAG_DER -> SICLM_DERSLV without point-group symmetry, on the path without libxc or meta-GGA terms
+ * (siclag_der.F:2-181, 186-640): DERSIC(x, i, A) = sum_k w_k dphi_i^A/dR_A,x (r_k) V_i(r_k) phi_i(r_k), where
+ * phi_i^A is the part of FLO i on the basis functions of atom A and GTGRAD (gtgrad.F) supplies the negative
+ * gradient of those functions with respect to the point, i.e. their derivative with respect to the centre
+ * (siclag_der.F:444-465, 591-595).  n FLOs of both spins (cflo: nbas x n column-major, spin 1 first), vpot: n x npts
+ * (V_i = V_H + V_xc, the ZPOT files); dersic[x + 3*(i + n*A)].  NSPN = 1 doubles the result
+ * (siclag_der.F:158-166). */
+void oracle_siclag_der(const ob_basis *b, int nspn, int64_t npts, const double *pts, const double *w, int n,
+                       const double *cflo, const double *vpot, double *dersic) {
+    const int nbas = basis_nbas(b), na = b->natom;
+    double *chi = malloc(sizeof(double) * 4 * (size_t)npts * nbas), *phi = malloc(sizeof(double) * (size_t)n * npts);
+    int *f0 = malloc(sizeof(int) * (na + 1));
+    oracle_basis_eval(b, 1, (int)npts, pts, chi);
+    flo_fields(b, npts, pts, n, cflo, phi, NULL);
+    f0[0] = 0;
+    for (int a = 0; a < na; ++a) f0[a + 1] = f0[a] + b->ncon[3 * a] + 3 * b->ncon[3 * a + 1] + 6 * b->ncon[3 * a + 2];
+    const double fac = nspn == 1 ? 2.0 : 1.0;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int a = 0; a < na; ++a)
+        for (int i = 0; i < n; ++i) {
+            ksum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int64_t k = 0; k < npts; ++k) {
+                const double u = w[k] * vpot[(size_t)i * npts + k] * phi[(size_t)i * npts + k];
+                for (int x = 0; x < 3; ++x) {
+                    const double *g = chi + ((size_t)(1 + x) * npts + k) * nbas;
+                    double d = 0.0;  /* dphi_i^A/dR_A,x = -sum_{s in A} c_si dchi_s/dx */
+                    for (int f = f0[a]; f < f0[a + 1]; ++f) d -= cflo[f + (size_t)nbas * i] * g[f];
+                    kadd(&acc[x], d * u);
+                }
+            }
+            for (int x = 0; x < 3; ++x) dersic[x + 3 * ((size_t)i + (size_t)n * a)] = fac * kval(&acc[x]);
+        }
+    free(chi); free(phi); free(f0);
+}
+
 /* kernel-level access for tests: which 0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v)        */
 /* 3 PBECOR(rs,zet,t,uu,vv,ww).  out: 0: exl,exn,vx  1: ec,vcup  2: exl,exn,vx  3: ec,vcup,h,dvcup      */
 /* The DFT-side LSDA potential of the total spin densities (getvlxc.F -> subvlxc_libxc.F90:290-470 with NSPN = 2

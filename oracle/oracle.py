@@ -63,6 +63,7 @@ def lib():
         L.oracle_flo_coef.argtypes = [C.POINTER(ObBasis), C.POINTER(ObSpin), _dp]
         L.oracle_lagmult.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_sic_hamiltonian.argtypes = [C.POINTER(ObBasis), C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_siclag_der.argtypes = [C.POINTER(ObBasis), C.c_int, C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_pbe_edens.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_dft_lsda.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_nbas.restype = C.c_int
@@ -280,6 +281,24 @@ def sic_hamiltonian(sysm, ispn: int, vpot) -> np.ndarray:
     lib().oracle_sic_hamiltonian(C.byref(h.s), w.size, _d(pts), _d(w), n, _d(cf),
                                  _d(np.ascontiguousarray(vpot, np.float64)), _d(H))
     return H.reshape(N, N).T.copy()
+
+
+def siclag_der(sysm, vpot, cflo=None, basis=None) -> np.ndarray:
+    """SICLAG_DER: dersic[A, i, x] = sum_k w dphi_i^A/dR_A,x V_i phi_i over the FLOs of both spins (spin 1 first);
+    vpot[i] = V_i on the mesh (nF x npts); cflo (nbas x nF) defaults to the FLOs of both spins, basis to the
+    system's (a basis with moved centres gives the finite-difference checks their fields)."""
+    if cflo is None:
+        cflo = np.hstack([flo_coefs(sysm, 1), flo_coefs(sysm, 2)])
+    cflo = np.asfortranarray(cflo, np.float64)
+    b = sysm.basis if basis is None else basis
+    h = _BasisHold(b)
+    pts = np.ascontiguousarray(sysm.mesh.xyz, np.float64)
+    w = np.ascontiguousarray(sysm.mesh.w, np.float64)
+    n = cflo.shape[1]
+    out = np.zeros(3 * n * b.natom)
+    lib().oracle_siclag_der(C.byref(h.s), int(sysm.nspn), w.size, _d(pts), _d(w), n, _d(cflo),
+                            _d(np.ascontiguousarray(vpot, np.float64)), _d(out))
+    return out.reshape(b.natom, n, 3)
 
 
 def flo_fields(sysm, ispn: int):

@@ -160,7 +160,7 @@ struct flosic_ctx {
     // evaluation buffers
     int nF = 0, ldh = 0;
     DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_Gpart, d_vabpart, d_vab,
-        d_lambda, d_asy, d_Ux, d_lamex, d_dft, d_dftpart;
+        d_lambda, d_asy, d_Ux, d_lamex, d_dft, d_dftpart, d_dplanes, d_dphi, d_dvt, d_du, d_dG;
     std::vector<double> h_energy, h_asy;
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
@@ -1880,6 +1880,55 @@ int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
         for (int q = 0; q < 5; ++q) energies[q] = 0.0;
         for (int b = 0; b < nblk; ++b)
             for (int q = 0; q < 5; ++q) energies[q] += part[(size_t)5 * b + q];
+    }
+    return FLOSIC_OK;
+}
+
+// SICLAG_DER: G_d = (d_d chi)^T U over the mesh for d = x, y, z with U = w V_i phi_i, walked in blocks whose
+// basis planes (value and gradient; PBE: all ten), FLO fields and U are rebuilt and dropped (the energy partials of
+// that recomputation land in the evaluation's partial buffer, which is scratch once the evaluation has finished);
+// then DERSIC(x, i, A) = -sum over atom A's functions s of c_si G_x(s, i), on the host.
+constexpr int64_t kDerBlockPts = 32768;  // points per block when the planes are resident (a multiple of 256)
+
+int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
+    if (!c || !dersic) return FLOSIC_EINVAL;
+    int r = need_eval(c);
+    if (r) return r;
+    const int N = c->nbas, nF = c->nF, nd = xc_planes(c), ng = std::max(nd, 4);
+    const int64_t M = c->M;
+    const int64_t mb = c->block_pts > 0 ? mesh_block(c) : std::min<int64_t>(M, kDerBlockPts);
+    const int nblk = (int)((M + 255) / 256);
+    ALLOC(c, c->d_dplanes, (size_t)ng * N * mb);
+    ALLOC(c, c->d_dphi, (size_t)nd * nF * mb);
+    ALLOC(c, c->d_dvt, (size_t)nF * mb);
+    ALLOC(c, c->d_du, (size_t)nF * mb);
+    ALLOC(c, c->d_dG, (size_t)3 * N * nF);
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        if ((r = block_planes(c, k0, m, ng, c->d_dplanes.p))) return r;
+        if ((r = orbitals_on_points(c, c->d_dplanes.p, m, nd, c->d_cflo.p, nF, c->d_dphi.p))) return r;
+        launch_xc(c->xc, nd, c->d_dphi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_dvt.p, c->d_du.p, c->d_part.p,
+                  nblk, (int)(k0 / 256), c->stream);
+        HIPCHK(c, hipGetLastError());
+        for (int d = 0; d < 3; ++d)
+            if ((r = gemm_tn_points(c, N, nF, c->d_dplanes.p + (size_t)(1 + d) * N * m, c->d_du.p, m,
+                                    c->d_dG.p + (size_t)d * N * nF, k0 > 0)))
+                return r;
+    }
+    std::vector<double> G((size_t)3 * N * nF), cf((size_t)N * nF);
+    HIPCHK(c, hipMemcpyAsync(G.data(), c->d_dG.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cf.data(), c->d_cflo.p, sizeof(double) * cf.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double fac = c->nspn == 1 ? 2.0 : 1.0;  // siclag_der.F:158-166
+    for (int a = 0; a < c->natom; ++a) {
+        const int f0 = c->fstart[3 * a], f1 = a + 1 < c->natom ? c->fstart[3 * (a + 1)] : N;
+        for (int i = 0; i < nF; ++i)
+            for (int x = 0; x < 3; ++x) {
+                const double *g = G.data() + (size_t)x * N * nF + (size_t)N * i;
+                double acc = 0.0;
+                for (int f = f0; f < f1; ++f) acc -= cf[f + (size_t)N * i] * g[f];
+                dersic[x + 3 * ((size_t)i + (size_t)nF * a)] = fac * acc;
+            }
     }
     return FLOSIC_OK;
 }

@@ -60,6 +60,7 @@ def library():
             "flosic_evaluate": [vp, _dp],
             "flosic_apotnl_sic": [vp, C.c_int32, _dp, _dp],
             "flosic_apotnl_dft": [vp, _dp, _dp],
+            "flosic_siclag_der": [vp, _dp],
             "flosic_siclagm": [vp, C.c_int32, _dp],
             "flosic_sic_vab": [vp, C.c_int32, _dp],
             "flosic_fod_forces": [vp, C.c_int32, _dp],
@@ -93,7 +94,7 @@ def library():
 def exported_symbols() -> list:
     return ["flosic_create", "flosic_destroy", "flosic_last_error", "flosic_set_basis", "flosic_set_mesh",
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
-            "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
+            "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_siclag_der", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
             "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
@@ -145,6 +146,7 @@ class Flosic:
         self._chk(self._L.flosic_set_basis(self._h, basis.natom, _d(x), bo.ctypes.data_as(_ip), _d(al),
                                            nc.ctypes.data_as(_ip), co.ctypes.data_as(_ip), _d(cn)))
         self.nbas = basis.nbas
+        self.natom = basis.natom
 
     def set_mesh(self, xyz, w):
         xyz = np.ascontiguousarray(xyz, np.float64)
@@ -212,6 +214,13 @@ class Flosic:
         e = np.zeros(5)
         self._chk(self._L.flosic_apotnl_dft(self._h, _d(pot), _d(e)))
         return pot.reshape(2, M), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), (float(x) for x in e)))
+
+    def siclag_der(self) -> np.ndarray:
+        """SICLAG_DER: [A, i, x] = sum_k w dphi_i^A/dR_A,x V_i phi_i, FLOs of both spins (spin 1 first)."""
+        nF = self._n[0] + self._n[1]
+        d = np.zeros(3 * nF * self.natom)
+        self._chk(self._L.flosic_siclag_der(self._h, _d(d)))
+        return d.reshape(self.natom, nF, 3)
 
     def siclagm(self, ispn: int) -> np.ndarray:
         n, nwf = self._n[ispn - 1], self._nwf[ispn - 1]

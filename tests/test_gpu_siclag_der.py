@@ -1,0 +1,33 @@
+"""flosic_siclag_der (SICLAG_DER, the SIC part of the atomic Pulay force) against the oracle's oracle_siclag_der
+(pinned by central differences in tests/test_oracle_siclag_der.py) fed with the oracle's own FLO potentials V_i,
+for LSDA and PBE, with resident planes and in mesh-block mode (several blocks)."""
+import numpy as np
+import pytest
+
+from publicrelease_2020_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("xc,block", [(S.XC_LSDA, 0), (S.XC_LSDA, 512), (S.XC_PBE, 0), (S.XC_PBE, 1024)])
+def test_siclag_der_matches_oracle(xc, block):
+    from oracle import oracle as O
+    from publicrelease_2020_amd import flosic as F
+    sysm = S.make_synthetic(3, 3, 7, xc, nrad=24, nang=50, box=5.0)
+    ctx = F.Flosic(0)
+    ctx.load_system(sysm)
+    if block:
+        ctx.set_mesh_block(block)
+    ctx.evaluate()
+    d = ctx.siclag_der()
+    ref = O.evaluate(sysm, want_potentials=True, want_vab=False)
+    assert ref["status"] == 0
+    vpot = np.vstack([ref["spins"][0]["vpot"], ref["spins"][1]["vpot"]])
+    d_ref = O.siclag_der(sysm, vpot)
+    scale = np.abs(d_ref).max()
+    err = np.abs(d - d_ref).max()
+    print(f"xc {xc} block {block}: max |dDERSIC| {err:.3e} of {scale:.3e}", flush=True)
+    assert d.shape == d_ref.shape
+    assert err < 1e-10 * scale
+    # a second call on the same evaluation returns the same values (the recomputation is deterministic)
+    np.testing.assert_array_equal(ctx.siclag_der(), d)
