@@ -92,13 +92,15 @@ int32_t flosic_evaluate(flosic_ctx *ctx, double *zsic);
 /* APOTNL_SIC for all FLOs: RESULTS(13,nfrm) of spin ispn (apotnl_sic.F:595-623) and ZSIC. */
 int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double *zsic);
 
-/* The DFT-side potential of the last evaluation's densities (APOTNL_DFT with NSPN = 2: getvlxc.F ->
- * subvlxc_libxc.F90:290-470 on the LDA path, Hartree part as COUPOT1 computes it): rho_s = sum of the FLO densities
- * of spin s (an orthonormal rotation of the occupied orbitals, occupations 1), V_H = the exact potential of
- * rho_up + rho_dn (the sum of the FLO Hartree potentials), exchange per spin at 2 rho_s (PW91EX, s = u = v = 0) and
- * PW92 correlation (PW91LC) at the point's zeta.  pot (nmsh x 2, may be NULL): V_H + V_xc of spin up, then spin
- * down (the POTOUT layout); energies (may be NULL): {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.
- * LSDA, NSPN = 2 and resident fields (no mesh-block mode) only; otherwise FLOSIC_EINVAL. */
+/* The DFT-side potential of the last evaluation's densities (APOTNL_DFT with NSPN = 2: getvlxc.F -> SUBVLXC,
+ * subvlxc.F:138-323, Hartree part as COUPOT1 computes it): rho_s = sum of the FLO densities of spin s (an
+ * orthonormal rotation of the occupied orbitals, occupations 1), V_H = the exact potential of rho_up + rho_dn (the
+ * sum of the FLO Hartree potentials).  LSDA: exchange per spin at 2 rho_s (PW91EX, s = u = v = 0) and PW92
+ * correlation (PW91LC) at the point's zeta.  PBE: exchange per spin at 2 rho_s (PBEEX with the s, u, v of 2 rho_s)
+ * and PBECOR at the point's zeta with t, uu, vv of the total density and the zeta-gradient term ww.  pot
+ * (nmsh x 2, may be NULL): V_H + V_xc of spin up, then spin down (the POTOUT layout); energies (may be NULL):
+ * {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.  NSPN = 2 and resident fields (no mesh-block mode)
+ * only; otherwise FLOSIC_EINVAL. */
 int32_t flosic_apotnl_dft(flosic_ctx *ctx, double *pot, double *energies);
 
 /* SICLAG_DER (siclag_der.F:2-181, SICLM_DERSLV siclag_der.F:186-640, without point-group symmetry and without the

@@ -642,9 +642,9 @@ static void pbeex(double RHO, double S, double U, double V, double *EXL, double 
     *VX = EXUNIF * (THRD4 * FXPBE - (U - THRD4 * S2 * S) * FSS - V * FS);
 }
 
-/* pbecor.f, LGGA=LPOT=1; returns EC, VCUP, H (=DEC), DVCUP */
-static void pbecor(double RS, double ZET, double T, double UU, double VV, double WW, double *EC, double *VCUP,
-                   double *H, double *DVCUP) {
+/* pbecor.f, LGGA=LPOT=1; returns EC, VCUP, VCDN, H (=DEC), DVCUP, DVCDN */
+static void pbecor6(double RS, double ZET, double T, double UU, double VV, double WW, double *EC, double *VCUP,
+                    double *VCDN, double *H, double *DVCUP, double *DVCDN) {
     const double THRD = 1.0 / 3.0, THRDM = -THRD, THRD2 = 2.0 * THRD, SIXTHM = THRDM / 2.0, THRD4 = 4.0 * THRD;
     const double GAM = 0.5198420997897463295344212145565, FZZ = 8.0 / (9.0 * GAM);
     const double GAMMA = 0.03109069086965489503494086371273;
@@ -663,6 +663,7 @@ static void pbecor(double RS, double ZET, double T, double UU, double VV, double
     double COMM = ec - RS * ECRS / 3.0 - ZET * ECZET;
     *EC = ec;
     *VCUP = COMM + ECZET;
+    *VCDN = COMM - ECZET;
     double G = (pow(1.0 + ZET, THRD2) + pow(1.0 - ZET, THRD2)) / 2.0;
     double G3 = G * G * G;
     double PON = -ec / (G3 * GAMMA);
@@ -696,6 +697,13 @@ static void pbecor(double RS, double ZET, double T, double UU, double VV, double
     comm = comm - PREF * ZET - UU * HTT - VV * HT - WW * (HZT - FACT5);
     *H = h;
     *DVCUP = comm + PREF;
+    *DVCDN = comm - PREF;
+}
+
+static void pbecor(double RS, double ZET, double T, double UU, double VV, double WW, double *EC, double *VCUP,
+                   double *H, double *DVCUP) {
+    double VCDN, DVCDN;
+    pbecor6(RS, ZET, T, UU, VV, WW, EC, VCUP, &VCDN, H, DVCUP, &DVCDN);
 }
 
 /* One point of SUBVLXC with RHT(:,1)=rho_i (10 comps: rho, grad, xx,yy,zz,xy,xz,yz), RHT(:,2)=0,   */
@@ -1425,7 +1433,8 @@ void oracle_siclag_der(const ob_basis *b, int nspn, int64_t npts, const double *
 }
 
 /* kernel-level access for tests: which 0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v)        */
-/* 3 PBECOR(rs,zet,t,uu,vv,ww).  out: 0: exl,exn,vx  1: ec,vcup  2: exl,exn,vx  3: ec,vcup,h,dvcup      */
+/* 3 PBECOR(rs,zet,t,uu,vv,ww).  out: 0: exl,exn,vx  1: ec,vcup,vcdn  2: exl,exn,vx                     */
+/* 3: ec,vcup,vcdn,h,dvcup,dvcdn                                                                      */
 /* The DFT-side LSDA potential of the total spin densities (getvlxc.F -> subvlxc_libxc.F90:290-470 with NSPN = 2
  * and the LDA path: exchange per spin at 2 rho_s with PW91EX(s = u = v = 0), weight ISPFAC * 0.5 * D; PW92
  * correlation PW91LC at the point's zeta; PW91NC vanishes for t = uu = vv = ww = 0), plus the Hartree potential
@@ -1467,11 +1476,84 @@ void oracle_dft_lsda(int n, const double *rup, const double *rdn, const double *
     }
 }
 
+/* The DFT-side PBE potential of the total spin densities: SUBVLXC with NSPN = 2 and IGGA = 1 (subvlxc.F:138-323).
+ * rgu, rgd: 10 x n (value, gradient, Hessian xx,yy,zz,xy,xz,yz) of rho_up and rho_dn.  As there, RHT(:,1) holds the
+ * total density and RHT(:,2) the down density; their values are clipped to [0, total]; |grad|^2, grad.H.grad and the
+ * Laplacian are formed for the total density and for each spin; exchange per spin at D = 2 rho_s with PBEEX(s,u,v) of
+ * 2 rho_s; correlation PBECOR at the point's zeta with t, uu, vv of the total density and
+ * ww = (D(|grad up|^2 - |grad dn|^2) - (up - dn)|grad D|^2) / D * rec2^2.  pot, e as oracle_dft_lsda, with
+ * E_x = EXL + EXN, E_c = ECL + ECN. */
+void oracle_dft_gga(int n, const double *rgu, const double *rgd, const double *w, const double *vh, double *pot,
+                    double *e) {
+    const double THIRD = 0.3333333333333333, THIRD2 = 0.6666666666666667;
+    const double TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627, DKF2RS = 1.9191582926775128;
+    const double EPS = 1.0e-20;
+    for (int q = 0; q < 5; ++q) e[q] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double *u = rgu + 10 * (size_t)k, *d = rgd + 10 * (size_t)k;
+        double R1[10], R2[10];  /* RHT(:,1) total, RHT(:,2) down */
+        for (int i = 0; i < 10; ++i) { R1[i] = u[i] + d[i]; R2[i] = d[i]; }
+        if (R1[0] < 0.0) R1[0] = 0.0;
+        if (R2[0] < 0.0) R2[0] = 0.0;
+        if (R2[0] > R1[0]) R2[0] = R1[0];
+        const double DEN1 = R1[0], DEN3 = R2[0], DEN2 = DEN1 - DEN3;
+        double DG2[3], DGG[3], DLP[3];
+        const double *RS_[3];
+        double RU[10];
+        for (int i = 0; i < 10; ++i) RU[i] = R1[i] - R2[i];
+        RS_[0] = R1; RS_[1] = RU; RS_[2] = R2;
+        for (int j = 0; j < 3; ++j) {
+            const double *r = RS_[j];
+            DG2[j] = r[1] * r[1] + r[2] * r[2] + r[3] * r[3];
+            DGG[j] = r[4] * r[1] * r[1] + r[5] * r[2] * r[2] + r[6] * r[3] * r[3] +
+                     2 * (r[7] * r[1] * r[2] + r[8] * r[1] * r[3] + r[9] * r[2] * r[3]);
+            DLP[j] = r[4] + r[5] + r[6];
+        }
+        double v[2] = {0.0, 0.0};
+        const double wk = w[k];
+        if (DEN1 >= EPS) {
+            for (int sp = 0; sp < 2; ++sp) {  /* exchange */
+                const double D = 2.0 * (sp == 0 ? DEN2 : DEN3);
+                if (!(D > EPS)) continue;
+                const double DKF = pow(TRPI2 * D, THIRD);
+                const double DGRAD = 2.0 * sqrt(DG2[sp + 1]), DLAP = 2.0 * DLP[sp + 1];
+                const double REC2 = 0.5 / (DKF * D), REC1 = REC2 * D, REC22 = REC2 * REC2;
+                const double S = DGRAD * REC2, V = DLAP * REC1 * REC2;
+                const double U = DGRAD > EPS ? 8.0 * DGG[sp + 1] * REC1 * REC22 / DGRAD : 0.0;
+                double EX, DEX, VX;
+                pbeex(D, S, U, V, &EX, &DEX, &VX);
+                v[sp] += VX;
+                e[0] += 0.5 * (EX + DEX) * D * wk;
+            }
+            /* correlation */
+            const double D = DEN1, DREC = 1.0 / D, ZET = (DEN2 - DEN3) * DREC;
+            const double DKF = pow(TRPI2 * D, THIRD), DKS = sqrt(FRDPI * DKF), RS = DKF2RS / DKF;
+            double G = 0.5 * (pow(1.0 + ZET, THIRD2) + pow(1.0 - ZET, THIRD2));
+            if (ZET == 1.0 || ZET == -1.0) G = 0.5 * pow(2.0, THIRD2);
+            const double DGRAD = sqrt(DG2[0]), DLAP = DLP[0];
+            const double REC3 = DREC, REC1 = 0.5 / (DKS * G), REC2 = REC1 * REC3, REC22 = REC2 * REC2;
+            const double T = DGRAD * REC2, VV = DLAP * REC1 * REC2;
+            const double WW = (D * (DG2[1] - DG2[2]) - (DEN2 - DEN3) * DG2[0]) * REC3 * REC22;
+            const double UU = DGRAD > EPS ? DGG[0] * REC1 * REC22 / DGRAD : 0.0;
+            double EC, VCUP, VCDN, DEC, DVCUP, DVCDN;
+            pbecor6(RS, ZET, T, UU, VV, WW, &EC, &VCUP, &VCDN, &DEC, &DVCUP, &DVCDN);
+            e[1] += (EC + DEC) * D * wk;
+            v[0] += VCUP + DVCUP;
+            v[1] += VCDN + DVCDN;
+        }
+        pot[k] = vh[k] + v[0];
+        pot[n + k] = vh[k] + v[1];
+        e[2] += 0.5 * wk * DEN1 * vh[k];
+        e[3] += wk * DEN2;
+        e[4] += wk * DEN3;
+    }
+}
+
 void oracle_xc_kernel(int which, const double *in, double *out) {
     switch (which) {
     case 0: pw91ex(in[0], in[1], in[2], in[3], out, out + 1, out + 2); break;
     case 1: pw91lc3(in[0], in[1], out, out + 1, out + 2); break;
     case 2: pbeex(in[0], in[1], in[2], in[3], out, out + 1, out + 2); break;
-    default: pbecor(in[0], in[1], in[2], in[3], in[4], in[5], out, out + 1, out + 2, out + 3); break;
+    default: pbecor6(in[0], in[1], in[2], in[3], in[4], in[5], out, out + 1, out + 2, out + 3, out + 4, out + 5); break;
     }
 }

@@ -66,6 +66,7 @@ def lib():
         L.oracle_siclag_der.argtypes = [C.POINTER(ObBasis), C.c_int, C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_pbe_edens.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_dft_lsda.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_dft_gga.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_nbas.restype = C.c_int
         L.oracle_nbas.argtypes = [C.POINTER(ObBasis)]
         _lib = L
@@ -150,7 +151,7 @@ def pbe_edens(rho, g) -> tuple:
 def xc_kernel(which: int, args) -> np.ndarray:
     """0 PW91EX(dkf,s,u,v) 1 PW91LC(rs,zet) 2 PBEEX(rho,s,u,v) 3 PBECOR(rs,zet,t,uu,vv,ww)."""
     a = np.ascontiguousarray(args, np.float64)
-    out = np.zeros(4)
+    out = np.zeros(6)
     lib().oracle_xc_kernel(which, _d(a), _d(out))
     return out
 
@@ -163,6 +164,19 @@ def dft_lsda(rho_up, rho_dn, w, vh):
     pot = np.zeros(2 * n)
     e = np.zeros(5)
     lib().oracle_dft_lsda(n, _d(ru), _d(rd), _d(ww), _d(v), _d(pot), _d(e))
+    return pot.reshape(2, n), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), e))
+
+
+def dft_gga(rg_up, rg_dn, w, vh):
+    """DFT-side PBE potential (SUBVLXC with NSPN = 2, IGGA = 1) of spin densities given as (n, 10) arrays (value,
+    gradient, Hessian xx,yy,zz,xy,xz,yz) plus the Hartree potential vh of the total density: pot (2, n) and
+    {E_x, E_c, E_H, N_up, N_dn}."""
+    ru, rd = (np.ascontiguousarray(a, np.float64).reshape(-1, 10) for a in (rg_up, rg_dn))
+    ww, v = (np.ascontiguousarray(a, np.float64) for a in (w, vh))
+    n = ru.shape[0]
+    pot = np.zeros(2 * n)
+    e = np.zeros(5)
+    lib().oracle_dft_gga(n, _d(ru), _d(rd), _d(ww), _d(v), _d(pot), _d(e))
     return pot.reshape(2, n), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), e))
 
 

@@ -1857,7 +1857,6 @@ int32_t flosic_apotnl_sic(flosic_ctx *c, int32_t ispn, double *results, double *
 
 int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
     if (!c) return FLOSIC_EINVAL;
-    if (c->xc != FLOSIC_XC_LSDA) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft: LSDA only");
     if (c->nspn != 2) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft: NSPN = 2 only");
     if (c->block_pts > 0)
         return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft needs the resident FLO fields (mesh-block mode is set)");
@@ -1867,8 +1866,12 @@ int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
     const int nblk = (int)((M + 255) / 256);
     ALLOC(c, c->d_dft, (size_t)2 * M);
     ALLOC(c, c->d_dftpart, (size_t)5 * nblk);
-    launch_dft_lsda(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
-                    c->stream);
+    if (c->xc == FLOSIC_XC_PBE)  // the FLO planes hold value, gradient and Hessian
+        launch_dft_gga(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
+                       c->stream);
+    else
+        launch_dft_lsda(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
+                        c->stream);
     HIPCHK(c, hipGetLastError());
     std::vector<double> part((size_t)5 * nblk);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->d_dftpart.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost,

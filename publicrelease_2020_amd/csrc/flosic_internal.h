@@ -150,6 +150,9 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
 void launch_fill_zero(double *p, size_t n, hipStream_t s);
 void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
                      double *pot, double *partial, hipStream_t s);
+// DFT-side PBE potential (NSPN = 2): phi 10 planes of [nF][M] (value, gradient, Hessian); otherwise as launch_dft_lsda
+void launch_dft_gga(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
+                    double *pot, double *partial, hipStream_t s);
 void launch_gather(const double *src, int64_t stride, int n, double *dst, hipStream_t s);
 
 // out[f][q] (+)= sum over b of partial[f][b][q], compensated, in block order

@@ -1570,6 +1570,185 @@ void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double
                        pot, partial);
 }
 
+// PBE correlation at general zeta (pbecor.f, LGGA = LPOT = 1): EC, VCUP, VCDN, H, DVCUP, DVCDN
+__device__ __forceinline__ void pbecor_gen(double RS, double ZET, double T, double UU, double VV, double WW, double &EC,
+                                           double &VCUP, double &VCDN, double &H, double &DVCUP, double &DVCDN) {
+    const double THRD = 1.0 / 3.0, THRD2 = 2.0 * THRD, THRD4 = 4.0 * THRD, SIXTHM = -THRD / 2.0;
+    const double GAM = 0.5198420997897463295344212145565, FZZ = 8.0 / (9.0 * GAM);
+    const double GAMMA = 0.03109069086965489503494086371273, BET = 0.06672455060314922, DELT = BET / GAMMA;
+    const double ETA = 1.0e-12;
+    const double RTRS = sqrt(RS);
+    double EU, EURS, EP, EPRS, ALFM, ALFRSM;
+    gcor2(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, RTRS, EU, EURS);
+    gcor2(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, RTRS, EP, EPRS);
+    gcor2(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, RTRS, ALFM, ALFRSM);
+    const double Z3 = ZET * ZET * ZET, Z4 = Z3 * ZET;
+    const double F = (pow(1.0 + ZET, THRD4) + pow(1.0 - ZET, THRD4) - 2.0) / GAM;
+    const double ec = EU * (1.0 - F * Z4) + EP * F * Z4 - ALFM * F * (1.0 - Z4) / FZZ;
+    const double ECRS = EURS * (1.0 - F * Z4) + EPRS * F * Z4 - ALFRSM * F * (1.0 - Z4) / FZZ;
+    const double FZ = THRD4 * (pow(1.0 + ZET, THRD) - pow(1.0 - ZET, THRD)) / GAM;
+    const double ECZET = 4.0 * Z3 * F * (EP - EU + ALFM / FZZ) + FZ * (Z4 * EP - Z4 * EU - (1.0 - Z4) * ALFM / FZZ);
+    const double COMM0 = ec - RS * ECRS / 3.0 - ZET * ECZET;
+    EC = ec;
+    VCUP = COMM0 + ECZET;
+    VCDN = COMM0 - ECZET;
+    const double G = (pow(1.0 + ZET, THRD2) + pow(1.0 - ZET, THRD2)) / 2.0;
+    const double G3 = G * G * G;
+    const double PON = -ec / (G3 * GAMMA);
+    const double B = DELT / (exp(PON) - 1.0);
+    const double B2 = B * B, T2 = T * T, T4 = T2 * T2;
+    const double Q4 = 1.0 + B * T2;
+    const double Q5 = 1.0 + B * T2 + B2 * T4;
+    const double h = G3 * (BET / DELT) * log(1.0 + DELT * Q4 * T2 / Q5);
+    const double G4 = G3 * G, T6 = T4 * T2, RSTHRD = RS / 3.0;
+    const double GZ = (pow((1.0 + ZET) * (1.0 + ZET) + ETA, SIXTHM) - pow((1.0 - ZET) * (1.0 - ZET) + ETA, SIXTHM)) / 3.0;
+    const double FAC = DELT / B + 1.0;
+    const double BG = -3.0 * B2 * ec * FAC / (BET * G4);
+    const double BEC = B2 * FAC / (BET * G3);
+    const double Q8 = Q5 * Q5 + DELT * Q4 * Q5 * T2;
+    const double Q9 = 1.0 + 2.0 * B * T2;
+    const double HB = -BET * G3 * B * T6 * (2.0 + B * T2) / Q8;
+    const double HRS = -RSTHRD * HB * BEC * ECRS;
+    const double FACT0 = 2.0 * DELT - 6.0 * B;
+    const double FACT1 = Q5 * Q9 + Q4 * Q9 * Q9;
+    const double HBT = 2.0 * BET * G3 * T4 * ((Q4 * Q5 * FACT0 - DELT * FACT1) / Q8) / Q8;
+    const double HRST = RSTHRD * T2 * HBT * BEC * ECRS;
+    const double HZ = 3.0 * GZ * h / G + HB * (BG * GZ + BEC * ECZET);
+    const double HT = 2.0 * BET * G3 * Q9 / Q8;
+    const double HZT = 3.0 * GZ * HT / G + HBT * (BG * GZ + BEC * ECZET);
+    const double FACT2 = Q4 * Q5 + B * T2 * (Q4 * Q9 + Q5);
+    const double FACT3 = 2.0 * B * Q5 * Q9 + DELT * FACT2;
+    const double HTT = 4.0 * BET * G3 * T * (2.0 * B / Q8 - (Q9 * FACT3 / Q8) / Q8);
+    double comm = h + HRS + HRST + T2 * HT / 6.0 + 7.0 * T2 * T * HTT / 6.0;
+    const double PREF = HZ - GZ * T2 * HT / G;
+    const double FACT5 = GZ * (2.0 * HT + T * HTT) / G;
+    comm = comm - PREF * ZET - UU * HTT - VV * HT - WW * (HZT - FACT5);
+    H = h;
+    DVCUP = comm + PREF;
+    DVCDN = comm - PREF;
+}
+
+// |grad|^2, grad.H.grad and the Laplacian of a 10-component density (value, gradient, xx,yy,zz,xy,xz,yz)
+__device__ __forceinline__ void gga_invariants(const double *r, double &dg2, double &dgg, double &dlp) {
+    dg2 = r[1] * r[1] + r[2] * r[2] + r[3] * r[3];
+    dgg = r[4] * r[1] * r[1] + r[5] * r[2] * r[2] + r[6] * r[3] * r[3] +
+          2 * (r[7] * r[1] * r[2] + r[8] * r[1] * r[3] + r[9] * r[2] * r[3]);
+    dlp = r[4] + r[5] + r[6];
+}
+
+// DFT-side PBE potential of the total spin densities (APOTNL_DFT with NSPN = 2 and IGGA = 1, subvlxc.F:138-323): per
+// point the value, gradient and Hessian of rho_up and rho_dn from the FLO planes (rho = sum phi^2,
+// grad rho = 2 sum phi grad phi, H = 2 sum (grad phi grad phi^T + phi H phi)); the total and down densities clipped
+// as there; exchange per spin at D = 2 rho_s (PBEEX with s, u, v of 2 rho_s), PBE correlation at the point's zeta with
+// t, uu, vv of the total density and the zeta-gradient term ww.  phi: 10 planes of [nF][M]; pot and partial as in
+// k_dft_lsda (E_x = EXL + EXN, E_c = ECL + ECN).
+__global__ void __launch_bounds__(256) k_dft_gga(const double *__restrict__ phi, int64_t M, int nup, int nF,
+                                                 const double *__restrict__ VH, int64_t ldvh,
+                                                 const double *__restrict__ w, double *__restrict__ pot,
+                                                 double *__restrict__ partial) {
+    const double THIRD = 0.3333333333333333, THIRD2 = 0.6666666666666667;
+    const double TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627, DKF2RS = 1.9191582926775128;
+    const double EPS = 1.0e-20;
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double q[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (k < M) {
+        const int64_t plane = (int64_t)nF * M;
+        double ru[10], rd[10], vh = 0.0;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) ru[i] = rd[i] = 0.0;
+        for (int f = 0; f < nF; ++f) {
+            const int64_t off = (int64_t)f * M + k;
+            const double ph = phi[off];
+            const double gx = phi[plane + off], gy = phi[2 * plane + off], gz = phi[3 * plane + off];
+            double *r = f < nup ? ru : rd;
+            r[0] += ph * ph;
+            r[1] += 2 * ph * gx;
+            r[2] += 2 * ph * gy;
+            r[3] += 2 * ph * gz;
+            r[4] += 2 * (gx * gx + phi[4 * plane + off] * ph);
+            r[5] += 2 * (gy * gy + phi[5 * plane + off] * ph);
+            r[6] += 2 * (gz * gz + phi[6 * plane + off] * ph);
+            r[7] += 2 * (gx * gy + phi[7 * plane + off] * ph);
+            r[8] += 2 * (gx * gz + phi[8 * plane + off] * ph);
+            r[9] += 2 * (gy * gz + phi[9 * plane + off] * ph);
+            vh += VH[(int64_t)f * ldvh + k];
+        }
+        // RHT(:,1) = total, RHT(:,2) = down, values clipped to [0, total]; the up components are their difference
+        double R1[10], R2[10], RU[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) {
+            R1[i] = ru[i] + rd[i];
+            R2[i] = rd[i];
+        }
+        if (R1[0] < 0.0) R1[0] = 0.0;
+        if (R2[0] < 0.0) R2[0] = 0.0;
+        if (R2[0] > R1[0]) R2[0] = R1[0];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) RU[i] = R1[i] - R2[i];
+        const double DEN1 = R1[0], DEN3 = R2[0], DEN2 = DEN1 - DEN3;
+        double DG2[3], DGG[3], DLP[3];
+        gga_invariants(R1, DG2[0], DGG[0], DLP[0]);
+        gga_invariants(RU, DG2[1], DGG[1], DLP[1]);
+        gga_invariants(R2, DG2[2], DGG[2], DLP[2]);
+        double v[2] = {0.0, 0.0}, ex = 0.0, ec = 0.0;
+        if (DEN1 >= EPS) {
+            for (int sp = 0; sp < 2; ++sp) {
+                const double D = 2.0 * (sp == 0 ? DEN2 : DEN3);
+                if (!(D > EPS)) continue;
+                const double DKF = pow(TRPI2 * D, THIRD);
+                const double DGRAD = 2.0 * sqrt(DG2[sp + 1]), DLAP = 2.0 * DLP[sp + 1];
+                const double REC2 = 0.5 / (DKF * D), REC1 = REC2 * D, REC22 = REC2 * REC2;
+                const double S = DGRAD * REC2, V = DLAP * REC1 * REC2;
+                const double U = DGRAD > EPS ? 8.0 * DGG[sp + 1] * REC1 * REC22 / DGRAD : 0.0;
+                double EX, DEX, VX;
+                pbeex(D, S, U, V, EX, DEX, VX);
+                v[sp] += VX;
+                ex += 0.5 * (EX + DEX) * D;
+            }
+            const double D = DEN1, DREC = 1.0 / D, ZET = (DEN2 - DEN3) * DREC;
+            const double DKF = pow(TRPI2 * D, THIRD), DKS = sqrt(FRDPI * DKF), RS = DKF2RS / DKF;
+            double G = 0.5 * (pow(1.0 + ZET, THIRD2) + pow(1.0 - ZET, THIRD2));
+            if (ZET == 1.0 || ZET == -1.0) G = 0.5 * pow(2.0, THIRD2);
+            const double DGRAD = sqrt(DG2[0]);
+            const double REC3 = DREC, REC1 = 0.5 / (DKS * G), REC2 = REC1 * REC3, REC22 = REC2 * REC2;
+            const double T = DGRAD * REC2, VV = DLP[0] * REC1 * REC2;
+            const double WW = (D * (DG2[1] - DG2[2]) - (DEN2 - DEN3) * DG2[0]) * REC3 * REC22;
+            const double UU = DGRAD > EPS ? DGG[0] * REC1 * REC22 / DGRAD : 0.0;
+            double EC, VCUP, VCDN, DEC, DVCUP, DVCDN;
+            pbecor_gen(RS, ZET, T, UU, VV, WW, EC, VCUP, VCDN, DEC, DVCUP, DVCDN);
+            ec = (EC + DEC) * D;
+            v[0] += VCUP + DVCUP;
+            v[1] += VCDN + DVCDN;
+        }
+        pot[k] = vh + v[0];
+        pot[M + k] = vh + v[1];
+        const double wk = w[k];
+        q[0] = wk * ex;
+        q[1] = wk * ec;
+        q[2] = 0.5 * wk * DEN1 * vh;
+        q[3] = wk * DEN2;
+        q[4] = wk * DEN3;
+    }
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) red[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[(int64_t)blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_dft_gga(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
+                    double *pot, double *partial, hipStream_t s) {
+    if (M <= 0) return;
+    hipLaunchKernelGGL(k_dft_gga, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, phi, M, nup, nF, VH, ldvh, w,
+                       pot, partial);
+}
+
 __global__ void k_fill_zero(double *__restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;

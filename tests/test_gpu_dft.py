@@ -1,8 +1,8 @@
-"""flosic_apotnl_dft (the DFT-side LSDA potential, APOTNL_DFT with NSPN = 2) against the oracle: spin densities
-from the oracle's FLO fields, the Hartree potential of the total density from the oracle's per-FLO potentials,
-then oracle_dft_lsda (pinned in tests/test_oracle_dft.py).  Also: the spin charges equal the
-sums of the per-FLO charges of the SIC path, and the call is refused where it does not apply (PBE, mesh-block
-mode)."""
+"""flosic_apotnl_dft (the DFT-side potential, APOTNL_DFT with NSPN = 2) against the oracle: spin densities
+(LSDA: values; PBE: value, gradient and Hessian) from the oracle's FLO fields, the Hartree potential of the total
+density from the oracle's per-FLO potentials, then oracle_dft_lsda / oracle_dft_gga (pinned in
+tests/test_oracle_dft.py).  Also: the spin charges equal the sums of the per-FLO charges of the SIC path, and the
+call is refused in mesh-block mode."""
 import numpy as np
 import pytest
 
@@ -38,13 +38,42 @@ def test_apotnl_dft_matches_oracle():
         assert abs(en[key] - sysm.nocc[s - 1]) < 0.05 * sysm.nocc[s - 1]
 
 
-def test_apotnl_dft_refuses_what_it_does_not_cover():
+def _flo_density_fields(sysm, s):
+    """value, gradient and Hessian (n, 10) of the spin-s density sum phi^2 from the oracle's FLOs"""
+    from oracle import oracle as O
+    cf = O.flo_coefs(sysm, s)
+    ph = np.einsum("gkb,bf->gkf", O.basis_eval(sysm.basis, sysm.mesh.xyz, 2), cf)  # (10, npts, n)
+    p, g = ph[0], ph[1:4]
+    rg = np.zeros((p.shape[0], 10))
+    rg[:, 0] = (p * p).sum(1)
+    rg[:, 1:4] = (2 * p[None] * g).sum(2).T
+    for c, (a, b) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        rg[:, 4 + c] = (2 * (g[a] * g[b] + p * ph[4 + c])).sum(1)
+    return rg
+
+
+def test_apotnl_dft_pbe_matches_oracle():
+    from oracle import oracle as O
     from publicrelease_2020_amd import flosic as F
-    sysm = S.make_synthetic(3, 3, 7, S.XC_PBE, nrad=16, nang=26, box=5.0)
+    sysm = S.make_synthetic(3, 3, 7, S.XC_PBE, nrad=24, nang=50, box=5.0)
     ctx = F.Flosic(0)
     ctx.load_system(sysm)
-    with pytest.raises(F.FlosicError):
-        ctx.apotnl_dft()
+    ctx.evaluate()
+    pot, en = ctx.apotnl_dft()
+    ref = O.evaluate(sysm, want_potentials=True, want_vab=False)
+    assert ref["status"] == 0
+    vh = ref["spins"][0]["vh"].sum(0) + ref["spins"][1]["vh"].sum(0)
+    pot_ref, en_ref = O.dft_gga(_flo_density_fields(sysm, 1), _flo_density_fields(sysm, 2), sysm.mesh.w, vh)
+    err = np.abs(pot - pot_ref).max()
+    print(f"PBE: max |dV| {err:.3e}, E_x {en['ex']:.12f} ({en_ref['ex']:.12f}), E_c {en['ec']:.12f} "
+          f"({en_ref['ec']:.12f})", flush=True)
+    assert err < 1e-9 * np.abs(pot_ref).max()
+    for k in ("ex", "ec", "eh", "n_up", "n_dn"):
+        assert abs(en[k] - en_ref[k]) < 1e-10 * abs(en_ref[k]) + 1e-13, k
+
+
+def test_apotnl_dft_refuses_what_it_does_not_cover():
+    from publicrelease_2020_amd import flosic as F
     lsda = S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=16, nang=26, box=5.0)
     ctx2 = F.Flosic(0)
     ctx2.load_system(lsda)
