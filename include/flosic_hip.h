@@ -99,8 +99,8 @@ int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double
  * correlation (PW91LC) at the point's zeta.  PBE: exchange per spin at 2 rho_s (PBEEX with the s, u, v of 2 rho_s)
  * and PBECOR at the point's zeta with t, uu, vv of the total density and the zeta-gradient term ww.  pot
  * (nmsh x 2, may be NULL): V_H + V_xc of spin up, then spin down (the POTOUT layout); energies (may be NULL):
- * {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.  NSPN = 2 and resident fields (no mesh-block mode)
- * only; otherwise FLOSIC_EINVAL. */
+ * {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.  NSPN = 2 only (otherwise FLOSIC_EINVAL); in
+ * mesh-block mode the FLO fields are rebuilt block by block. */
 int32_t flosic_apotnl_dft(flosic_ctx *ctx, double *pot, double *energies);
 
 /* SICLAG_DER (siclag_der.F:2-181, SICLM_DERSLV siclag_der.F:186-640, without point-group symmetry and without the

@@ -1858,21 +1858,32 @@ int32_t flosic_apotnl_sic(flosic_ctx *c, int32_t ispn, double *results, double *
 int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
     if (!c) return FLOSIC_EINVAL;
     if (c->nspn != 2) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft: NSPN = 2 only");
-    if (c->block_pts > 0)
-        return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft needs the resident FLO fields (mesh-block mode is set)");
-    const int r = need_eval(c);
+    int r = need_eval(c);
     if (r) return r;
     const int64_t M = c->M;
     const int nblk = (int)((M + 255) / 256);
     ALLOC(c, c->d_dft, (size_t)2 * M);
     ALLOC(c, c->d_dftpart, (size_t)5 * nblk);
-    if (c->xc == FLOSIC_XC_PBE)  // the FLO planes hold value, gradient and Hessian
-        launch_dft_gga(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
-                       c->stream);
-    else
-        launch_dft_lsda(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, c->d_dft.p, c->d_dftpart.p,
-                        c->stream);
-    HIPCHK(c, hipGetLastError());
+    // resident FLO fields: one pass; mesh-block mode: the planes and fields of each block are built again (blocks are
+    // multiples of 256 points, so the 256-point partials are those of the resident pass)
+    const bool blocked = c->block_pts > 0;
+    const int64_t mb = mesh_block(c);
+    const int nd = xc_planes(c), nF = c->nF;  // PBE: the FLO planes hold value, gradient and Hessian
+    for (int64_t k0 = 0; k0 < M; k0 += mb) {
+        const int64_t m = std::min(mb, M - k0);
+        if (blocked) {
+            if ((r = block_planes(c, k0, m, nd, c->d_planes.p))) return r;
+            if ((r = orbitals_on_points(c, c->d_planes.p, m, nd, c->d_cflo.p, nF, c->d_phi.p))) return r;
+        }
+        double *part = c->d_dftpart.p + (size_t)5 * (k0 / 256);
+        if (c->xc == FLOSIC_XC_PBE)
+            launch_dft_gga(c->d_phi.p, m, c->sp[0].nocc, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_dft.p + k0, M, part,
+                           c->stream);
+        else
+            launch_dft_lsda(c->d_phi.p, m, c->sp[0].nocc, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_dft.p + k0, M,
+                            part, c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
     std::vector<double> part((size_t)5 * nblk);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->d_dftpart.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
                              c->stream));

@@ -148,11 +148,13 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
 
 // p[0..n) = 0 (a kernel: usable inside captured graphs); dst[i] = src[i * stride], i < n
 void launch_fill_zero(double *p, size_t n, hipStream_t s);
+// DFT-side LSDA potential (NSPN = 2) over M points: phi values [nF][M] (FLOs 0..nup-1 spin up), VH[f * ldvh + k],
+// w[k]; pot[k] spin up, pot[ldpot + k] spin down; partial[b * 5 + q] per 256-point block b of these M points
 void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
-                     double *pot, double *partial, hipStream_t s);
+                     double *pot, int64_t ldpot, double *partial, hipStream_t s);
 // DFT-side PBE potential (NSPN = 2): phi 10 planes of [nF][M] (value, gradient, Hessian); otherwise as launch_dft_lsda
 void launch_dft_gga(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
-                    double *pot, double *partial, hipStream_t s);
+                    double *pot, int64_t ldpot, double *partial, hipStream_t s);
 void launch_gather(const double *src, int64_t stride, int n, double *dst, hipStream_t s);
 
 // out[f][q] (+)= sum over b of partial[f][b][q], compensated, in block order

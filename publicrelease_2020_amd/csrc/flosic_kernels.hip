@@ -1505,11 +1505,11 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
 // subvlxc_libxc.F90:290-470 on the LDA path): rho_s = sum of the FLO densities of spin s (the FLOs of a spin are an
 // orthonormal rotation of its occupied orbitals), V_H = sum of the FLO Hartree potentials (the potential of the total
 // density), exchange per spin at 2 rho_s (PW91EX with s = u = v = 0), PW92 correlation at the point's zeta.
-// pot[0, M) spin up, pot[M, 2M) spin down = V_H + V_xc,s; partial[block * 5 + q] = block sums of
+// pot[0, M) spin up, pot[ldpot, ldpot + M) spin down = V_H + V_xc,s; partial[block * 5 + q] = block sums of
 // {E_x, E_c, E_H = 1/2 w rho V_H, N_up, N_dn} in a fixed order.
 __global__ void __launch_bounds__(256) k_dft_lsda(const double *__restrict__ phi, int64_t M, int nup, int nF,
                                                   const double *__restrict__ VH, int64_t ldvh,
-                                                  const double *__restrict__ w, double *__restrict__ pot,
+                                                  const double *__restrict__ w, double *__restrict__ pot, int64_t ldpot,
                                                   double *__restrict__ partial) {
     const double THIRD = 0.3333333333333333, TRPI2 = 29.6088132032680759, DKF2RS = 1.9191582926775128;
     const double EPS = 1.0e-20;
@@ -1542,7 +1542,7 @@ __global__ void __launch_bounds__(256) k_dft_lsda(const double *__restrict__ phi
             ec = EC * D;
         }
         pot[k] = vh + vxu + vcu;
-        pot[M + k] = vh + vxd + vcd;
+        pot[ldpot + k] = vh + vxd + vcd;
         const double wk = w[k];
         q[0] = wk * ex;
         q[1] = wk * ec;
@@ -1564,10 +1564,10 @@ __global__ void __launch_bounds__(256) k_dft_lsda(const double *__restrict__ phi
 }
 
 void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
-                     double *pot, double *partial, hipStream_t s) {
+                     double *pot, int64_t ldpot, double *partial, hipStream_t s) {
     if (M <= 0) return;
     hipLaunchKernelGGL(k_dft_lsda, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, phi, M, nup, nF, VH, ldvh, w,
-                       pot, partial);
+                       pot, ldpot, partial);
 }
 
 // PBE correlation at general zeta (pbecor.f, LGGA = LPOT = 1): EC, VCUP, VCDN, H, DVCUP, DVCDN
@@ -1644,7 +1644,7 @@ __device__ __forceinline__ void gga_invariants(const double *r, double &dg2, dou
 // k_dft_lsda (E_x = EXL + EXN, E_c = ECL + ECN).
 __global__ void __launch_bounds__(256) k_dft_gga(const double *__restrict__ phi, int64_t M, int nup, int nF,
                                                  const double *__restrict__ VH, int64_t ldvh,
-                                                 const double *__restrict__ w, double *__restrict__ pot,
+                                                 const double *__restrict__ w, double *__restrict__ pot, int64_t ldpot,
                                                  double *__restrict__ partial) {
     const double THIRD = 0.3333333333333333, THIRD2 = 0.6666666666666667;
     const double TRPI2 = 29.6088132032680759, FRDPI = 1.2732395447351627, DKF2RS = 1.9191582926775128;
@@ -1721,7 +1721,7 @@ __global__ void __launch_bounds__(256) k_dft_gga(const double *__restrict__ phi,
             v[1] += VCDN + DVCDN;
         }
         pot[k] = vh + v[0];
-        pot[M + k] = vh + v[1];
+        pot[ldpot + k] = vh + v[1];
         const double wk = w[k];
         q[0] = wk * ex;
         q[1] = wk * ec;
@@ -1743,10 +1743,10 @@ __global__ void __launch_bounds__(256) k_dft_gga(const double *__restrict__ phi,
 }
 
 void launch_dft_gga(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,
-                    double *pot, double *partial, hipStream_t s) {
+                    double *pot, int64_t ldpot, double *partial, hipStream_t s) {
     if (M <= 0) return;
     hipLaunchKernelGGL(k_dft_gga, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, phi, M, nup, nF, VH, ldvh, w,
-                       pot, partial);
+                       pot, ldpot, partial);
 }
 
 __global__ void k_fill_zero(double *__restrict__ p, int64_t n) {

@@ -1,8 +1,8 @@
 """flosic_apotnl_dft (the DFT-side potential, APOTNL_DFT with NSPN = 2) against the oracle: spin densities
 (LSDA: values; PBE: value, gradient and Hessian) from the oracle's FLO fields, the Hartree potential of the total
 density from the oracle's per-FLO potentials, then oracle_dft_lsda / oracle_dft_gga (pinned in
-tests/test_oracle_dft.py).  Also: the spin charges equal the sums of the per-FLO charges of the SIC path, and the
-call is refused in mesh-block mode."""
+tests/test_oracle_dft.py).  Also: the spin charges equal the sums of the per-FLO charges of the SIC path, mesh-block
+mode gives the resident result, and the call is refused with one spin channel (NSPN = 1)."""
 import numpy as np
 import pytest
 
@@ -72,11 +72,32 @@ def test_apotnl_dft_pbe_matches_oracle():
         assert abs(en[k] - en_ref[k]) < 1e-10 * abs(en_ref[k]) + 1e-13, k
 
 
-def test_apotnl_dft_refuses_what_it_does_not_cover():
+@pytest.mark.parametrize("xc", [S.XC_LSDA, S.XC_PBE])
+def test_apotnl_dft_mesh_block_mode_matches_resident(xc):
+    """mesh-block mode rebuilds the FLO fields per block; the 256-point partials are those of the resident pass"""
     from publicrelease_2020_amd import flosic as F
-    lsda = S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=16, nang=26, box=5.0)
-    ctx2 = F.Flosic(0)
-    ctx2.load_system(lsda)
-    ctx2.set_mesh_block(512)
+    sysm = S.make_synthetic(3, 3, 7, xc, nrad=24, nang=50, box=5.0)
+    out = []
+    for block in (0, 512):
+        ctx = F.Flosic(0)
+        ctx.load_system(sysm)
+        if block:
+            ctx.set_mesh_block(block)
+        ctx.evaluate()
+        out.append(ctx.apotnl_dft())
+    (p0, e0), (p1, e1) = out
+    err = np.abs(p1 - p0).max()
+    print(f"xc {xc}: block vs resident max |dV| {err:.3e}", flush=True)
+    assert err < 1e-12 * np.abs(p0).max()
+    for k in e0:
+        assert abs(e1[k] - e0[k]) < 1e-12 * abs(e0[k]) + 1e-14, k
+
+
+def test_apotnl_dft_refuses_one_spin_channel():
+    import dataclasses
+    from publicrelease_2020_amd import flosic as F
+    sysm = S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=16, nang=26, box=5.0)
+    ctx = F.Flosic(0)
+    ctx.load_system(dataclasses.replace(sysm, nspn=1))
     with pytest.raises(F.FlosicError):
-        ctx2.apotnl_dft()
+        ctx.apotnl_dft()
