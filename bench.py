@@ -113,7 +113,35 @@ def phase_roofline(sysm, counts, phases_ms, prec_name: str) -> dict:
     return out
 
 
-def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 0) -> dict:
+def next_rows(ctx, sysm, counts, reps: int = 3) -> dict:
+    """Wall time of the DFT-side potential (APOTNL_DFT) and of SICLAG_DER on an evaluated context (both calls end
+    with their results on the host), with the bound each is priced against: the DFT kernel streams the FLO values
+    and Hartree potentials once (M (16 nF + 24) bytes, HBM); SICLAG_DER is three FP64 products over the points,
+    6 N nF M flops (the basis gradient planes it rebuilds per block are recomputation, not counted), FP64 MFMA."""
+    M, nF, N = int(counts[2]), int(counts[4]), int(sysm.nbas)
+    out = {}
+    for name, call in (("apotnl_dft", ctx.apotnl_dft), ("siclag_der", ctx.siclag_der)):
+        call()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        out[name] = {"ms": 1000.0 * (time.perf_counter() - t0) / reps}
+    d = out["apotnl_dft"]
+    d["algorithmic_bytes"] = float(M * (16 * nF + 24))
+    d["bound"] = "hbm"
+    d["achieved_GBps"] = d["algorithmic_bytes"] / (d["ms"] * 1e-3) / 1e9
+    d["frac"] = d["achieved_GBps"] / 8000.0
+    d = out["siclag_der"]
+    d["flops"] = float(6 * N * nF * M)
+    d["bound"] = "mfma"
+    d["achieved_TFLOPs"] = d["flops"] / (d["ms"] * 1e-3) / 1e12
+    d["frac"] = d["achieved_TFLOPs"] / 78.6
+    log(f"next rows: apotnl_dft {out['apotnl_dft']['ms']:.2f} ms ({out['apotnl_dft']['achieved_GBps']:.0f} GB/s), "
+        f"siclag_der {out['siclag_der']['ms']:.2f} ms ({out['siclag_der']['achieved_TFLOPs']:.1f} TFLOP/s)")
+    return out
+
+
+def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 0, measure_next: bool = False) -> dict:
     import numpy as np
 
     from publicrelease_2020_amd import flosic as F
@@ -143,6 +171,8 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 
                    counts=ctx.counts())
         log(f"{prec_name}: {out['ms_per_step']:.1f} ms/eval, Coulomb kernel {out['kernel_ms']:.1f} ms "
             f"({ctx.coulomb_cols()} FLO columns per block)")
+        if measure_next:
+            out["next_rows"] = next_rows(ctx, sysm, out["counts"])
         return out
     finally:
         ctx.close()
@@ -408,7 +438,7 @@ def main() -> int:
     log(f"workload K={sysm.meta['K']} nbas={sysm.nbas} npts={sysm.mesh.npts} nocc={sysm.nocc} "
         f"xc={'PBE' if sysm.xc else 'LSDA'} (generated in {time.perf_counter() - t0:.1f} s)")
 
-    ref = run_form(sysm, "fp64", args.steps, args.warmup, args.coulomb_cols)
+    ref = run_form(sysm, "fp64", args.steps, args.warmup, args.coulomb_cols, measure_next=True)
     forms = [ref]
     for name in [f for f in args.forms.split(",") if f]:
         r = run_form(sysm, name, args.steps, args.warmup, args.coulomb_cols)
@@ -471,6 +501,7 @@ def main() -> int:
              "coulomb_kernel"], best["phases"])},
         "phase_roofline": phase_roofline(sysm, best["counts"], best["phases"], best["prec"]),
         "esic": best["esic"],
+        "next_rows": ref["next_rows"],
     }
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

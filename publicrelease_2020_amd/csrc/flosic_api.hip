@@ -902,21 +902,23 @@ int run_frmorb(flosic_ctx *c) {
 // point dimension is split into slices (the products are skinny with K = M) and the slice partials are summed
 // in a fixed order, onto out when accumulate is set
 int gemm_tn_points(flosic_ctx *c, int rows, int n, const double *A, const double *B, int64_t M, double *out,
-                   bool accumulate = false) {
+                   bool accumulate = false, int64_t lda = 0, int64_t ldb = 0) {
     if (rows <= 0 || n <= 0) return 0;
+    if (lda <= 0) lda = M;
+    if (ldb <= 0) ldb = M;
     const double one = 1.0, zero = 0.0;
     int nsl = 64;
     while (nsl > 1 && M / nsl < 4096) --nsl;
     const int64_t ms = M / nsl, rest = M - ms * nsl;  // nsl slices of ms points, then the remainder
     ALLOC(c, c->d_Gpart, (size_t)(nsl + 1) * rows * n);
     BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, rows, n,
-                                             (rocblas_int)ms, &one, A, (rocblas_int)M, ms, B, (rocblas_int)M, ms,
+                                             (rocblas_int)ms, &one, A, (rocblas_int)lda, ms, B, (rocblas_int)ldb, ms,
                                              &zero, c->d_Gpart.p, rows, (rocblas_stride)rows * n, nsl));
     int nparts = nsl;
     if (rest > 0) {
         BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, rows, n,
-                                 (rocblas_int)rest, &one, A + ms * nsl, (rocblas_int)M, B + ms * nsl, (rocblas_int)M,
-                                 &zero, c->d_Gpart.p + (size_t)nsl * rows * n, rows));
+                                 (rocblas_int)rest, &one, A + ms * nsl, (rocblas_int)lda, B + ms * nsl,
+                                 (rocblas_int)ldb, &zero, c->d_Gpart.p + (size_t)nsl * rows * n, rows));
         nparts = nsl + 1;
     }
     launch_reduce_partials(c->d_Gpart.p, 1, nparts, rows * n, out, c->stream, accumulate);
@@ -1898,10 +1900,11 @@ int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
     return FLOSIC_OK;
 }
 
-// SICLAG_DER: G_d = (d_d chi)^T U over the mesh for d = x, y, z with U = w V_i phi_i, walked in blocks whose
-// basis planes (value and gradient; PBE: all ten), FLO fields and U are rebuilt and dropped (the energy partials of
-// that recomputation land in the evaluation's partial buffer, which is scratch once the evaluation has finished);
-// then DERSIC(x, i, A) = -sum over atom A's functions s of c_si G_x(s, i), on the host.
+// SICLAG_DER: G_d = (d_d chi)^T U over the mesh for d = x, y, z with U = w V_i phi_i, then
+// DERSIC(x, i, A) = -sum over atom A's functions s of c_si G_x(s, i), on the host.  With resident fields U is the
+// evaluation's; PBE keeps the gradient planes resident too (one pass over the mesh), LSDA rebuilds them per block of
+// points.  In mesh-block mode the planes, FLO fields and U of each block are rebuilt and dropped (the energy
+// partials of that recomputation land in the evaluation's partial buffer, scratch once the evaluation has finished).
 constexpr int64_t kDerBlockPts = 32768;  // points per block when the planes are resident (a multiple of 256)
 
 int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
@@ -1910,24 +1913,40 @@ int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
     if (r) return r;
     const int N = c->nbas, nF = c->nF, nd = xc_planes(c), ng = std::max(nd, 4);
     const int64_t M = c->M;
-    const int64_t mb = c->block_pts > 0 ? mesh_block(c) : std::min<int64_t>(M, kDerBlockPts);
-    const int nblk = (int)((M + 255) / 256);
-    ALLOC(c, c->d_dplanes, (size_t)ng * N * mb);
-    ALLOC(c, c->d_dphi, (size_t)nd * nF * mb);
-    ALLOC(c, c->d_dvt, (size_t)nF * mb);
-    ALLOC(c, c->d_du, (size_t)nF * mb);
+    const bool blocked = c->block_pts > 0;
     ALLOC(c, c->d_dG, (size_t)3 * N * nF);
-    for (int64_t k0 = 0; k0 < M; k0 += mb) {
-        const int64_t m = std::min(mb, M - k0);
-        if ((r = block_planes(c, k0, m, ng, c->d_dplanes.p))) return r;
-        if ((r = orbitals_on_points(c, c->d_dplanes.p, m, nd, c->d_cflo.p, nF, c->d_dphi.p))) return r;
-        launch_xc(c->xc, nd, c->d_dphi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_dvt.p, c->d_du.p, c->d_part.p,
-                  nblk, (int)(k0 / 256), c->stream);
-        HIPCHK(c, hipGetLastError());
+    if (!blocked && nd == 10) {  // resident PBE planes: value, gradient, Hessian of every function on the mesh
         for (int d = 0; d < 3; ++d)
-            if ((r = gemm_tn_points(c, N, nF, c->d_dplanes.p + (size_t)(1 + d) * N * m, c->d_du.p, m,
-                                    c->d_dG.p + (size_t)d * N * nF, k0 > 0)))
+            if ((r = gemm_tn_points(c, N, nF, c->d_planes.p + (size_t)(1 + d) * N * M, c->d_U.p, M,
+                                    c->d_dG.p + (size_t)d * N * nF)))
                 return r;
+    } else {
+        const int64_t mb = blocked ? mesh_block(c) : std::min<int64_t>(M, kDerBlockPts);
+        const int nblk = (int)((M + 255) / 256);
+        ALLOC(c, c->d_dplanes, (size_t)ng * N * mb);
+        if (blocked) {
+            ALLOC(c, c->d_dphi, (size_t)nd * nF * mb);
+            ALLOC(c, c->d_dvt, (size_t)nF * mb);
+            ALLOC(c, c->d_du, (size_t)nF * mb);
+        }
+        for (int64_t k0 = 0; k0 < M; k0 += mb) {
+            const int64_t m = std::min(mb, M - k0);
+            if ((r = block_planes(c, k0, m, ng, c->d_dplanes.p))) return r;
+            const double *u = c->d_U.p + k0;  // resident U: [nF][M]
+            int64_t ldu = M;
+            if (blocked) {
+                if ((r = orbitals_on_points(c, c->d_dplanes.p, m, nd, c->d_cflo.p, nF, c->d_dphi.p))) return r;
+                launch_xc(c->xc, nd, c->d_dphi.p, m, nF, c->d_VH.p + k0, M, c->d_w.p + k0, c->d_dvt.p, c->d_du.p,
+                          c->d_part.p, nblk, (int)(k0 / 256), c->stream);
+                HIPCHK(c, hipGetLastError());
+                u = c->d_du.p;
+                ldu = m;
+            }
+            for (int d = 0; d < 3; ++d)
+                if ((r = gemm_tn_points(c, N, nF, c->d_dplanes.p + (size_t)(1 + d) * N * m, u, m,
+                                        c->d_dG.p + (size_t)d * N * nF, k0 > 0, m, ldu)))
+                    return r;
+        }
     }
     std::vector<double> G((size_t)3 * N * nF), cf((size_t)N * nF);
     HIPCHK(c, hipMemcpyAsync(G.data(), c->d_dG.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
