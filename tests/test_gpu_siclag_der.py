@@ -31,3 +31,23 @@ def test_siclag_der_matches_oracle(xc, block):
     assert err < 1e-10 * scale
     # a second call on the same evaluation returns the same values (the recomputation is deterministic)
     np.testing.assert_array_equal(ctx.siclag_der(), d)
+
+
+def test_siclag_der_spin_restricted():
+    """NSPN = 1: one spin channel of FLOs, the result doubled (siclag_der.F:158-166)"""
+    import dataclasses
+
+    from oracle import oracle as O
+    from publicrelease_2020_amd import flosic as F
+    sysm = dataclasses.replace(S.make_synthetic(3, 3, 7, S.XC_LSDA, nrad=24, nang=50, box=5.0), nspn=1)
+    ctx = F.Flosic(0)
+    ctx.load_system(sysm)
+    ctx.evaluate()
+    d = ctx.siclag_der()
+    ref = O.evaluate(sysm, want_potentials=True, want_vab=False)
+    assert ref["status"] == 0
+    d_ref = O.siclag_der(sysm, ref["spins"][0]["vpot"], O.flo_coefs(sysm, 1))
+    err = np.abs(d - d_ref).max()
+    print(f"NSPN = 1: max |dDERSIC| {err:.3e} of {np.abs(d_ref).max():.3e}", flush=True)
+    assert d.shape == d_ref.shape
+    assert err < 1e-10 * np.abs(d_ref).max()
