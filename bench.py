@@ -115,10 +115,12 @@ def phase_roofline(sysm, counts, phases_ms, prec_name: str) -> dict:
 
 def next_rows(ctx, sysm, counts, reps: int = 3) -> dict:
     """Wall time of the DFT-side potential (APOTNL_DFT) and of SICLAG_DER on an evaluated context (both calls end
-    with their results on the host), with the bound each is priced against: the DFT kernel streams the FLO values
-    and Hartree potentials once (M (16 nF + 24) bytes, HBM); SICLAG_DER is three FP64 products over the points,
-    6 N nF M flops (the basis gradient planes it rebuilds per block are recomputation, not counted), FP64 MFMA."""
+    with their results on the host), with the bound each is priced against: the DFT kernel streams the FLO fields
+    (nd planes: 1 for LSDA, 10 for PBE) and Hartree potentials once, M (8 nd nF + 8 nF + 24) bytes, HBM; SICLAG_DER
+    is three FP64 products over the points, 6 N nF M flops, FP64 MFMA.  Planes and fields rebuilt in mesh-block
+    mode are recomputation and not counted."""
     M, nF, N = int(counts[2]), int(counts[4]), int(sysm.nbas)
+    nd = 10 if sysm.xc else 1
     out = {}
     for name, call in (("apotnl_dft", ctx.apotnl_dft), ("siclag_der", ctx.siclag_der)):
         call()
@@ -127,7 +129,7 @@ def next_rows(ctx, sysm, counts, reps: int = 3) -> dict:
             call()
         out[name] = {"ms": 1000.0 * (time.perf_counter() - t0) / reps}
     d = out["apotnl_dft"]
-    d["algorithmic_bytes"] = float(M * (16 * nF + 24))
+    d["algorithmic_bytes"] = float(M * (8 * nd * nF + 8 * nF + 24))
     d["bound"] = "hbm"
     d["achieved_GBps"] = d["algorithmic_bytes"] / (d["ms"] * 1e-3) / 1e9
     d["frac"] = d["achieved_GBps"] / 8000.0
@@ -352,6 +354,7 @@ def bench_adjoint(args) -> int:
         modes[name] = dict(ms_per_eval=dt * 1e3, peak_device_bytes=e.peak_device_bytes(), esic=z, grad=g,
                            debdax=np.concatenate([e.fod_forces(1), e.fod_forces(2)]), engine=e)
         log(f"{name}: {dt * 1e3:.1f} ms/eval, peak device memory {e.peak_device_bytes() / 2**30:.2f} GiB")
+        modes[name]["next_rows"] = next_rows(e, sysm, e.counts())
     e = modes["mesh_block"]["engine"]
     ref = modes["resident"]
     h = 1e-4
@@ -390,6 +393,7 @@ def bench_adjoint(args) -> int:
         "fd_h_bohr": h, "fd_coordinates": coords, "max_fd_error_adjoint": float(max(fd_err)),
         "max_fd_error_frmorb2_formula": float(max(deb_err)),
         "mixed_dE": abs(zm - ref["esic"]), "mixed_max_dgrad": float(np.abs(gm - ref["grad"]).max()),
+        "next_rows": {k: modes[k]["next_rows"] for k in ("resident", "mesh_block")},
     }
     print(json.dumps(out), flush=True)
     return 0
