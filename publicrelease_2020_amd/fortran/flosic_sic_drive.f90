@@ -1,6 +1,7 @@
 ! flosic_sic_drive INFILE OUTFILE: reads one system (basis, mesh, options, KS orbitals and FODs of both
 ! spins) from an unformatted stream file, runs flosic_sic_evaluate on HIP device 0 and writes E_SIC,
-! RESULTS, SIC and DEBDAX of both spins.  Exit status 0 on success, 10 + library status otherwise.
+! RESULTS, SIC and DEBDAX of both spins, then the force step's DERSIC (SICLAG_DER) and, for NSPN = 2, the DFT-side
+! POTOUT(NMSH,2) and its energies (APOTNL_DFT).  Exit status 0 on success, 10 + library status otherwise.
 program flosic_sic_drive
   use iso_c_binding
   use flosic_hip
@@ -10,7 +11,8 @@ program flosic_sic_drive
   integer(c_int32_t) :: natom, nbare, ncoef, nspn, xc, prec, nbas, nwf(2), nocc(2), status, s
   integer(c_int64_t) :: nmsh
   real(c_double), allocatable :: xyz(:), alpha(:), con(:), rmsh(:), wmsh(:), c1(:, :), c2(:, :), f1(:, :), f2(:, :)
-  real(c_double), allocatable :: r1(:, :), r2(:, :), s1(:, :), s2(:, :), d1(:, :), d2(:, :)
+  real(c_double), allocatable :: r1(:, :), r2(:, :), s1(:, :), s2(:, :), d1(:, :), d2(:, :), dersic(:), potdft(:)
+  real(c_double) :: edft(5)
   integer(c_int32_t), allocatable :: bare_off(:), ncon(:), con_off(:)
   real(c_double) :: zsic
   type(c_ptr) :: ctx
@@ -33,8 +35,16 @@ program flosic_sic_drive
   if (status /= FLOSIC_OK) call fail(status)
   call flosic_sic_evaluate(ctx, nbas, nwf, nocc, c1, c2, f1, f2, r1, r2, s1, s2, d1, d2, zsic, status)
   if (status /= FLOSIC_OK) call fail(status)
+  allocate(dersic(3 * (nocc(1) + nocc(2)) * natom), potdft(2 * nmsh))
+  status = flosic_siclag_der(ctx, dersic)
+  if (status /= FLOSIC_OK) call fail(status)
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(11) zsic, r1, r2, s1, s2, d1, d2
+  write(11) zsic, r1, r2, s1, s2, d1, d2, dersic
+  if (nspn == 2) then
+    status = flosic_apotnl_dft(ctx, potdft, edft)
+    if (status /= FLOSIC_OK) call fail(status)
+    write(11) potdft, edft
+  end if
   close(11)
   s = flosic_destroy(ctx)
 contains

@@ -43,7 +43,7 @@ def test_fortran_driver_built():
 def test_fortran_host_matches_oracle():
     from oracle import oracle as O
     sysm = S.make_synthetic(3, 3, 7, nrad=24, nang=50, box=5.0)
-    ref = O.evaluate(sysm)
+    ref = O.evaluate(sysm, want_potentials=True)
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_input(fin, sysm)
@@ -67,3 +67,17 @@ def test_fortran_host_matches_oracle():
         assert np.abs(res[s] - ref["spins"][s]["results"]).max() < 1e-10 * np.abs(ref["spins"][s]["results"]).max() + 1e-12
         assert np.abs(sic[s] - ref["spins"][s]["sic"]).max() < 1e-10 * np.abs(ref["spins"][s]["sic"]).max() + 1e-12
         assert np.abs(deb[s] - ref["spins"][s]["debdax"]).max() < 1e-9
+    # force step: DERSIC(3, nF, natom), then POTOUT(NMSH, 2) and {E_x, E_c, E_H, N_up, N_dn}
+    nF, natom, M = n[0] + n[1], sysm.basis.natom, sysm.mesh.npts
+    dersic = raw[o:o + 3 * nF * natom].reshape(natom, nF, 3); o += 3 * nF * natom
+    d_ref = O.siclag_der(sysm, np.vstack([ref["spins"][0]["vpot"], ref["spins"][1]["vpot"]]))
+    assert np.abs(dersic - d_ref).max() < 1e-10 * np.abs(d_ref).max()
+    pot = raw[o:o + 2 * M].reshape(2, M); o += 2 * M
+    edft = raw[o:o + 5]; o += 5
+    assert o == raw.size
+    vh = ref["spins"][0]["vh"].sum(0) + ref["spins"][1]["vh"].sum(0)
+    rho = [(O.flo_fields(sysm, s) ** 2).sum(0) for s in (1, 2)]
+    pot_ref, e_ref = O.dft_lsda(rho[0], rho[1], sysm.mesh.w, vh)
+    assert np.abs(pot - pot_ref).max() < 1e-10 * np.abs(pot_ref).max()
+    for q, k in enumerate(("ex", "ec", "eh", "n_up", "n_dn")):
+        assert abs(edft[q] - e_ref[k]) < 1e-10 * abs(e_ref[k])
