@@ -858,7 +858,7 @@ int ensure_planes(flosic_ctx *c) {
 
 // Basis planes of the mesh block [k0, k0 + m) into dst (leading dimension m); nd as launch_basis_planes
 int block_planes(flosic_ctx *c, int64_t k0, int64_t m, int nd, double *dst) {
-    const size_t cnt = (size_t)(nd < 0 ? 1 : nd) * c->nbas * (size_t)m;
+    const size_t cnt = (size_t)(nd == -1 ? 1 : nd == -3 ? 3 : nd) * c->nbas * (size_t)m;
     launch_fill_zero(dst, cnt, c->stream);  // a kernel, not a memset node, so that the block loop can be captured
     launch_basis_planes<double>(basis_view(c), c->d_x.p + k0, c->d_y.p + k0, c->d_z.p + k0, m, nd, dst, m, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -1923,7 +1923,9 @@ int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
     } else {
         const int64_t mb = blocked ? mesh_block(c) : std::min<int64_t>(M, kDerBlockPts);
         const int nblk = (int)((M + 255) / 256);
-        ALLOC(c, c->d_dplanes, (size_t)ng * N * mb);
+        // mesh-block mode needs the value planes too (the FLO fields are rebuilt); resident LSDA only the gradients
+        const int pk = blocked ? ng : -3, np = blocked ? ng : 3, g0 = blocked ? 1 : 0;
+        ALLOC(c, c->d_dplanes, (size_t)np * N * mb);
         if (blocked) {
             ALLOC(c, c->d_dphi, (size_t)nd * nF * mb);
             ALLOC(c, c->d_dvt, (size_t)nF * mb);
@@ -1931,7 +1933,7 @@ int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
         }
         for (int64_t k0 = 0; k0 < M; k0 += mb) {
             const int64_t m = std::min(mb, M - k0);
-            if ((r = block_planes(c, k0, m, ng, c->d_dplanes.p))) return r;
+            if ((r = block_planes(c, k0, m, pk, c->d_dplanes.p))) return r;
             const double *u = c->d_U.p + k0;  // resident U: [nF][M]
             int64_t ldu = M;
             if (blocked) {
@@ -1943,7 +1945,7 @@ int32_t flosic_siclag_der(flosic_ctx *c, double *dersic) {
                 ldu = m;
             }
             for (int d = 0; d < 3; ++d)
-                if ((r = gemm_tn_points(c, N, nF, c->d_dplanes.p + (size_t)(1 + d) * N * m, u, m,
+                if ((r = gemm_tn_points(c, N, nF, c->d_dplanes.p + (size_t)(g0 + d) * N * m, u, m,
                                         c->d_dG.p + (size_t)d * N * nF, k0 > 0, m, ldu)))
                     return r;
         }

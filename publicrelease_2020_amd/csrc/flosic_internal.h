@@ -80,7 +80,8 @@ int coulomb_cols(int nF);
 constexpr int kBoysTableDoubles = kBoysRows * kBoysCols * 3 / 2;
 void boys_table(double *h_tab);
 
-// nd = 1 (values), 4 (+ gradient), 10 (+ Hessian xx,yy,zz,xy,xz,yz) planes, or -1 (one Laplacian plane)
+// nd = 1 (values), 4 (+ gradient), 10 (+ Hessian xx,yy,zz,xy,xz,yz) planes, -1 (one Laplacian plane) or -3 (the
+// three gradient planes alone)
 template <typename T>
 void launch_basis_planes(const BasisDev &B, const double *x, const double *y, const double *z, int64_t M, int nd,
                          T *phi, int64_t ldphi, hipStream_t s);

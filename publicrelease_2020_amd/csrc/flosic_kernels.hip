@@ -75,10 +75,15 @@ __device__ __forceinline__ void store_comps(double x, double y, double z, double
                                             int64_t ldphi, int64_t plane, int f0, int stride, int64_t k) {
     if constexpr (M0 < ncart(L)) {
         const int64_t col = (int64_t)(f0 + M0 * stride) * ldphi + k;
-        if constexpr (ND < 0) {
+        if constexpr (ND == -1) {
             double o[10];
             cart_eval<loff(L) + M0, 10>(x, y, z, S0, S1, S2, o);
             phi[col] = (T)(o[4] + o[5] + o[6]);
+        } else if constexpr (ND == -3) {
+            double o[4];
+            cart_eval<loff(L) + M0, 4>(x, y, z, S0, S1, S2, o);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) phi[d * plane + col] = (T)o[1 + d];
         } else {
             double o[ND];
             cart_eval<loff(L) + M0, ND>(x, y, z, S0, S1, S2, o);
@@ -133,6 +138,7 @@ void launch_basis_planes(const BasisDev &B, const double *x, const double *y, co
     if (nd == 1) hipLaunchKernelGGL((k_basis_planes<T, 1>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
     else if (nd == 4) hipLaunchKernelGGL((k_basis_planes<T, 4>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
     else if (nd == -1) hipLaunchKernelGGL((k_basis_planes<T, -1>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
+    else if (nd == -3) hipLaunchKernelGGL((k_basis_planes<T, -3>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
     else hipLaunchKernelGGL((k_basis_planes<T, 10>), grid, blk, 0, s, B, x, y, z, M, phi, ldphi);
 }
 template void launch_basis_planes<double>(const BasisDev &, const double *, const double *, const double *, int64_t,
