@@ -155,6 +155,27 @@ def test_batch_graph_replays_match_eager():
     assert c[1] == 1 and c[2] == 3
 
 
+def test_force_step_getters_after_graph_replay():
+    """SICLAG_DER and the DFT-side potential on contexts last evaluated by a replayed batch graph equal the same calls
+    on contexts evaluated one at a time (LSDA and PBE, resident and mesh-block contexts)."""
+    F = _F()
+    systems = _systems()
+    engines = _engines(systems)
+    single = _engines(systems)
+    batch = F.FlosicBatch(engines, use_graph=True)
+    for _ in range(3):  # eager, capture, replay
+        z = batch.evaluate()
+    assert int(batch.timings()[3]) == 2
+    _close(z, np.array([e.evaluate() for e in single]), 1e-13)
+    for e, r in zip(engines, single):
+        d, d_ref = e.siclag_der(), r.siclag_der()
+        assert np.abs(d - d_ref).max() <= 1e-12 * np.abs(d_ref).max()
+        (p, en), (p_ref, en_ref) = e.apotnl_dft(), r.apotnl_dft()
+        assert np.abs(p - p_ref).max() <= 1e-12 * np.abs(p_ref).max()
+        for k in en_ref:
+            assert abs(en[k] - en_ref[k]) <= 1e-12 * abs(en_ref[k]) + 1e-14, k
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "mixed", "bf16x3"])
 def test_coulomb_column_blocks_and_autotuner(prec):
     """Every legal Coulomb tile width gives the same potentials (the columns are independent); the autotuner
