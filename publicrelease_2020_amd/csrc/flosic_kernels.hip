@@ -1837,24 +1837,47 @@ __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi
             acc[it][r] = (T)0;
             sum[it][r] = 0.0;
         }
-    for (int64_t t0 = kb; t0 < ke; t0 += kVabKS) {
-        for (int e = threadIdx.x; e < NI * kVabKS; e += 256) {
-            const int i = e / kVabKS, kk = e - i * kVabKS;
+    // the 32 points of the next step are loaded into registers while the MFMAs consume the current step from LDS
+    constexpr int NV = NI * kVabKS / 256, NP = 16 * kVabKS / 256;
+    double rv[NV], rp[NP], rw = 0.0;
+    auto load = [&](int64_t t0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int e = threadIdx.x + 256 * j, i = e / kVabKS, kk = e - i * kVabKS;
             const int64_t k = t0 + kk;
-            Vl[i * LDV + kk] = (T)((i0 + i < n && k < ke) ? Vt[(int64_t)(i0 + i) * M + k] : 0.0);
+            rv[j] = (i0 + i < n && k < ke) ? Vt[(int64_t)(i0 + i) * M + k] : 0.0;
         }
-        for (int e = threadIdx.x; e < 16 * kVabKS; e += 256) {
-            const int r = e / kVabKS, kk = e - r * kVabKS;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int e = threadIdx.x + 256 * j, r = e / kVabKS, kk = e - r * kVabKS;
             const int64_t k = t0 + kk;
             const int f = r < 8 ? 8 * A + r : 8 * B + r - 8;
-            const double v = (f < n && k < ke) ? phi[(int64_t)f * M + k] : 0.0;
-            if (r < 8)
-                Pa[r * LDV + kk] = v;
-            else
-                Pb[(r - 8) * LDV + kk] = v;
+            rp[j] = (f < n && k < ke) ? phi[(int64_t)f * M + k] : 0.0;
         }
-        if (threadIdx.x < kVabKS) Wl[threadIdx.x] = t0 + threadIdx.x < ke ? w[t0 + threadIdx.x] : 0.0;
-        __syncthreads();
+        if (threadIdx.x < kVabKS) rw = t0 + threadIdx.x < ke ? w[t0 + threadIdx.x] : 0.0;
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int e = threadIdx.x + 256 * j, i = e / kVabKS, kk = e - i * kVabKS;
+            Vl[i * LDV + kk] = (T)rv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int e = threadIdx.x + 256 * j, r = e / kVabKS, kk = e - r * kVabKS;
+            if (r < 8)
+                Pa[r * LDV + kk] = rp[j];
+            else
+                Pb[(r - 8) * LDV + kk] = rp[j];
+        }
+        if (threadIdx.x < kVabKS) Wl[threadIdx.x] = rw;
+    };
+    load(kb);
+    store();
+    __syncthreads();
+    for (int64_t t0 = kb; t0 < ke; t0 += kVabKS) {
+        const bool more = t0 + kVabKS < ke;  // uniform over the block
+        if (more) load(t0 + kVabKS);
 #pragma unroll
         for (int ks = 0; ks < kVabKS; ks += 4) {
             const int kk = ks + lr;
@@ -1871,6 +1894,8 @@ __global__ void __launch_bounds__(256) k_vab_mfma(const double *__restrict__ phi
                     acc[it][r] = (T)0;
                 }
         }
+        __syncthreads();
+        if (more) store();
         __syncthreads();
     }
     const int a = 8 * A + 2 * wave + (lc >> 3), b = 8 * B + (lc & 7);
