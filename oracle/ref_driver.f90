@@ -97,6 +97,46 @@ contains
     call GORBDRV(nderv, iupdat, icount, npts, p25, 1, grad)
   end subroutine ref_gorbdrv
 
+  ! SUBVLXC with NSPN = 2 and no atom centres (no core density, MODE = 1: no local potential): gga = 0 takes
+  ! IGGA = 0 and IDFTYP = (6, 6) (PW91EX at s = u = v = 0, PW91LC + PW91NC), gga = 1 IGGA = 1 and IDFTYP = (7, 7)
+  ! (PBEEX, PBECOR).  rgu, rgd: value, gradient and Hessian of the spin densities; pot(:, 1) spin up, pot(:, 2)
+  ! spin down; e = EXCVEC summed over the 100-point blocks SUBVLXC takes at a time.
+  subroutine ref_subvlxc(gga, npts, rgu, rgd, w, pot, e) bind(C, name="ref_subvlxc")
+    use mesh1, only : rmsh, wmsh, nmsh
+    use common2, only : nspn, igga, idftyp, ncnt
+    integer(c_int), value :: gga, npts
+    real(c_double) :: rgu(10, npts), rgd(10, npts), w(npts), pot(npts, 2), e(4)
+    real(8) :: rhov(10 * 2 * 100), vxcs(2 * 100), vlos(100), excvec(4)
+    integer :: i0, m, k, ng
+    nmsh = npts
+    rmsh(:, 1:npts) = 0.0d0
+    wmsh(1:npts) = w
+    nspn = 2
+    ncnt = 0
+    if (gga /= 0) then
+      igga = 1
+      idftyp = 7
+      ng = 10
+    else
+      igga = 0
+      idftyp = 6
+      ng = 1
+    end if
+    e = 0.0d0
+    do i0 = 1, npts, 100
+      m = min(100, npts - i0 + 1)
+      rhov = 0.0d0
+      do k = 1, m
+        rhov(ng * 2 * (k - 1) + 1 : ng * 2 * (k - 1) + ng) = rgu(1:ng, i0 + k - 1) + rgd(1:ng, i0 + k - 1)
+        rhov(ng * (2 * (k - 1) + 1) + 1 : ng * (2 * (k - 1) + 1) + ng) = rgd(1:ng, i0 + k - 1)
+      end do
+      call SUBVLXC(1, i0 - 1, m, rhov, vxcs, vlos, excvec)
+      pot(i0 : i0 + m - 1, 1) = vxcs(1:m)
+      pot(i0 : i0 + m - 1, 2) = vxcs(m + 1 : 2 * m)
+      e = e + excvec
+    end do
+  end subroutine ref_subvlxc
+
   subroutine ref_pw91ex(dkf, s, u, v, exl, exn, vx) bind(C, name="ref_pw91ex")
     real(c_double), value :: dkf, s, u, v
     real(c_double) :: exl, exn, vx

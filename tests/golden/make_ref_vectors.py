@@ -7,6 +7,7 @@ mounted).  This script calls, through oracle/ref_driver.f90:
   GORBDRV               values, gradients and Hessians of an O-atom ladder basis at 25 points
   PW91EX/PW91LC/PBEEX/PBECOR  at random arguments
   FRMORB2(-ISPN,-1)     FOD force (DEBDAX) and SLAT for FMAT/SIC of a small seeded system
+  SUBVLXC (NSPN = 2)    DFT-side LSDA and PBE potentials and energies of random spin densities
 and stores inputs and outputs in ref_vectors.npz.  Run from the repository root:
     python tests/golden/make_ref_vectors.py
 """
@@ -118,6 +119,22 @@ def main():
         out[f"frm_debdax{s}"] = deb
         out[f"frm_slat{s}"] = slat.T.copy()
     out.update(frm_fmat1=f1, frm_fmat2=f2)
+    # SUBVLXC (NSPN = 2): spin densities with value, gradient and Hessian, LSDA (IGGA = 0) and PBE (IGGA = 1)
+    rng = np.random.default_rng(23)
+    nd = 250
+    rgu = rng.normal(size=(nd, 10)) * 10.0 ** rng.uniform(-3, 1, (nd, 1))
+    rgd = rng.normal(size=(nd, 10)) * 10.0 ** rng.uniform(-3, 1, (nd, 1))
+    rgu[:, 0] = 10.0 ** rng.uniform(-4, 2, nd)
+    rgd[:, 0] = rgu[:, 0] * 10.0 ** rng.uniform(-3, 0.5, nd)
+    rgd[:10, :] = 0.0  # fully polarised points
+    wd = rng.uniform(0.1, 2.0, nd)
+    for name, gga in (("lsda", 0), ("pbe", 1)):
+        pot = np.zeros((2, nd))
+        e = np.zeros(4)
+        R.ref_subvlxc(C.c_int(gga), C.c_int(nd), d(rgu), d(rgd), d(wd), d(pot), d(e))
+        out[f"dft_pot_{name}"] = pot
+        out[f"dft_e_{name}"] = e
+    out.update(dft_rgu=rgu, dft_rgd=rgd, dft_w=wd)
     os.chdir(here)
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "ref_vectors.npz"), **out)
     print("wrote", sorted(out.keys()))

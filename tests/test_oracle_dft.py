@@ -134,3 +134,18 @@ def test_gga_potential_is_the_functional_derivative_for_each_spin():
         dE = ((ep["ex"] + ep["ec"]) - (em["ex"] + em["ec"])) / (2 * dl)
         dv = (pot[s] * g[:, 0] * wr).sum()
         assert abs(dE - dv) < 2e-6 * abs(dv), (s, dE, dv)
+
+
+def test_dft_matches_reference_subvlxc(golden):
+    """The reference's own SUBVLXC (NSPN = 2, no centres; tests/golden/make_ref_vectors.py) on random spin densities
+    with gradients and Hessians, ten of them fully polarised: LSDA (IGGA = 0, PW91EX + PW91LC/PW91NC) and PBE."""
+    rgu, rgd, w = golden["dft_rgu"], golden["dft_rgd"], golden["dft_w"]
+    zero = np.zeros(w.size)
+    for name, pot_e in (("lsda", O.dft_lsda(rgu[:, 0], rgd[:, 0], w, zero)), ("pbe", O.dft_gga(rgu, rgd, w, zero))):
+        pot, e = pot_e
+        ref_pot, ref_e = golden[f"dft_pot_{name}"], golden[f"dft_e_{name}"]
+        err = np.abs(pot - ref_pot).max() / np.abs(ref_pot).max()
+        print(name, "max rel potential error", err, "energies", e["ex"], ref_e[0] + ref_e[1], e["ec"], ref_e[2] + ref_e[3])
+        assert err < 1e-12, name
+        assert abs(e["ex"] - (ref_e[0] + ref_e[1])) < 1e-12 * abs(ref_e[0]), name
+        assert abs(e["ec"] - (ref_e[2] + ref_e[3])) < 1e-12 * abs(ref_e[2]), name
