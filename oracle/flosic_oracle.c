@@ -19,6 +19,9 @@
  *   V^i_ab ..................... sicmat.F:380-411 restricted to the FLO space
  *   FOD force formula .......... frmorb2.F:589-717 (including the undoubled overlap derivative in its first term)
  *   exact FOD gradient ......... back-propagation of dE/dTMAT = 2 (SIC^T TMAT) through the Lowdin step
+ *   LAGMULT_SLV, APATCH ........ lagmult_slv.F:2-187; sicham.F:920-945, sicmat.F:146-262
+ *   DFT-side potential ......... subvlxc.F:138-323 with NSPN = 2 (LSDA and PBE), plus the electronic Hartree part
+ *   SICLAG_DER (DERSIC) ........ siclag_der.F:186-640 without point-group symmetry and libxc/meta-GGA terms
  *
  * Deliberate, documented differences from the reference text:
  *   - the Coulomb potential is the exact analytic one at every point (the reference's own DEBUG/exact mode);
