@@ -196,7 +196,9 @@ def read_input(text: str) -> InputFile:
 
 # ------------------------------------------------------------------------------------------------ FRMORB, fforce
 def read_frmorb(text: str) -> Tuple[np.ndarray, np.ndarray]:
-    """FRMORB (legacy format, main.F:150-163): LFRM(1) LFRM(2), then the up FODs, then the down FODs."""
+    """FRMORB (main.F:150-163): LFRM(1) LFRM(2) on the first record, then one FOD per record, up then down.  Each
+    READ takes the first values of its record, so symfrm.f's layout (four counts in the header; each FOD followed
+    by its weight and equivalence indices; blank records between the spins) reads the same way."""
     r = _ListReader(text)
     n_up, n_dn = (int(v) for v in r.values(2))
     up = np.array([r.values(3) for _ in range(n_up)], dtype=np.float64).reshape(n_up, 3)
@@ -205,12 +207,18 @@ def read_frmorb(text: str) -> Tuple[np.ndarray, np.ndarray]:
 
 
 def write_frmorb(fod_up, fod_dn) -> str:
+    """FRMORB in symfrm.f's layout (symfrm.f:136-165) without symmetry: header LFRM(1) LFRM(2) LFRM(1) LFRM(2)
+    (every FOD its own identity member), each FOD at full precision followed by its weight 1 and its own index,
+    two blank records after each spin.  main.F reads the first two counts and three coordinates per record; the FOD
+    optimiser (electronic_geometry_lbfgs.f:70-83) also reads the weight and index."""
     up = np.asarray(fod_up, dtype=np.float64).reshape(-1, 3)
     dn = np.asarray(fod_dn, dtype=np.float64).reshape(-1, 3)
     out = io.StringIO()
-    out.write(f"{up.shape[0]:d} {dn.shape[0]:d}\n")
-    for p in np.vstack([up, dn]):
-        out.write(" ".join(f"{x:.15E}" for x in p) + "\n")
+    out.write(f"{up.shape[0]:5d}{dn.shape[0]:5d}{up.shape[0]:9d}{dn.shape[0]:5d}\n")
+    for fods in (up, dn):
+        for i, p in enumerate(fods):
+            out.write(" ".join(f"{x:24.16E}" for x in p) + f" {1:4d} {i + 1:4d}\n")
+        out.write("  \n  \n")
     return out.getvalue()
 
 
@@ -252,7 +260,7 @@ class Wavefunctions:
     nwfs: List[int]
     evlocc: np.ndarray
     occupancy: List[np.ndarray]   # per spin
-    coef: List[np.ndarray]        # per spin (ns_tot, n_occ): PSI_COEF columns
+    coef: List[np.ndarray]        # per spin (ns_tot, n_occ): PSI_COEF columns, rows in the file's SALC order
 
 
 def read_wfout(data: bytes) -> Wavefunctions:
@@ -274,6 +282,37 @@ def read_wfout(data: bytes) -> Wavefunctions:
             cols.append(np.frombuffer(recs[k], dtype="<f8").copy()); k += 1
         coef.append(np.array(cols, dtype=np.float64).reshape(n_occ, ns_tot).T.copy())
     return Wavefunctions(nspn=nspn, nwf=nwf, nwfs=nwfs, evlocc=evlocc, occupancy=occ, coef=coef)
+
+
+def salc_order(basis: Basis) -> np.ndarray:
+    """Index map between WFOUT's coefficient order and the local basis order, point group NONE.
+
+    WFOUT holds PSI_COEF in the symmetry-adapted (SALC) basis.  With point group NONE every site is its own shell and
+    U_MAT is the identity, so UNRAVEL's loops (unravel.f:51-89) read, per site, l outer, then the contraction IBASE,
+    then the SALC index IQ = the Cartesian component: coefficient j of the file is local function ``order[j]`` of the
+    site-major (l, component, contraction) order of gorbdrv.f / frmorb2.F:262-266."""
+    local = {}
+    k = 0
+    for a in range(basis.natom):
+        for l in range(3):
+            for m in range(NCART[l]):
+                for c in range(int(basis.ncon[a, l])):
+                    local[(a, l, m, c)] = k
+                    k += 1
+    return np.array([local[(a, l, m, c)] for a in range(basis.natom) for l in range(3)
+                     for c in range(int(basis.ncon[a, l])) for m in range(NCART[l])], dtype=np.int64)
+
+
+def salc_to_local(basis: Basis, coef: np.ndarray) -> np.ndarray:
+    """WFOUT coefficient rows (SALC order) -> local basis order."""
+    out = np.empty_like(coef)
+    out[salc_order(basis)] = coef
+    return out
+
+
+def local_to_salc(basis: Basis, coef: np.ndarray) -> np.ndarray:
+    """Local basis order -> WFOUT coefficient rows (SALC order)."""
+    return np.ascontiguousarray(np.asarray(coef)[salc_order(basis)])
 
 
 def write_wfout(wf: Wavefunctions) -> bytes:
@@ -342,7 +381,9 @@ def load_run(directory: str, frmorb: Optional[str] = None) -> System:
         c = wf.coef[min(s, wf.nspn - 1)]  # NSPN = 1: both spins share the orbitals
         if c.shape[1] < fods.shape[0]:
             raise ValueError(f"spin {s + 1}: {fods.shape[0]} FODs but {c.shape[1]} orbitals in {inp.wffile}")
-        coef.append(c)
+        if c.shape[0] != inp.basis.nbas:
+            raise ValueError(f"{inp.wffile}: {c.shape[0]} coefficients per orbital, the basis has {inp.basis.nbas}")
+        coef.append(salc_to_local(inp.basis, c))
         nocc.append(fods.shape[0])  # one FOD per occupied orbital (frmiorb.f:443-449)
     return System(name=os.path.basename(os.path.normpath(directory)), basis=inp.basis, mesh=Mesh(xyz=xyz, w=w),
                   coef=coef, nocc=nocc, fod=[up, dn], xc=functional_xc(sym.functional), nspn=wf.nspn)

@@ -11,7 +11,12 @@ program flosic_sic_drive
   integer(c_int32_t) :: natom, nbare, ncoef, nspn, xc, prec, nbas, nwf(2), nocc(2), status, s
   integer(c_int64_t) :: nmsh
   real(c_double), allocatable :: xyz(:), alpha(:), con(:), rmsh(:), wmsh(:), c1(:, :), c2(:, :), f1(:, :), f2(:, :)
-  real(c_double), allocatable :: r1(:, :), r2(:, :), s1(:, :), s2(:, :), d1(:, :), d2(:, :), dersic(:), potdft(:)
+  real(c_double), allocatable :: r1(:, :), r2(:, :), s1(:, :), s2(:, :), d1(:, :), d2(:, :), dpk(:, :, :), potdft(:)
+  ! SICMAT's DERSIC(3,MAX_OCC,MX_CNT) layout (modules_sic.F90:44, MAX_OCC = 1500): the packed library result is
+  ! copied into its leading columns, as a binding into the reference's modules must do
+  integer, parameter :: MAX_OCC = 1500
+  real(c_double), allocatable :: dersic(:, :, :)
+  integer :: nf
   real(c_double) :: edft(5)
   integer(c_int32_t), allocatable :: bare_off(:), ncon(:), con_off(:)
   real(c_double) :: zsic
@@ -35,11 +40,14 @@ program flosic_sic_drive
   if (status /= FLOSIC_OK) call fail(status)
   call flosic_sic_evaluate(ctx, nbas, nwf, nocc, c1, c2, f1, f2, r1, r2, s1, s2, d1, d2, zsic, status)
   if (status /= FLOSIC_OK) call fail(status)
-  allocate(dersic(3 * (nocc(1) + nocc(2)) * natom), potdft(2 * nmsh))
-  status = flosic_siclag_der(ctx, dersic)
+  nf = nocc(1) + nocc(2)
+  allocate(dpk(3, nf, natom), dersic(3, MAX_OCC, natom), potdft(2 * nmsh))
+  dersic = 0.0d0
+  status = flosic_siclag_der(ctx, dpk)
   if (status /= FLOSIC_OK) call fail(status)
+  dersic(:, 1:nf, 1:natom) = dpk
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(11) zsic, r1, r2, s1, s2, d1, d2, dersic
+  write(11) zsic, r1, r2, s1, s2, d1, d2, dersic(:, 1:nf, 1:natom)
   if (nspn == 2) then
     status = flosic_apotnl_dft(ctx, potdft, edft)
     if (status /= FLOSIC_OK) call fail(status)

@@ -418,6 +418,18 @@ def make_config(cfg: int, index: int = 0, **kw) -> System:
     raise ValueError(f"unknown config {cfg}")
 
 
+def subsample_mesh(sysm: System, npts: int) -> System:
+    """The same system on every m-th mesh point (m = M // npts), weights times m: a bounded sample of a large
+    workload on which the CPU oracle finishes in seconds."""
+    M = sysm.mesh.npts
+    step = max(1, M // max(1, npts))
+    idx = np.arange(0, M, step)
+    return dataclasses.replace(sysm, name=f"{sysm.name}-every{step}",
+                               mesh=Mesh(xyz=np.ascontiguousarray(sysm.mesh.xyz[idx]),
+                                         w=np.ascontiguousarray(sysm.mesh.w[idx] * step)),
+                               meta=dict(sysm.meta, mesh_step=step))
+
+
 def make_tiny(K: int = 3, nocc: int = 3, seed: int = 7, xc: int = XC_LSDA, nrad: int = 24, nang: int = 50,
               nwf: Optional[int] = None, box: float = 4.5) -> System:
     """Small system for parity tests (oracle finishes in seconds)."""

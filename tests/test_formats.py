@@ -138,7 +138,8 @@ def test_load_run_directory(tmp_path):
     rng = np.random.default_rng(5)
     C = S.random_orbitals(Smat, 5, 8, rng)
     wf = Fm.Wavefunctions(nspn=2, nwf=16, nwfs=[8, 8], evlocc=np.zeros(16),
-                          occupancy=[np.r_[np.ones(5), np.zeros(3)]] * 2, coef=[C, C])
+                          occupancy=[np.r_[np.ones(5), np.zeros(3)]] * 2,
+                          coef=[Fm.local_to_salc(b, C)] * 2)
     (tmp_path / "WFOUT").write_bytes(Fm.write_wfout(wf))
     mesh = S.becke_mesh(b.xyz, np.random.default_rng(6), nrad=10, nang=14)
     (tmp_path / "OLDMESH").write_bytes(Fm.write_mesh(mesh.xyz, mesh.w))

@@ -1,0 +1,224 @@
+"""The benchmark workloads themselves under test, through the C ABI (FP64 parity with the oracle on bounded mesh
+samples, the reduced-precision gate on the full meshes, the PBE gradient against central differences, and a
+batch of the batch generator's systems).
+
+Workloads (publicrelease_2020_amd.system.make_config): K=16 LSDA (556 basis functions, 185 772 points, 24+24
+FODs), K=64 LSDA (1832, 737 129, 96+96), K=32 PBE (1028, 370 369, 48+48), batch members of the K in [4, 96]
+generator.
+
+Tolerances:
+  FP64 GPU vs oracle on the same mesh sample   E_SIC |d| < 1e-10 Ha; RESULTS, SIC, V^i_ab |d| < 1e-10 max|x|;
+                                               FOD forces and gradient |d| < 1e-9 Ha/bohr; pair list bit-exact
+  reduced forms vs FP64 (full mesh)            |dE_SIC| < 1e-6 Ha, max |dF| < 1e-5 Ha/bohr (the gate);
+                                               max |dV^i_ab| < 1e-6 Ha (measured ~1e-7)
+  PBE gradient vs central differences          h = 1e-4 bohr, |d| < 1e-5 Ha/bohr (measured ~3e-6: quadrature of
+                                               the Hartree term, see test_gpu_autograd.py)
+  mesh-block vs resident                       E_SIC |d| < 1e-11 Ha, gradients |d| < 1e-11 Ha/bohr
+"""
+import numpy as np
+import pytest
+
+from publicrelease_2020_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _F():
+    from publicrelease_2020_amd import flosic as F
+    return F
+
+
+def _close(a, b, rel, abs_=0.0, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    tol = rel * max(np.abs(b).max(), 1e-300) + abs_
+    err = np.abs(a - b).max()
+    assert err <= tol, f"{what}: max abs diff {err:.3e} > {tol:.3e}"
+    return err
+
+
+def _eval(sysm, prec=0, block=0):
+    F = _F()
+    ctx = F.Flosic(0)
+    ctx.load_system(sysm, prec)
+    if block:
+        ctx.set_mesh_block(block)
+    e = ctx.evaluate()
+    out = ctx.results()
+    out["esic"] = e
+    return ctx, out
+
+
+@pytest.mark.parametrize("cfg,npts", [(2, 2000), (4, 300), (3, 150)], ids=["K16", "K32-PBE", "K64"])
+def test_workload_oracle_parity_on_mesh_sample(cfg, npts):
+    """The full workload (basis, orbitals, FODs, every shell pair) on every m-th mesh point: the HIP FP64 path
+    against the oracle, and the full shell-pair list bit-exact."""
+    from oracle import oracle as O
+    full = S.make_config(cfg)
+    sysm = S.subsample_mesh(full, npts)
+    print(f"config {cfg}: {sysm.name} nbas={sysm.nbas} M={sysm.mesh.npts} nocc={sysm.nocc}", flush=True)
+    ref = O.evaluate(sysm, nthreads=16)
+    assert ref["status"] == 0
+    ctx, got = _eval(sysm)
+    try:
+        assert np.array_equal(ctx.pairs(), O.pairs(full.basis))
+        print(f"config {cfg}: {ctx.pairs().shape[0]} shell pairs bit-exact; E_SIC {got['esic']:.12f} "
+              f"(oracle {ref['esic']:.12f})", flush=True)
+        assert abs(got["esic"] - ref["esic"]) < 1e-10
+        for s in range(2):
+            g, r = got["spins"][s], ref["spins"][s]
+            _close(g["results"], r["results"], 1e-10, 1e-12, "RESULTS")
+            _close(g["sic"], r["sic"], 1e-10, 1e-12, "SIC")
+            _close(g["vab"], r["vab"], 1e-10, 1e-12, "V^i_ab")
+            _close(g["debdax"], r["debdax"], 0.0, 1e-9, "DEBDAX")
+            _close(g["grad"], r["grad"], 0.0, 1e-9, "gradient")
+    finally:
+        ctx.close()
+
+
+@pytest.fixture(scope="module")
+def k64_fp64():
+    sysm = S.make_config(3)
+    ctx, got = _eval(sysm)
+    ctx.close()
+    print(f"K=64 FP64: E_SIC {got['esic']:.12f}", flush=True)
+    return sysm, got
+
+
+def test_k64_fp64_properties(k64_fp64):
+    """lambda_ii (SICLAGM) = XDEL (APOTNL_SIC), V^i_ai = -SIC(a,i), E_SIC = sum RESULTS(9), adjoint gradient equal
+    to the FRMORB2 formula (LSDA), unit FLO charges up to the mesh quadrature."""
+    sysm, got = k64_fp64
+    esum = 0.0
+    for s in range(2):
+        sp = got["spins"][s]
+        n = sysm.nocc[s]
+        assert sp["sic"].shape == (sysm.coef[s].shape[1], n)
+        _close(np.diag(sp["sic"][:n, :n]), sp["results"][:, 6], 1e-11, 0.0, "lambda_ii vs XDEL")
+        _close(np.array([sp["vab"][i, :, i] for i in range(n)]).T, -sp["sic"][:n, :n], 1e-10, 0.0, "V^i_ai")
+        _close(sp["grad"], sp["debdax"], 1e-9, 1e-13, "gradient vs DEBDAX")
+        assert np.all(np.abs(sp["results"][:, 1] - 1.0) < 2e-3)
+        esum += sp["results"][:, 8].sum()
+    assert abs(esum - got["esic"]) < 1e-11
+
+
+@pytest.mark.parametrize("prec", [1, 3], ids=["mixed", "bf16x3"])
+def test_k64_reduced_forms_within_gate(k64_fp64, prec):
+    """The precision workload's reduced forms on the full K=64 mesh against its FP64 form."""
+    F = _F()
+    sysm, ref = k64_fp64
+    p = {1: F.PREC_MIXED, 3: F.PREC_BF16X3}[prec]
+    ctx, got = _eval(sysm, p)
+    ctx.close()
+    de = abs(got["esic"] - ref["esic"])
+    df = max(np.abs(got["spins"][s]["debdax"] - ref["spins"][s]["debdax"]).max() for s in range(2))
+    dg = max(np.abs(got["spins"][s]["grad"] - ref["spins"][s]["grad"]).max() for s in range(2))
+    dv = max(np.abs(got["spins"][s]["vab"] - ref["spins"][s]["vab"]).max() for s in range(2))
+    print(f"K=64 form {prec}: |dE| {de:.3e} Ha, max|dF| {df:.3e}, max|dgrad| {dg:.3e} Ha/bohr, "
+          f"max|dV^i_ab| {dv:.3e} Ha", flush=True)
+    assert de < 1e-6
+    assert df < 1e-5 and dg < 1e-5
+    assert dv < 1e-6
+
+
+def test_k32_pbe_gradient_fd_and_mesh_blocks():
+    """Differentiation workload: the adjoint gradient against central differences on four FOD coordinates;
+    mesh-block recomputation equal to resident planes; the mixed form's gradient error against FP64."""
+    F = _F()
+    sysm = S.make_config(4)
+    ctx, got = _eval(sysm)
+    try:
+        print(f"K=32 PBE: E_SIC {got['esic']:.12f}, peak device memory {ctx.peak_device_bytes() / 2**30:.2f} GiB",
+              flush=True)
+        h = 1e-4
+        fods = [sysm.fod[0].copy(), sysm.fod[1].copy()]
+        for (s, i, x) in [(0, 0, 0), (1, 5, 1), (0, 17, 2), (1, 40, 0)]:
+            e = []
+            for sgn in (1, -1):
+                f = fods[s].copy()
+                f[i, x] += sgn * h
+                ctx.set_fods(s + 1, f)
+                e.append(ctx.evaluate())
+            ctx.set_fods(s + 1, fods[s])
+            fd = (e[0] - e[1]) / (2 * h)
+            g = got["spins"][s]["grad"][i, x]
+            print(f"K=32 PBE coord {(s, i, x)}: fd {fd:.9f} adjoint {g:.9f} diff {abs(fd - g):.2e}", flush=True)
+            assert abs(fd - g) < 1e-5
+        peak_res = ctx.peak_device_bytes()
+    finally:
+        ctx.close()
+    blk, gb = _eval(sysm, 0, 65536)
+    peak_blk = blk.peak_device_bytes()
+    blk.close()
+    print(f"K=32 PBE mesh-block: E_SIC {gb['esic']:.12f}, peak {peak_blk / 2**30:.2f} GiB "
+          f"(resident {peak_res / 2**30:.2f} GiB)", flush=True)
+    assert abs(gb["esic"] - got["esic"]) < 1e-11
+    assert peak_blk < peak_res
+    for s in range(2):
+        _close(gb["spins"][s]["grad"], got["spins"][s]["grad"], 0.0, 1e-11, "mesh-block gradient")
+    mx, gm = _eval(sysm, F.PREC_MIXED)
+    mx.close()
+    dg = max(np.abs(gm["spins"][s]["grad"] - got["spins"][s]["grad"]).max() for s in range(2))
+    print(f"K=32 PBE mixed: |dE| {abs(gm['esic'] - got['esic']):.3e} Ha, max|dgrad| {dg:.3e} Ha/bohr", flush=True)
+    assert abs(gm["esic"] - got["esic"]) < 1e-6
+    assert dg < 1e-5
+
+
+BATCH_MEMBERS = [1, 3, 4, 11]  # K = 15 LSDA, 28 PBE, 4 PBE, 12 PBE
+
+
+def test_batch_members_graph_and_gate():
+    """Members of the batch workload (mesh-block contexts, 8192-point blocks, FP32-MFMA form): the replayed batch
+    graph, with the FODs moved between replays, equals the contexts evaluated one at a time, and every system is
+    within the gate of its plain FP64 evaluation."""
+    F = _F()
+    systems = [S.make_config(5, index=i) for i in BATCH_MEMBERS]
+    for sy in systems:
+        print(f"batch member {sy.name}: nbas={sy.nbas} M={sy.mesh.npts} nocc={sy.nocc} xc={sy.xc}", flush=True)
+
+    def ctxs(prec):
+        out = []
+        for sy in systems:
+            e = F.Flosic(0)
+            e.load_system(sy, prec)
+            e.set_mesh_block(8192)
+            out.append(e)
+        return out
+
+    engines = ctxs(F.PREC_MIXED)
+    single = ctxs(F.PREC_MIXED)
+    fp64 = ctxs(F.PREC_FP64)
+    batch = F.FlosicBatch(engines, use_graph=True)
+    try:
+        rng = np.random.default_rng(3)
+        last = [[sy.fod[0], sy.fod[1]] for sy in systems]
+        for rep in range(3):
+            if rep:
+                for k, (sy, a, b) in enumerate(zip(systems, engines, single)):
+                    for s in (1, 2):
+                        f = sy.fod[s - 1] + rng.normal(0, 0.01, sy.fod[s - 1].shape)
+                        a.set_fods(s, f)
+                        b.set_fods(s, f)
+                        last[k][s - 1] = f
+            z = batch.evaluate()
+            z1 = np.array([e.evaluate() for e in single])
+            print(f"batch replay {rep}: {z}", flush=True)
+            _close(z, z1, 1e-13, 0.0, "batch vs one at a time")
+            for a, b in zip(engines, single):
+                for s in (1, 2):
+                    _close(a.fod_gradient(s), b.fod_gradient(s), 1e-13, 1e-15, "batch gradient")
+        # the gate: the last FOD positions in the plain FP64 form, one system at a time
+        for k, e64 in enumerate(fp64):
+            for s in (1, 2):
+                e64.set_fods(s, last[k][s - 1])
+        z64 = np.array([e.evaluate() for e in fp64])
+        dE = np.abs(z - z64)
+        dF = [max(np.abs(a.fod_forces(s) - e.fod_forces(s)).max() for s in (1, 2)) for a, e in zip(engines, fp64)]
+        print(f"batch vs FP64: max |dE| {dE.max():.3e} Ha, max |dF| {max(dF):.3e} Ha/bohr", flush=True)
+        assert dE.max() < 1e-6
+        assert max(dF) < 1e-5
+    finally:
+        batch.close()
+        for e in engines + single + fp64:
+            e.close()

@@ -1,0 +1,46 @@
+"""The HIP path (through the C ABI) against the reference program's own outputs (tests/golden/reference_runs, written
+by nrlmol_exe built from the reference sources): the reference's run directories (basis, mesh, converged orbitals,
+FODs) evaluated on the GPU.
+
+Tolerances: FP64 as tests/test_reference_runs.py (E_SIC |d| < 1e-9 Ha, FOD forces |d| < 1e-9 Ha/bohr, RESULTS and
+SIC(j,i) within the rounding of the reference's print); the reduced forms within the gate of the reference's ZSIC
+and forces (|dE| < 1e-6 Ha, |dF| < 1e-5 Ha/bohr)."""
+import numpy as np
+import pytest
+
+from reference_runs import CASES, check, load
+
+pytestmark = pytest.mark.gpu
+
+
+def _eval(sysm, prec):
+    from publicrelease_2020_amd import flosic as F
+    ctx = F.Flosic(0)
+    try:
+        ctx.load_system(sysm, prec)
+        e = ctx.evaluate()
+        r = ctx.results()
+    finally:
+        ctx.close()
+    return e, r
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_hip_fp64_reproduces_reference_run(case):
+    sysm, ref = load(case)
+    print(f"reference run {case}: nbas={sysm.nbas} M={sysm.mesh.npts} FODs={sysm.nocc}", flush=True)
+    e, r = _eval(sysm, 0)
+    d = check(e, [sp["results"] for sp in r["spins"]], [sp["sic"] for sp in r["spins"]],
+              [sp["debdax"] for sp in r["spins"]], ref)
+    print(case, d, flush=True)
+
+
+@pytest.mark.parametrize("case", ["neon", "h2o_lsda", "h2o_pbe"])
+@pytest.mark.parametrize("prec", [1, 2], ids=["mixed", "bf16x3"])
+def test_reduced_forms_within_gate_of_reference(case, prec):
+    sysm, ref = load(case)
+    e, r = _eval(sysm, prec)
+    deb = np.vstack([sp["debdax"] for sp in r["spins"] if sp["debdax"].size])
+    dE, dF = abs(e - ref["zsic"]), float(np.abs(deb - ref["fforce"][:deb.shape[0]]).max())
+    print(f"{case} form {prec}: |dE| {dE:.3e} Ha, max |dF| {dF:.3e} Ha/bohr", flush=True)
+    assert dE < 1e-6 and dF < 1e-5

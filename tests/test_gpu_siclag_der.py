@@ -51,3 +51,34 @@ def test_siclag_der_spin_restricted():
     print(f"NSPN = 1: max |dDERSIC| {err:.3e} of {np.abs(d_ref).max():.3e}", flush=True)
     assert d.shape == d_ref.shape
     assert err < 1e-10 * np.abs(d_ref).max()
+
+
+@pytest.mark.parametrize("xc", [S.XC_LSDA, S.XC_PBE], ids=["LSDA", "PBE"])
+def test_siclag_der_several_point_blocks(xc):
+    """A mesh of 46 560 points: resident LSDA rebuilds its gradient planes in more than one 32 768-point block and
+    accumulates across them; the same call in mesh-block mode and the oracle give the same DERSIC."""
+    from oracle import oracle as O
+    from publicrelease_2020_amd import flosic as F
+    sysm = S.make_synthetic(4, 4, 5, xc, nrad=60, nang=194, box=5.0)
+    assert sysm.mesh.npts > 40000
+    ctx = F.Flosic(0)
+    ctx.load_system(sysm)
+    ctx.evaluate()
+    d = ctx.siclag_der()
+    blk = F.Flosic(0)
+    blk.load_system(sysm)
+    blk.set_mesh_block(16384)
+    blk.evaluate()
+    d_blk = blk.siclag_der()
+    ref = O.evaluate(sysm, want_potentials=True, want_vab=False)
+    assert ref["status"] == 0
+    vpot = np.vstack([ref["spins"][0]["vpot"], ref["spins"][1]["vpot"]])
+    d_ref = O.siclag_der(sysm, vpot)
+    scale = np.abs(d_ref).max()
+    err, err_blk = np.abs(d - d_ref).max(), np.abs(d_blk - d).max()
+    print(f"xc {xc}, M={sysm.mesh.npts}: max |dDERSIC| {err:.3e} (mesh-block vs resident {err_blk:.3e}) of "
+          f"{scale:.3e}", flush=True)
+    assert err < 1e-10 * scale
+    assert err_blk < 1e-11 * scale
+    ctx.close()
+    blk.close()
