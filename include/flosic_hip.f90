@@ -83,6 +83,11 @@ module flosic_hip
       type(c_ptr), value :: ctx
       real(c_double) :: pot(*), energies(5)
     end function
+    integer(c_int32_t) function flosic_apotnl_dft_noalloc(ctx, pot, energies) bind(C, name="flosic_apotnl_dft_noalloc")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: pot(*), energies(5)
+    end function
     integer(c_int32_t) function flosic_siclag_der(ctx, dersic) bind(C, name="flosic_siclag_der")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

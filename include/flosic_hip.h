@@ -104,6 +104,15 @@ int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double
  * mesh-block mode the FLO fields are rebuilt block by block. */
 int32_t flosic_apotnl_dft(flosic_ctx *ctx, double *pot, double *energies);
 
+/* No counterpart in the reference: flosic_apotnl_dft for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same kernels, same outputs bit for
+ * bit, but it allocates nothing: the staged potential planes and the block partials live in a fixed per-GPU
+ * scratch array of the library (16.8 MB, held until the process ends), so the mesh may have at most 1 048 576
+ * points (FLOSIC_EINVAL beyond).  It runs no evaluation (FLOSIC_ESTATE unless the last flosic_evaluate is current)
+ * and needs the resident FLO fields (FLOSIC_EINVAL in mesh-block mode).  All calls on one GPU share the scratch:
+ * calls that overlap on one GPU (several host threads, several contexts) race. */
+int32_t flosic_apotnl_dft_noalloc(flosic_ctx *ctx, double *pot, double *energies);
+
 /* SICLAG_DER (siclag_der.F:2-181, SICLM_DERSLV siclag_der.F:186-640, without point-group symmetry and without the
  * libxc/meta-GGA terms): DERSIC(x, i, A) = sum_k w_k dphi_i^A/dR_A,x V_i phi_i for the FLOs of both spins (spin 1
  * first, nF = nfrm_up + nfrm_dn) and every atom A, phi_i^A the part of FLO i on atom A's basis functions and V_i the

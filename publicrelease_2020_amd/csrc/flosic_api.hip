@@ -197,7 +197,32 @@ struct flosic_batch {
     Status st;
 };
 
+// Scratch of flosic_apotnl_dft_noalloc, one copy per GPU for the life of the process: the two potential planes of
+// at most kNoAllocMaxPts points, their 256-point block partials (5 per block) and the 5 ordered sums.  Every call on
+// one GPU shares it; calls that overlap on one GPU race.  Host code never passes `scratch` itself to a kernel or a
+// copy (on the host the name is a stand-in, not a device address): it uses the address hipGetSymbolAddress returns
+// for the current device, looked up once per device when the first context on that device is created.
+constexpr int64_t kNoAllocMaxPts = int64_t(1) << 20;
+constexpr size_t kNoAllocScratchDoubles = 2 * (size_t)kNoAllocMaxPts + 5 * (size_t)(kNoAllocMaxPts / 256) + 8;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current); nullptr if the lookup fails.
+double *scratch_on_device(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 int fail(flosic_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -1622,6 +1647,10 @@ int32_t flosic_create(int32_t device, flosic_ctx **out) {
         delete c;
         return FLOSIC_EDEVICE;
     }
+    if (!scratch_on_device(device)) {  // the no-allocation entry point's scratch, looked up with the device current
+        delete c;
+        return FLOSIC_EDEVICE;
+    }
     std::vector<double> tab((size_t)kBoysTableDoubles);
     boys_table(tab.data());
     if (upload(c, c->d_btab, tab) || hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1897,6 +1926,38 @@ int32_t flosic_apotnl_dft(flosic_ctx *c, double *pot, double *energies) {
         for (int b = 0; b < nblk; ++b)
             for (int q = 0; q < 5; ++q) energies[q] += part[(size_t)5 * b + q];
     }
+    return FLOSIC_OK;
+}
+
+int32_t flosic_apotnl_dft_noalloc(flosic_ctx *c, double *pot, double *energies) {
+    if (!c) return FLOSIC_EINVAL;
+    if (c->nspn != 2) return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft_noalloc: NSPN = 2 only");
+    if (!c->evaluated)
+        return fail(c, FLOSIC_ESTATE, "flosic_apotnl_dft_noalloc: no current evaluation (call flosic_evaluate first; "
+                                      "this entry point runs no evaluation)");
+    if (c->block_pts > 0)
+        return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft_noalloc: needs resident FLO fields (mesh-block mode rebuilds "
+                                      "them in allocated buffers)");
+    const int64_t M = c->M;
+    if (M > kNoAllocMaxPts)
+        return fail(c, FLOSIC_EINVAL, "flosic_apotnl_dft_noalloc: %lld mesh points exceed the scratch limit of %lld",
+                    (long long)M, (long long)kNoAllocMaxPts);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "hipSetDevice");
+    double *base = scratch_on_device(c->device);
+    if (!base) return fail(c, FLOSIC_EDEVICE, "flosic_apotnl_dft_noalloc: scratch address lookup failed");
+    const int nblk = (int)((M + 255) / 256);
+    double *d_pot = base, *d_part = base + 2 * M, *d_sum = d_part + (size_t)5 * nblk;
+    // every block writes its 5 partials and every point its two potentials: nothing in the scratch needs clearing
+    if (c->xc == FLOSIC_XC_PBE)
+        launch_dft_gga(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, d_pot, M, d_part, c->stream);
+    else
+        launch_dft_lsda(c->d_phi.p, M, c->sp[0].nocc, c->nF, c->d_VH.p, M, c->d_w.p, d_pot, M, d_part, c->stream);
+    HIPCHK(c, hipGetLastError());
+    launch_sum_partials_ordered(d_part, nblk, 5, d_sum, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if (energies) HIPCHK(c, hipMemcpyAsync(energies, d_sum, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    if (pot) HIPCHK(c, hipMemcpyAsync(pot, d_pot, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return FLOSIC_OK;
 }
 

@@ -149,6 +149,7 @@ void launch_xc(int xc, int nd, const double *phi, int64_t M, int nF, const doubl
 
 // p[0..n) = 0 (a kernel: usable inside captured graphs); dst[i] = src[i * stride], i < n
 void launch_fill_zero(double *p, size_t n, hipStream_t s);
+void launch_sum_partials_ordered(const double *part, int nblk, int nq, double *out, hipStream_t s);
 // DFT-side LSDA potential (NSPN = 2) over M points: phi values [nF][M] (FLOs 0..nup-1 spin up), VH[f * ldvh + k],
 // w[k]; pot[k] spin up, pot[ldpot + k] spin down; partial[b * 5 + q] per 256-point block b of these M points
 void launch_dft_lsda(const double *phi, int64_t M, int nup, int nF, const double *VH, int64_t ldvh, const double *w,

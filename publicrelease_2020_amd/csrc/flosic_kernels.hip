@@ -1755,6 +1755,20 @@ void launch_dft_gga(const double *phi, int64_t M, int nup, int nF, const double 
                        pot, ldpot, partial);
 }
 
+// out[q] = sum over b in increasing order of part[b * nq + q]: the host's block-order sum, on one thread (the same
+// additions in the same order, so the same bits).
+__global__ void k_sum_partials_ordered(const double *__restrict__ part, int nblk, int nq, double *__restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int q = 0; q < nq; ++q) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * nq + q];
+        out[q] = s;
+    }
+}
+void launch_sum_partials_ordered(const double *part, int nblk, int nq, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials_ordered, dim3(1), dim3(64), 0, s, part, nblk, nq, out);
+}
+
 __global__ void k_fill_zero(double *__restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;

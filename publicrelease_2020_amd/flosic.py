@@ -60,6 +60,7 @@ def library():
             "flosic_evaluate": [vp, _dp],
             "flosic_apotnl_sic": [vp, C.c_int32, _dp, _dp],
             "flosic_apotnl_dft": [vp, _dp, _dp],
+            "flosic_apotnl_dft_noalloc": [vp, _dp, _dp],
             "flosic_siclag_der": [vp, _dp],
             "flosic_siclagm": [vp, C.c_int32, _dp],
             "flosic_sic_vab": [vp, C.c_int32, _dp],
@@ -94,7 +95,7 @@ def library():
 def exported_symbols() -> list:
     return ["flosic_create", "flosic_destroy", "flosic_last_error", "flosic_set_basis", "flosic_set_mesh",
             "flosic_set_options", "flosic_set_orbitals", "flosic_set_fods", "flosic_frmorb", "flosic_get_fo",
-            "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_siclag_der", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
+            "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_apotnl_dft_noalloc", "flosic_siclag_der", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
             "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
@@ -213,6 +214,15 @@ class Flosic:
         pot = np.zeros(2 * M)
         e = np.zeros(5)
         self._chk(self._L.flosic_apotnl_dft(self._h, _d(pot), _d(e)))
+        return pot.reshape(2, M), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), (float(x) for x in e)))
+
+    def apotnl_dft_noalloc(self):
+        """flosic_apotnl_dft_noalloc: the same outputs as apotnl_dft from the library's fixed scratch (no device
+        allocation; needs a current evaluation with resident fields)."""
+        M = int(self.counts()[2])
+        pot = np.zeros(2 * M)
+        e = np.zeros(5)
+        self._chk(self._L.flosic_apotnl_dft_noalloc(self._h, _d(pot), _d(e)))
         return pot.reshape(2, M), dict(zip(("ex", "ec", "eh", "n_up", "n_dn"), (float(x) for x in e)))
 
     def siclag_der(self) -> np.ndarray:
