@@ -7,7 +7,12 @@ OPTIONS, ROCm's amdflang for Fortran (gfortran is not in this image), gcc for th
 Makefile links (MKL from /opt/conda/lib here).  Module-defining files are compiled first in the Makefile's order,
 the rest in parallel.
 
-    python oracle/build_nrlmol.py [-j N]
+    python oracle/build_nrlmol.py [-j N] [--dropin]
+
+--dropin also links oracle/_ref/nrlmol/nrlmol_flosic_hip: the same objects with apotnl_sic.o, frmorb2.o, frmiorb.o
+and frmorb_mpi.o replaced by the library's Fortran drop-ins (publicrelease_2020_amd/fortran/flosic_dropin.F90,
+compiled against the reference's module files) and libflosic_hip.so: the reference program with its per-FLO SIC
+block on the GPU.
 """
 from __future__ import annotations
 
@@ -23,6 +28,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "_ref", "nrlmol")
 OBJ = os.path.join(OUT, "obj")
 EXE = os.path.join(OUT, "nrlmol_exe")
+DROPIN_EXE = os.path.join(OUT, "nrlmol_flosic_hip")
+REPO = os.path.dirname(HERE)
+PKG = os.path.join(REPO, "publicrelease_2020_amd")
+REPLACED = ["apotnl_sic.o", "frmorb2.o", "frmiorb.o", "frmorb_mpi.o"]
 FC = "/opt/rocm/bin/amdflang"
 MKL = "/opt/conda/lib"
 LISTS = ["FTNBASICS", "FBASICS", "FBASICSMPI", "LOCALSIC", "METAGGA", "SIC", "SIC2", "SIC3", "EXTRAFIL",
@@ -71,6 +80,7 @@ def compile_one(obj: str) -> tuple:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--dropin", action="store_true")
     a = ap.parse_args()
     if not os.path.isdir(REF):
         print(f"reference sources not present: {REF}")
@@ -101,6 +111,37 @@ def main() -> int:
         return 1
     os.replace(EXE + ".tmp", EXE)
     print(f"{EXE}: {len(objs)} objects")
+    if a.dropin:
+        return build_dropin(objs)
+    return 0
+
+
+def build_dropin(objs: list) -> int:
+    lib = os.path.join(PKG, "libflosic_hip.so")
+    if not os.path.exists(lib):
+        print(f"{lib} is missing: build the library first")
+        return 1
+    steps = [[FC, "-O2", "-J" + OBJ, "-I" + OBJ, "-c", os.path.join(REPO, "include", "flosic_hip.f90"), "-o",
+              os.path.join(OBJ, "flosic_hip_iface.o")],
+             [FC, "-O2", "-cpp", "-J" + OBJ, "-I" + OBJ, "-I" + REF, "-c",
+              os.path.join(PKG, "fortran", "flosic_dropin.F90"), "-o", os.path.join(OBJ, "flosic_dropin.o")]]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode:
+            print(r.stderr[-4000:])
+            return 1
+    keep = [os.path.join(OBJ, o) for o in objs if o not in REPLACED]
+    rpath = os.path.relpath(PKG, OUT)
+    cmd = [FC, "-O2", "-o", DROPIN_EXE + ".tmp"] + keep + \
+          [os.path.join(OBJ, "flosic_hip_iface.o"), os.path.join(OBJ, "flosic_dropin.o"), "-L" + PKG, "-lflosic_hip",
+           "-Wl,-rpath,$ORIGIN/" + rpath, "-L" + MKL, "-Wl,-rpath," + MKL, "-lmkl_intel_lp64", "-lmkl_sequential",
+           "-lmkl_core", "-lpthread", "-lm", "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        print(r.stderr[-4000:])
+        return 1
+    os.replace(DROPIN_EXE + ".tmp", DROPIN_EXE)
+    print(f"{DROPIN_EXE}: {len(keep)} reference objects + the library's drop-ins")
     return 0
 
 

@@ -15,9 +15,15 @@ Cases:
               polarised LDA-PW91 DFT SCF; then FLO-SIC with 5+5 FODs (core at O, two bonds at 0.6 of O->H, two lone
               pairs 0.9 bohr from O perpendicular to the plane, seeded jitter sigma 0.02 bohr: system.make_config(1))
   h2o_pbe     the same with GGA-PBE*GGA-PBE
+  h2o_lsda_scf  the reference's FLO-SIC SCF (SIC block every iteration, SCFSICW / DIAGGE_FO consuming the SIC
+              matrix) from the converged LDA state of h2o_lsda with its FODs: the SUMMARY trace of every iteration,
+              the last ZSIC and the final fforce.dat
+  h2o_lsda_fodloop  the reference's frozen-density FOD optimisation (FOD_LOOPV) on the same state: energies per
+              optimiser step (fande.out), FODs and forces per step (records), the final FRMORB and fforce.dat
 
 Kept per case in tests/golden/reference_runs/<case>/: SYMBOL, ISYMGEN, FRMORB, WFOUT.gz, VMOLD.gz (the reference's
-files as written), fforce.dat, and reference.json with the numbers read from the reference's output: ZSIC (full
+files as written), fforce.dat, for the H2O cases the SCF's inputs (inputs/: CLUSTER, GRPMAT, NRLMOL_INPUT.DAT), and
+reference.json with the numbers read from the reference's output: ZSIC (full
 precision, 'MAIN : ZSIC ='), the ORBITAL SIC lines of each spin (RESULTS(1:9) in F10.6), the printed SIC(j,i)
 matrix per spin (F12.5), the SCF's total energy and the counts.
 
@@ -124,6 +130,9 @@ def make_case(case: str):
         fods = h2o_inputs("GGA-PBE*GGA-PBE", scf).fod
     else:
         raise ValueError(case)
+    for f in ("CLUSTER", "GRPMAT", "NRLMOL_INPUT.DAT"):
+        if os.path.exists(os.path.join(scf, f)):
+            shutil.copy(os.path.join(scf, f), os.path.join(base, "inputs_" + f))
     run(scf)
     etot = scf_energy(scf)
     shutil.copytree(scf, sic)
@@ -143,19 +152,84 @@ def make_case(case: str):
     os.makedirs(dst)
     for f in ("SYMBOL", "ISYMGEN", "fforce.dat"):
         shutil.copy(os.path.join(sic, f), dst)
+    if fods is not None:  # the SCF's own inputs, so the whole run can be repeated where the reference is absent
+        os.makedirs(os.path.join(dst, "inputs"))
+        for f in ("CLUSTER", "GRPMAT", "NRLMOL_INPUT.DAT"):
+            shutil.copy(os.path.join(base, "inputs_" + f), os.path.join(dst, "inputs", f))
     open(os.path.join(dst, "FRMORB"), "w").write(frm)
     for f in ("WFOUT", "VMOLD"):
-        with open(os.path.join(sic, f), "rb") as fi, gzip.open(os.path.join(dst, f + ".gz"), "wb", 9) as fo:
+        with open(os.path.join(sic, f), "rb") as fi, open(os.path.join(dst, f + ".gz"), "wb") as raw, \
+                gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as fo:
             fo.write(fi.read())
     json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
     print(f"{case}: ZSIC {res['zsic']!r}, SCF total energy {etot!r}, {res['nmsh']} mesh points, FODs {res['nfod']}")
 
 
+def make_scf_case(case: str):
+    """FLO-SIC SCF of the config-1 H2O (the reference's SCF loop with its SIC block every iteration, started from the
+    converged LDA state of h2o_lsda): the per-iteration SUMMARY trace and the final FOD forces."""
+    from publicrelease_2020_amd import formats as FM
+    src = os.path.join(RUNS, "h2o_lsda", "scf")
+    if not os.path.isdir(src):
+        make_case("h2o_lsda")
+    d = os.path.join(RUNS, case)
+    shutil.rmtree(d, ignore_errors=True)
+    shutil.copytree(src, d)
+    shutil.copy(os.path.join(OUT, "h2o_lsda", "FRMORB"), d)
+    text = run(d)
+    lines = open(os.path.join(d, "SUMMARY")).read().splitlines()
+    last = max(i for i, ln in enumerate(lines) if ln.split()[:1] == ["IT"])  # the SCF appends a block per run
+    rows = [[float(v) for v in ln.split()] for ln in lines[last + 1:] if ln.strip()]
+    res = dict(case=case, summary_columns=["IT", "TRACE", "ETOT", "EKIN+ENONLOC", "CHARGE", "EDFT+SIC", "LOWEST"],
+               summary=rows, zsic=float(re.findall(r"MAIN : ZSIC =\s*(\S+)", text)[-1].replace("D", "E")))
+    dst = os.path.join(OUT, case)
+    shutil.rmtree(dst, ignore_errors=True)
+    os.makedirs(dst)
+    shutil.copy(os.path.join(d, "fforce.dat"), dst)
+    json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
+    print(f"{case}: {len(rows)} SCF iterations, final EDFT+SIC {rows[-1][5]!r}, last ZSIC {res['zsic']!r}")
+
+
+def make_fodloop_case(case: str):
+    """The reference's frozen-density FOD optimisation of the config-1 H2O (FOD_LOOPV = 'Y' with NONSCFV: its
+    electronic_geometry optimiser moving the FODs on the converged LDA orbitals, one SIC evaluation with forces per
+    step, main.F:862-888): the trajectory in fande.out (energy per step) and records (FODs and forces per step)."""
+    src = os.path.join(RUNS, "h2o_lsda", "scf")
+    if not os.path.isdir(src):
+        make_case("h2o_lsda")
+    d = os.path.join(RUNS, case)
+    shutil.rmtree(d, ignore_errors=True)
+    shutil.copytree(src, d)
+    shutil.copy(os.path.join(OUT, "h2o_lsda", "FRMORB"), d)
+    for key in ("NONSCFV", "NONSCFFORCESV"):
+        set_option(os.path.join(d, "NRLMOL_INPUT.DAT"), key, "'Y'")
+    txt = open(os.path.join(d, "NRLMOL_INPUT.DAT")).read()
+    txt = txt.replace("&end", "FOD_LOOPV     = 'Y' ! frozen-density FOD optimisation\n&end")
+    open(os.path.join(d, "NRLMOL_INPUT.DAT"), "w").write(txt)
+    text = run(d)
+    fande = [ln.split() for ln in open(os.path.join(d, "fande.out")).read().splitlines() if ln.strip()]
+    res = dict(case=case, zsic=[float(v.replace("D", "E")) for v in re.findall(r"MAIN : ZSIC =\s*(\S+)", text)],
+               fande=[[float(x) for x in r] for r in fande])
+    dst = os.path.join(OUT, case)
+    shutil.rmtree(dst, ignore_errors=True)
+    os.makedirs(dst)
+    for f in ("fforce.dat", "FRMORB", "records"):
+        shutil.copy(os.path.join(d, f), dst)
+    json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
+    print(f"{case}: {len(res['zsic'])} SIC evaluations, {len(fande)} optimiser steps, last ZSIC {res['zsic'][-1]!r}")
+
+
 def main(argv) -> int:
     if not os.path.exists(EXE):
         subprocess.check_call([sys.executable, os.path.join(ROOT, "oracle", "build_nrlmol.py")])
-    for case in argv or ["hydrogen", "neon", "h2o_lsda", "h2o_pbe"]:
-        make_case(case)
+    for case in argv or ["hydrogen", "neon", "h2o_lsda", "h2o_pbe", "h2o_lsda_scf",
+                                 "h2o_lsda_fodloop"]:
+        if case == "h2o_lsda_scf":
+            make_scf_case(case)
+        elif case == "h2o_lsda_fodloop":
+            make_fodloop_case(case)
+        else:
+            make_case(case)
     return 0
 
 

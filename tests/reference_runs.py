@@ -21,6 +21,8 @@ def load(case: str):
     d = tempfile.mkdtemp(prefix=f"flosic_{case}_")
     try:
         for f in os.listdir(src):
+            if os.path.isdir(os.path.join(src, f)):
+                continue  # inputs/: the SCF's own inputs, not part of the evaluated run directory
             if f.endswith(".gz"):
                 with gzip.open(os.path.join(src, f)) as fi, open(os.path.join(d, f[:-3]), "wb") as fo:
                     fo.write(fi.read())
