@@ -107,7 +107,17 @@ def parse(text: str, nup: int, ndn: int) -> dict:
             if len(vals) >= n * n:
                 break
         sic[str(s)] = [vals[i * n:(i + 1) * n] for i in range(n)]
-    return dict(zsic=float(z[-1].replace("D", "E")), orbital_sic=[rows[:nup], rows[nup:nup + ndn]], sic_matrix=sic)
+    # the energy summary of the evaluation's density (main.F's SUMMARY OF ENERGY CONTRIBUTIONS, F-format 6 decimals):
+    # exchange-correlation (local + nonlocal parts) and the mean-field Coulomb (Hartree) energy of the spin densities
+    def last(key):
+        v = re.findall(rf"^{key}:\s*(\S+)", text, flags=re.M)
+        return float(v[-1]) if v else None
+    energies = {k: last(label) for k, label in (("exc", "EXCHANGE-CORRELATION"), ("ex_local", "LOCAL EXCHANGE"),
+                                                 ("ec_local", "LOCAL CORRELATION"), ("ex_nonlocal", "NONLOCAL EXCHANGE"),
+                                                 ("ec_nonlocal", "NONLOCAL CORRELATION"),
+                                                 ("hartree", "MEAN-FIELD COULOMB"))}
+    return dict(zsic=float(z[-1].replace("D", "E")), orbital_sic=[rows[:nup], rows[nup:nup + ndn]], sic_matrix=sic,
+                energies=energies)
 
 
 def scf_energy(directory: str) -> float:

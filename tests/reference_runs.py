@@ -57,3 +57,16 @@ def check(esic, results, sic, debdax, ref, tol_e=1e-9, tol_f=1e-9):
     assert dres < 6e-6, out   # F11.6 on the reference's side
     assert dsic < 6e-6, out   # F12.5
     return out
+
+
+def check_dft_energies(e, ref, gga):
+    """The DFT-side energies of the evaluation's spin densities ({ex, ec, eh} from APOTNL_DFT's kernels) against the
+    reference's SUMMARY OF ENERGY CONTRIBUTIONS of the same density (F-format, 6 decimals): exchange and correlation
+    (local plus nonlocal parts for PBE) and the mean-field Coulomb energy."""
+    r = ref["energies"]
+    want = dict(ex=r["ex_local"] + r["ex_nonlocal"], ec=r["ec_local"] + r["ec_nonlocal"], eh=r["hartree"],
+                exc=r["exc"])
+    got = dict(ex=e["ex"], ec=e["ec"], eh=e["eh"], exc=e["ex"] + e["ec"])
+    d = {k: abs(got[k] - want[k]) for k in want}
+    assert max(d.values()) < 1.1e-6, (d, got, want)
+    return d

@@ -8,7 +8,7 @@ and forces (|dE| < 1e-6 Ha, |dF| < 1e-5 Ha/bohr)."""
 import numpy as np
 import pytest
 
-from reference_runs import CASES, check, load
+from reference_runs import CASES, check, check_dft_energies, load
 
 pytestmark = pytest.mark.gpu
 
@@ -44,3 +44,19 @@ def test_reduced_forms_within_gate_of_reference(case, prec):
     dE, dF = abs(e - ref["zsic"]), float(np.abs(deb - ref["fforce"][:deb.shape[0]]).max())
     print(f"{case} form {prec}: |dE| {dE:.3e} Ha, max |dF| {dF:.3e} Ha/bohr", flush=True)
     assert dE < 1e-6 and dF < 1e-5
+
+
+@pytest.mark.parametrize("case", ["neon", "h2o_lsda", "h2o_pbe"])
+def test_dft_side_energies_match_reference_run(case):
+    """flosic_apotnl_dft on the evaluation's densities against the energies the reference program printed for the same
+    density (exchange, correlation, mean-field Coulomb; its linked SUBVLXC, subvlxc_libxc.F90)."""
+    from publicrelease_2020_amd import flosic as F
+    sysm, ref = load(case)
+    ctx = F.Flosic(0)
+    try:
+        ctx.load_system(sysm)
+        ctx.evaluate()
+        _, e = ctx.apotnl_dft()
+    finally:
+        ctx.close()
+    print(case, check_dft_energies(e, ref, bool(sysm.xc)), flush=True)

@@ -40,3 +40,37 @@ def test_loaded_orbitals_are_orthonormal():
         Smat = S.overlap_matrix(sysm.basis)
         for C in sysm.coef:
             assert np.abs(C.T @ Smat @ C - np.eye(C.shape[1])).max() < 1e-9
+
+
+def _spin_density(phi, C, n, gga):
+    """(value, gradient, Hessian xx,yy,zz,xy,xz,yz) of sum_j psi_j^2 over the first n orbitals, from basis values
+    phi[d, point, function] with d over value, gradient and Hessian components."""
+    psi = np.einsum("dmb,bj->dmj", phi, C[:, :n])
+    rho = (psi[0] ** 2).sum(1)
+    if not gga:
+        return rho
+    out = np.zeros((rho.size, 10))
+    out[:, 0] = rho
+    for a in range(3):
+        out[:, 1 + a] = 2.0 * (psi[0] * psi[1 + a]).sum(1)
+    for k, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+        out[:, 4 + k] = 2.0 * ((psi[1 + a] * psi[1 + b]).sum(1) + (psi[0] * psi[4 + k]).sum(1))
+    return out
+
+
+@pytest.mark.parametrize("case", ["neon", "h2o_lsda", "h2o_pbe"])
+def test_oracle_dft_side_energies_match_reference_run(case):
+    """The oracle's DFT-side potential (SUBVLXC with two spins, oracle_dft_lsda / oracle_dft_gga) on the total spin
+    densities of the evaluated orbitals, with the exact Hartree potential, against the energies the reference program
+    printed for the same density (its linked SUBVLXC, subvlxc_libxc.F90)."""
+    from oracle import oracle as O
+    from reference_runs import check_dft_energies
+    sysm, ref = load(case)
+    gga = bool(sysm.xc)
+    phi = O.basis_eval(sysm.basis, sysm.mesh.xyz, 2 if gga else 0)
+    r = O.evaluate(sysm, nthreads=8, want_potentials=True, want_vab=False)
+    vh = r["spins"][0]["vh"].sum(0) + r["spins"][1]["vh"].sum(0)
+    ru = _spin_density(phi, sysm.coef[0], sysm.nocc[0], gga)
+    rd = _spin_density(phi, sysm.coef[1], sysm.nocc[1], gga)
+    _, e = (O.dft_gga if gga else O.dft_lsda)(ru, rd, sysm.mesh.w, vh)
+    print(case, check_dft_energies(e, ref, gga))
