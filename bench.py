@@ -180,11 +180,31 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 
         ctx.close()
 
 
-def cpu_baseline(sysm, npts_sample: int, threads: int) -> dict:
-    """The CPU FP64 oracle on a bounded sample of the same workload: all FLOs and pairs, every m-th mesh point
-    (weights x m).  The oracle splits its wall time into mesh-independent work (Fermi orbitals, the Hermite
-    expansion of every pair for every FLO, forces) and work proportional to the points; one eval is
-    estimated as fixed + per-point x M."""
+def host_threads() -> int:
+    """Host threads this process may use (its CPU affinity), at most 16 (the GPU box's CPU share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sysm, npts_sample: int, threads: int, nsamples: int = 3) -> dict:
+    """The CPU FP64 oracle on bounded samples of the same workload: all FLOs and pairs on every m-th mesh point
+    (weights x m), for nsamples disjoint strided samples (offsets 0, m/nsamples, ...).  The oracle splits its wall
+    time into mesh-independent work (Fermi orbitals, the Hermite expansion of every pair for every FLO, forces) and
+    work proportional to the points; each sample estimates one eval as fixed + per-point x M.  The value is the mean
+    estimate; the spread over the samples is reported with it."""
     import numpy as np
 
     from oracle import oracle as O
@@ -192,19 +212,109 @@ def cpu_baseline(sysm, npts_sample: int, threads: int) -> dict:
 
     M = sysm.mesh.npts
     step = max(1, M // npts_sample)
-    idx = np.arange(0, M, step)
-    sub = System(name=sysm.name, basis=sysm.basis, mesh=Mesh(xyz=sysm.mesh.xyz[idx].copy(), w=sysm.mesh.w[idx] * step),
-                 coef=sysm.coef, nocc=sysm.nocc, fod=sysm.fod, xc=sysm.xc, nspn=sysm.nspn)
-    t0 = time.perf_counter()
-    r = O.evaluate(sub, nthreads=threads, want_vab=True)
-    dt = time.perf_counter() - t0
-    fixed, mesh = r["stage_s"]["fixed"], r["stage_s"]["mesh"]
-    est = fixed + mesh * M / idx.size
-    return {"value": 1.0 / est, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"oracle_eval (FP64 C/OpenMP) on {idx.size} of {M} mesh points (every {step}th, weights "
-                      f"x{step}), all {sum(sysm.nocc)} FLOs, {sysm.nbas} basis functions; {dt:.1f} s measured = "
-                      f"{fixed:.1f} s mesh-independent + {mesh:.1f} s over the sample points; one eval estimated "
-                      f"as {fixed:.1f} + {mesh:.1f} x {M}/{idx.size} = {est:.0f} s"}
+    est, walls, fixed, per_pt = [], [], [], []
+    for k in range(nsamples):
+        idx = np.arange((k * step) // nsamples, M, step)
+        sub = System(name=sysm.name, basis=sysm.basis,
+                     mesh=Mesh(xyz=sysm.mesh.xyz[idx].copy(), w=sysm.mesh.w[idx] * step),
+                     coef=sysm.coef, nocc=sysm.nocc, fod=sysm.fod, xc=sysm.xc, nspn=sysm.nspn)
+        t0 = time.perf_counter()
+        r = O.evaluate(sub, nthreads=threads, want_vab=True)
+        walls.append(time.perf_counter() - t0)
+        f, m = r["stage_s"]["fixed"], r["stage_s"]["mesh"]
+        fixed.append(f)
+        per_pt.append(m / idx.size)
+        est.append(f + m * M / idx.size)
+        log(f"CPU sample {k}: {idx.size} points, {walls[-1]:.1f} s -> one eval {est[-1]:.0f} s")
+    mean = float(np.mean(est))
+    return {"value": 1.0 / mean, "unit": "evals/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "estimate_s": {"mean": mean, "min": float(min(est)), "max": float(max(est)),
+                           "per_sample": [float(e) for e in est]},
+            "sample": f"oracle_eval (FP64 C/OpenMP, exact Coulomb at every point) on {nsamples} disjoint strided "
+                      f"samples of {M // step} of the {M} mesh points (every {step}th, weights x{step}), all "
+                      f"{sum(sysm.nocc)} FLOs and shell pairs, {sysm.nbas} basis functions; {sum(walls):.1f} s "
+                      f"measured; one eval estimated per sample as {np.mean(fixed):.1f} s mesh-independent + "
+                      f"{np.mean(per_pt) * 1e3:.1f} ms per point x {M}: {mean:.0f} s "
+                      f"(samples {min(est):.0f}..{max(est):.0f} s)"}
+
+
+def reference_program_leg(steps: int) -> dict:
+    """The reference program itself (nrlmol_exe, built from the reference sources, serial) against the library on one
+    input: the config-1 H2O (NRLMOL default basis, the reference's own 65238-point mesh and converged LDA orbitals,
+    5+5 FODs, tests/golden/reference_runs/h2o_lsda).  The reference's LDA SCF runs first (CPU) to make the state its
+    non-SCF FLO-SIC evaluation starts from; then the same evaluation (NONSCF with forces) by the reference (its own
+    'MAIN TIME FOR SIC' timer: CPU seconds of the SIC block on one core) and by the reference linked with the
+    library's drop-ins (GPU); and the library alone on the run directory read by formats.load_run (FP64 and
+    split-BF16x3, wall time per evaluation).  Skipped where the programs were not built."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    nrl = os.path.join(ROOT, "oracle", "_ref", "nrlmol")
+    fix = os.path.join(ROOT, "tests", "golden", "reference_runs", "h2o_lsda")
+    if not (os.path.exists(os.path.join(nrl, "nrlmol_exe")) and os.path.exists(os.path.join(nrl, "nrlmol_flosic_hip"))):
+        return {"skipped": "the reference program was not built (needs the reference sources at build time)"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from reference_runs import load
+
+    from publicrelease_2020_amd import flosic as F
+    tmp = tempfile.mkdtemp(prefix="flosic_ref_")
+    out = {"input": "config 1: H2O, NRLMOL default basis (73 functions), the reference's 65238-point mesh and "
+                    "converged LDA orbitals, 5+5 FODs", "cpu_model": cpu_model(), "reference_cores": 1}
+    try:
+        scf = os.path.join(tmp, "scf")
+        shutil.copytree(os.path.join(fix, "inputs"), scf, copy_function=shutil.copyfile)
+
+        def run(exe, d):
+            t0 = time.perf_counter()
+            with open(os.path.join(d, "out.txt"), "w") as f:
+                subprocess.run([os.path.join(nrl, exe)], cwd=d, stdout=f, stderr=subprocess.STDOUT, timeout=600)
+            return open(os.path.join(d, "out.txt")).read(), time.perf_counter() - t0
+
+        _, dt = run("nrlmol_exe", scf)
+        log(f"reference program: LDA SCF {dt:.1f} s")
+        for name, exe in (("reference", "nrlmol_exe"), ("dropin", "nrlmol_flosic_hip")):
+            d = os.path.join(tmp, name)
+            shutil.copytree(scf, d, copy_function=shutil.copyfile)
+            shutil.copyfile(os.path.join(fix, "FRMORB"), os.path.join(d, "FRMORB"))
+            p = os.path.join(d, "NRLMOL_INPUT.DAT")
+            txt = open(p).read()
+            for key in ("NONSCFV", "NONSCFFORCESV"):
+                txt = re.sub(rf"^({key}\s*=\s*)'N'", r"\1'Y'", txt, flags=re.M)
+            open(p, "w").write(txt)
+            text, dt = run(exe, d)
+            z = re.findall(r"MAIN : ZSIC =\s*(\S+)", text)
+            t_sic = re.findall(r"MAIN TIME FOR SIC :\s*(\S+)", text)
+            out[name] = {"zsic": float(z[-1]) if z else None, "wall_s": dt,
+                         "sic_block_cpu_s": float(t_sic[-1]) if t_sic else None}
+            log(f"reference program ({name}): ZSIC {out[name]['zsic']}, SIC block {out[name]['sic_block_cpu_s']} s, "
+                f"wall {dt:.1f} s")
+        sysm, ref = load("h2o_lsda")
+        for prec, pname in ((F.PREC_FP64, "fp64"), (F.PREC_BF16X3, "bf16x3")):
+            ctx = F.Flosic(0)
+            try:
+                ctx.load_system(sysm, prec)
+                ctx.evaluate()
+                t0 = time.perf_counter()
+                for _ in range(max(1, steps)):
+                    e = ctx.evaluate()
+                dt = (time.perf_counter() - t0) / max(1, steps)
+            finally:
+                ctx.close()
+            out[f"library_{pname}"] = {"eval_ms": 1e3 * dt, "zsic": e, "dE_vs_reference": abs(e - ref["zsic"])}
+            log(f"library {pname} on the same input: {1e3 * dt:.2f} ms per evaluation, |dE| vs reference "
+                f"{abs(e - ref['zsic']):.2e} Ha")
+        t_ref = out["reference"]["sic_block_cpu_s"]
+        if t_ref:
+            out["speedup_sic_block_vs_reference"] = {
+                k: t_ref / (out[f"library_{k}"]["eval_ms"] * 1e-3) for k in ("fp64", "bf16x3")}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
 
 
 def copy_calibration(mib: int, reps: int) -> int:
@@ -232,8 +342,8 @@ def bench_batch(args) -> int:
     """The batch workload: n_sys independent systems from the batch generator (K uniform in [4,96], LSDA or PBE
     at random, 4..128 FODs per spin), every context in mesh-block mode.  Timed: the batch in the reduced form
     `--batch-prec` with its device part captured once and replayed (the reported value), the same systems
-    evaluated one at a time, and each system in the plain FP64 form (the accuracy reference; first evaluation of
-    a fresh context, so it includes its buffer allocations)."""
+    evaluated one at a time, and each system in the plain FP64 form (the accuracy reference; timed on its second
+    evaluation, so no buffer allocation is inside the timed region)."""
     import numpy as np
 
     from publicrelease_2020_amd import flosic as F
@@ -288,12 +398,14 @@ def bench_batch(args) -> int:
     batch.close()
     for e in mixed:
         e.close()
-    # plain FP64 form, one system at a time: accuracy reference and speed comparison
+    # plain FP64 form, one system at a time: accuracy reference and speed comparison, timed on a warm context (the
+    # second evaluation: buffers allocated by the first, as the batch's contexts are when it is timed)
     t_fp64, z64, dF = 0.0, [], 0.0
     for i, sysm in enumerate(systems):
         e = F.Flosic(0)
         e.load_system(sysm, F.PREC_FP64)
         e.set_mesh_block(args.batch_block)
+        e.evaluate()
         t1 = time.perf_counter()
         z64.append(e.evaluate())
         t_fp64 += time.perf_counter() - t1
@@ -410,10 +522,14 @@ def main() -> int:
     ap.add_argument("--coulomb-cols", type=int, default=0,
                     help="FLO columns per Coulomb block: 0 widest, -1 autotuned, or 64/128/192/256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256, help="mesh points in the CPU oracle sample")
-    ap.add_argument("--workload", default="precision", choices=("precision", "batch", "adjoint"),
+    ap.add_argument("--cpu-sample", type=int, default=128, help="mesh points in each CPU oracle sample")
+    ap.add_argument("--cpu-samples", type=int, default=3, help="disjoint strided samples of the mesh for the CPU oracle")
+    ap.add_argument("--no-reference-program", action="store_true",
+                    help="skip the reference program (nrlmol_exe) against the library on the config-1 H2O inputs")
+    ap.add_argument("--workload", default="precision", choices=("precision", "batch", "adjoint", "reference"),
                     help="precision: one system, FP64 and reduced forms; batch: many systems in one batch; "
-                         "adjoint: K=32 PBE gradient, recomputation and memory")
+                         "adjoint: K=32 PBE gradient, recomputation and memory; reference: only the reference program "
+                         "against the library on the config-1 H2O inputs")
     ap.add_argument("--adjoint-block", type=int, default=65536, help="mesh-block points of the adjoint workload")
     ap.add_argument("--fd-coords", type=int, default=6, help="FOD coordinates checked by central differences")
     ap.add_argument("--batch-systems", type=int, default=48)
@@ -432,6 +548,9 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "reference":
+        print(json.dumps({"reference_program": reference_program_leg(args.steps)}), flush=True)
+        return 0
 
     import numpy as np
 
@@ -508,9 +627,11 @@ def main() -> int:
         "next_rows": ref["next_rows"],
     }
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"CPU oracle sample on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(sysm, args.cpu_sample, threads)
+        threads = host_threads()
+        log(f"CPU oracle samples on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(sysm, args.cpu_sample, threads, args.cpu_samples)
+    if not args.no_reference_program:
+        out["reference_program"] = reference_program_leg(args.steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -161,11 +161,11 @@ def make_case(case: str):
     shutil.rmtree(dst, ignore_errors=True)
     os.makedirs(dst)
     for f in ("SYMBOL", "ISYMGEN", "fforce.dat"):
-        shutil.copy(os.path.join(sic, f), dst)
+        shutil.copyfile(os.path.join(sic, f), os.path.join(dst, f))
     if fods is not None:  # the SCF's own inputs, so the whole run can be repeated where the reference is absent
         os.makedirs(os.path.join(dst, "inputs"))
         for f in ("CLUSTER", "GRPMAT", "NRLMOL_INPUT.DAT"):
-            shutil.copy(os.path.join(base, "inputs_" + f), os.path.join(dst, "inputs", f))
+            shutil.copyfile(os.path.join(base, "inputs_" + f), os.path.join(dst, "inputs", f))
     open(os.path.join(dst, "FRMORB"), "w").write(frm)
     for f in ("WFOUT", "VMOLD"):
         with open(os.path.join(sic, f), "rb") as fi, open(os.path.join(dst, f + ".gz"), "wb") as raw, \
@@ -195,7 +195,7 @@ def make_scf_case(case: str):
     dst = os.path.join(OUT, case)
     shutil.rmtree(dst, ignore_errors=True)
     os.makedirs(dst)
-    shutil.copy(os.path.join(d, "fforce.dat"), dst)
+    shutil.copyfile(os.path.join(d, "fforce.dat"), os.path.join(dst, "fforce.dat"))
     json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
     print(f"{case}: {len(rows)} SCF iterations, final EDFT+SIC {rows[-1][5]!r}, last ZSIC {res['zsic']!r}")
 
@@ -224,7 +224,7 @@ def make_fodloop_case(case: str):
     shutil.rmtree(dst, ignore_errors=True)
     os.makedirs(dst)
     for f in ("fforce.dat", "FRMORB", "records"):
-        shutil.copy(os.path.join(d, f), dst)
+        shutil.copyfile(os.path.join(d, f), os.path.join(dst, f))
     json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
     print(f"{case}: {len(res['zsic'])} SIC evaluations, {len(fande)} optimiser steps, last ZSIC {res['zsic'][-1]!r}")
 

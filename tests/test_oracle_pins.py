@@ -447,3 +447,32 @@ def test_sic_hamiltonian_projects_onto_the_sic_matrix(tiny_system, tiny_oracle):
         d = np.array([[vab[i, j, i] for i in range(n)] for j in range(n)])  # d[j, i] = <phi_j|V_i|phi_i>
         want = 0.5 * (d @ ovl + (d @ ovl).T)
         np.testing.assert_allclose(cf.T @ H @ cf, want, atol=1e-12 * np.abs(want).max())
+
+
+def test_results_slot5_xexx_closed_form():
+    """RESULTS(5) (XEXX, apotnl_sic.F:571,580: 0.75 (6/pi)^(1./3.) sum_k w_k |rho_k|^(4./3.) with REAL*4 exponents)
+    for a one-Gaussian FLO, against the closed form of the integral: rho = N^2 exp(-2 a r^2) with N^2 = (2a/pi)^(3/2),
+    int rho^p d^3r = N^(2p) (pi / (2 a p))^(3/2).  Also RESULTS(2) (charge) = 1 and RESULTS(4) (ASY, the outermost
+    point's Coulomb potential times its distance) = 1 to the mesh quadrature."""
+    import math
+
+    from oracle import oracle as O
+    from publicrelease_2020_amd import system as S
+    a = 0.8
+    xyz = np.zeros((1, 3))
+    b = S.Basis(xyz=xyz, Z=np.array([1], np.int32), bare_off=np.array([0, 1], np.int32), alpha=np.array([a]),
+                ncon=np.array([[1, 0, 0]], np.int32), con_off=np.zeros((1, 3), np.int32),
+                con=np.array([S.prim_norm(a, 0)]))
+    mesh = S.becke_mesh(xyz, np.random.default_rng(2), nrad=160, nang=302)
+    sysm = S.System(name="gauss", basis=b, mesh=mesh, coef=[np.ones((1, 1)), np.ones((1, 1))], nocc=[1, 0],
+                    fod=[np.array([[0.3, 0.0, 0.0]]), np.zeros((0, 3))], xc=S.XC_LSDA)
+    r = O.evaluate(sysm, nthreads=4)
+    assert r["status"] == 0
+    res = r["spins"][0]["results"][0]
+    p = float(np.float32(4.0) / np.float32(3.0))       # 4./3. in single precision
+    third = float(np.float32(1.0) / np.float32(3.0))   # 1./3.
+    n2 = (2.0 * a / math.pi) ** 1.5
+    want = 0.75 * (6.0 / math.pi) ** third * n2 ** p * (math.pi / (2.0 * a * p)) ** 1.5
+    assert abs(res[4] - want) < 1e-8 * abs(want), (res[4], want)
+    assert abs(res[1] - 1.0) < 1e-9
+    assert abs(res[3] - 1.0) < 1e-6
