@@ -18,8 +18,9 @@ Cases:
   h2o_lsda_scf  the reference's FLO-SIC SCF (SIC block every iteration, SCFSICW / DIAGGE_FO consuming the SIC
               matrix) from the converged LDA state of h2o_lsda with its FODs: the SUMMARY trace of every iteration,
               the last ZSIC and the final fforce.dat
-  h2o_lsda_fodloop  the reference's frozen-density FOD optimisation (FOD_LOOPV) on the same state: energies per
-              optimiser step (fande.out), FODs and forces per step (records), the final FRMORB and fforce.dat
+  h2o_lsda_fodopt  the reference's FOD optimisation as its users run it: repeated non-SCF evaluations with forces
+              in one directory, each ending in its FOD optimiser (ELECTRONIC_GEOMETRY, electronic_geometry_lbfgs.f),
+              which writes the next FRMORB; per step the evaluated FODs, ZSIC, fforce.dat and the fande.out line
 
 Kept per case in tests/golden/reference_runs/<case>/: SYMBOL, ISYMGEN, FRMORB, WFOUT.gz, VMOLD.gz (the reference's
 files as written), fforce.dat, for the H2O cases the SCF's inputs (inputs/: CLUSTER, GRPMAT, NRLMOL_INPUT.DAT), and
@@ -200,44 +201,50 @@ def make_scf_case(case: str):
     print(f"{case}: {len(rows)} SCF iterations, final EDFT+SIC {rows[-1][5]!r}, last ZSIC {res['zsic']!r}")
 
 
-def make_fodloop_case(case: str):
-    """The reference's frozen-density FOD optimisation of the config-1 H2O (FOD_LOOPV = 'Y' with NONSCFV: its
-    electronic_geometry optimiser moving the FODs on the converged LDA orbitals, one SIC evaluation with forces per
-    step, main.F:862-888): the trajectory in fande.out (energy per step) and records (FODs and forces per step)."""
+FODOPT_STEPS = 6
+
+
+def make_fodopt_case(case: str):
+    """FOD optimisation of the config-1 H2O by the reference's own optimiser, FODOPT_STEPS steps: each step is one run
+    of the program (NONSCF with forces on the converged LDA state of h2o_lsda) in the same directory; the run's
+    ELECTRONIC_GEOMETRY (FOD_OPT, its state in the directory's optimiser files) writes the FRMORB of the next step."""
+    from publicrelease_2020_amd import formats as FM
     src = os.path.join(RUNS, "h2o_lsda", "scf")
     if not os.path.isdir(src):
         make_case("h2o_lsda")
     d = os.path.join(RUNS, case)
     shutil.rmtree(d, ignore_errors=True)
     shutil.copytree(src, d)
-    shutil.copy(os.path.join(OUT, "h2o_lsda", "FRMORB"), d)
+    shutil.copyfile(os.path.join(OUT, "h2o_lsda", "FRMORB"), os.path.join(d, "FRMORB"))
     for key in ("NONSCFV", "NONSCFFORCESV"):
         set_option(os.path.join(d, "NRLMOL_INPUT.DAT"), key, "'Y'")
-    txt = open(os.path.join(d, "NRLMOL_INPUT.DAT")).read()
-    txt = txt.replace("&end", "FOD_LOOPV     = 'Y' ! frozen-density FOD optimisation\n&end")
-    open(os.path.join(d, "NRLMOL_INPUT.DAT"), "w").write(txt)
-    text = run(d)
-    fande = [ln.split() for ln in open(os.path.join(d, "fande.out")).read().splitlines() if ln.strip()]
-    res = dict(case=case, zsic=[float(v.replace("D", "E")) for v in re.findall(r"MAIN : ZSIC =\s*(\S+)", text)],
-               fande=[[float(x) for x in r] for r in fande])
+    steps = []
+    for k in range(FODOPT_STEPS):
+        up, dn = FM.read_frmorb(open(os.path.join(d, "FRMORB")).read())
+        text = run(d)
+        steps.append(dict(fod_up=up.tolist(), fod_dn=dn.tolist(),
+                          zsic=float(re.findall(r"MAIN : ZSIC =\s*(\S+)", text)[-1].replace("D", "E")),
+                          fforce=FM.read_fforce(open(os.path.join(d, "fforce.dat")).read()).tolist()))
+    fande = [[float(x) for x in ln.split()] for ln in open(os.path.join(d, "fande.out")).read().splitlines()
+             if ln.strip()]
+    up, dn = FM.read_frmorb(open(os.path.join(d, "FRMORB")).read())
+    res = dict(case=case, steps=steps, fande=fande, next_fod_up=up.tolist(), next_fod_dn=dn.tolist())
     dst = os.path.join(OUT, case)
     shutil.rmtree(dst, ignore_errors=True)
     os.makedirs(dst)
-    for f in ("fforce.dat", "FRMORB", "records"):
-        shutil.copyfile(os.path.join(d, f), os.path.join(dst, f))
     json.dump(res, open(os.path.join(dst, "reference.json"), "w"), indent=1)
-    print(f"{case}: {len(res['zsic'])} SIC evaluations, {len(fande)} optimiser steps, last ZSIC {res['zsic'][-1]!r}")
+    print(f"{case}: {len(steps)} optimiser steps, E {fande[0][1]!r} -> {fande[-1][1]!r}")
 
 
 def main(argv) -> int:
     if not os.path.exists(EXE):
         subprocess.check_call([sys.executable, os.path.join(ROOT, "oracle", "build_nrlmol.py")])
     for case in argv or ["hydrogen", "neon", "h2o_lsda", "h2o_pbe", "h2o_lsda_scf",
-                                 "h2o_lsda_fodloop"]:
+                                 "h2o_lsda_fodopt"]:
         if case == "h2o_lsda_scf":
             make_scf_case(case)
-        elif case == "h2o_lsda_fodloop":
-            make_fodloop_case(case)
+        elif case == "h2o_lsda_fodopt":
+            make_fodopt_case(case)
         else:
             make_case(case)
     return 0

@@ -13,8 +13,11 @@ Then the reference's FLO-SIC SCF itself (SCF-ONLY with the FODs: the SIC block e
 matrix to the Hamiltonian and DIAGGE_FO diagonalising it, scfsicw.f:290-404) runs with the drop-ins from the same
 converged LDA state: every iteration's EDFT+SIC in SUMMARY follows the reference's own run
 (tests/golden/reference_runs/h2o_lsda_scf) to 1e-6 Ha, with the same iteration count, the final energy to 1e-7 Ha and
-the final FOD forces to 1e-6 Ha/bohr.  Skipped where the programs were not built (they need the reference sources at
-build time)."""
+the final FOD forces to 1e-6 Ha/bohr.
+
+Finally the reference's FOD optimisation (repeated non-SCF evaluations, each ending in its FOD optimiser) runs with the
+drop-ins for six steps: the FODs of every step, ZSIC and forces follow the reference's own trajectory to 1e-8.
+Skipped where the programs were not built (they need the reference sources at build time)."""
 import os
 import re
 import shutil
@@ -119,3 +122,35 @@ def test_reference_flosic_scf_with_gpu_dropins(lda_state, tmp_path):
     ff = FM.read_fforce(open(d / "fforce.dat").read())
     ffr = FM.read_fforce(open(os.path.join(fix, "fforce.dat")).read())
     assert np.abs(ff - ffr).max() < 1e-6
+
+
+@pytest.mark.skipif(not BUILT, reason=SKIP)
+def test_reference_fod_optimisation_with_gpu_dropins(lda_state, tmp_path):
+    """The reference's FOD optimisation as its users run it (repeated NONSCF evaluations with forces in one directory,
+    each ending in ELECTRONIC_GEOMETRY, which writes the next FRMORB) with the drop-ins: the FODs evaluated at every
+    step, ZSIC and the forces follow the reference's own trajectory (tests/golden/reference_runs/h2o_lsda_fodopt)."""
+    import json
+
+    from publicrelease_2020_amd import formats as FM
+    fix = os.path.join(ROOT, "tests", "golden", "reference_runs", "h2o_lsda_fodopt")
+    ref = json.load(open(os.path.join(fix, "reference.json")))
+    d = tmp_path / "fodopt"
+    shutil.copytree(lda_state, d, copy_function=shutil.copyfile)
+    shutil.copyfile(os.path.join(FIX, "FRMORB"), d / "FRMORB")
+    _nonscf(str(d))
+    worst = dict(fod=0.0, zsic=0.0, force=0.0)
+    for k, want in enumerate(ref["steps"]):
+        up, dn = FM.read_frmorb(open(d / "FRMORB").read())
+        text, dt = _run(os.path.join(NRLMOL, "nrlmol_flosic_hip"), str(d), 300)
+        z = float(re.findall(r"MAIN : ZSIC =\s*(\S+)", text)[-1])
+        ff = FM.read_fforce(open(d / "fforce.dat").read())
+        worst["fod"] = max(worst["fod"], float(np.abs(up - np.array(want["fod_up"])).max()),
+                           float(np.abs(dn - np.array(want["fod_dn"])).max()))
+        worst["zsic"] = max(worst["zsic"], abs(z - want["zsic"]))
+        worst["force"] = max(worst["force"], float(np.abs(ff - np.array(want["fforce"])).max()))
+        print(f"FOD step {k}: ZSIC {z!r} (reference {want['zsic']!r}), {dt:.1f} s", flush=True)
+    print(f"drop-in FOD optimisation, {len(ref['steps'])} steps: max |dFOD| {worst['fod']:.2e} bohr, max |dZSIC| "
+          f"{worst['zsic']:.2e} Ha, max |dF| {worst['force']:.2e} Ha/bohr", flush=True)
+    assert worst["fod"] < 1e-8 and worst["zsic"] < 1e-8 and worst["force"] < 1e-8
+    fande = np.array([[float(x) for x in ln.split()] for ln in open(d / "fande.out").read().splitlines() if ln.strip()])
+    np.testing.assert_allclose(fande, np.array(ref["fande"]), rtol=0, atol=1e-8)
