@@ -811,25 +811,41 @@ int build_pairs(flosic_ctx *c) {
                              (p.a * p.A[2] + p.b * p.B[2]) / pe, pe);
     }
     c->nrows = row;
-    // chunk lists: consecutive groups of one L, at most `cap_rows` rows, zero-extended to a multiple of `quant`
-    auto make_chunks = [&](int cap_rows, int quant, int64_t &padded) {
+    // chunk lists: consecutive groups of one L, at most `cap_rows` rows, zero-extended to a multiple of `quant`.  With
+    // split_l4 (the split-BF16 list, 32-row steps) the L = 4 groups (35 rows) go into two chunk types instead: chunk
+    // type 5 holds the first 32 rows of two groups, type 6 the last 3 rows of 21 groups; a whole 35-row group would
+    // pad its 64-row chunk with 29 zero rows.
+    auto make_chunks = [&](int cap_rows, int quant, int64_t &padded, bool split_l4) {
         std::vector<ChunkDev> out;
         padded = 0;
+        auto emit = [&](int k, int e, int type, int rpg) {
+            const int kr = ((e - k) * rpg + quant - 1) / quant * quant;
+            out.push_back(ChunkDev{k, e, type, kr, pairs[k].row, (int)padded});
+            padded += kr;
+        };
         for (int k = 0; k < c->ngroup;) {
             const int L = pairs[k].lA + pairs[k].lB;
             const int nt = (L + 1) * (L + 2) * (L + 3) / 6;
+            if (split_l4 && L == 4) {
+                int end = k;
+                while (end < c->ngroup && pairs[end].lA + pairs[end].lB == 4) ++end;
+                for (int type = 5; type <= 6; ++type) {
+                    const int rpg = type == 5 ? 32 : 3, cap = std::max(1, cap_rows / rpg);
+                    for (int b = k; b < end; b += cap) emit(b, std::min(end, b + cap), type, rpg);
+                }
+                k = end;
+                continue;
+            }
             const int cap = std::max(1, cap_rows / nt);
             int e = k;
             while (e < c->ngroup && e - k < cap && pairs[e].lA + pairs[e].lB == L) ++e;
-            const int kr = ((e - k) * nt + quant - 1) / quant * quant;
-            out.push_back(ChunkDev{k, e, L, kr, pairs[k].row, (int)padded});
-            padded += kr;
+            emit(k, e, L, nt);
             k = e;
         }
         return out;
     };
-    const std::vector<ChunkDev> chunks = make_chunks(kChunkRows, 4, c->rows_padded);
-    const std::vector<ChunkDev> chunks16 = make_chunks(kChunkRowsBf16, 32, c->rows_padded_bf16);
+    const std::vector<ChunkDev> chunks = make_chunks(kChunkRows, 4, c->rows_padded, false);
+    const std::vector<ChunkDev> chunks16 = make_chunks(kChunkRowsBf16, 32, c->rows_padded_bf16, true);
     c->nchunks = (int)chunks.size();
     c->nchunks_bf16 = (int)chunks16.size();
     // segment offsets: segL[L] = first group with lA+lB >= L

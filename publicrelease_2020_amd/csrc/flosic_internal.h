@@ -45,6 +45,8 @@ struct PairDev {
 constexpr int kChunkRows = 64;      // FP64 / FP32 MFMA kernels: K step 4
 constexpr int kChunkRowsBf16 = 64;  // split-BF16 kernel: K step 32
 struct ChunkDev {
+    // groups [pb, pe); L: their Hermite order, or in the split-BF16 list 5 / 6 for the first 32 / last 3 rows of
+    // L = 4 groups; kr: rows after zero extension; hrow0: first Hermite row of group pb
     int pb, pe, L, kr, hrow0;
     int prow0;  // first row of the chunk in the zero-extended row space of its list (split-BF16 H layout)
 };

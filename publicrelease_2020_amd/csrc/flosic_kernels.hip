@@ -198,6 +198,12 @@ void launch_prim_coef(int nps, const int *ps_f0, const int *ps_stride, const int
 // descending (t+u+v = L')
 // ---------------------------------------------------------------------------------------------------
 __host__ __device__ constexpr int ntuv(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+// Chunk types of the split-BF16 chunk list: L = 0..4 whole groups of that L; L = 5 the first 32 Hermite rows of L = 4
+// groups, L = 6 their last 3 rows (a 35-row group would otherwise fill a 64-row chunk with 29 zero rows).  Rows per
+// group in the chunk, offset of those rows inside the group's Hermite block, and the group's Hermite order:
+__host__ __device__ constexpr int chunk_rpg(int L) { return L == 5 ? 32 : L == 6 ? 3 : ntuv(L); }
+__host__ __device__ constexpr int chunk_roff(int L) { return L == 6 ? 32 : 0; }
+__host__ __device__ constexpr int chunk_herm(int L) { return L > 4 ? 4 : L; }
 __host__ __device__ constexpr int tuv_idx(int t, int u, int v) {
     // position inside order block: enumerate t from L' down, u from L'-t down
     int L = t + u + v;
@@ -542,11 +548,14 @@ template <> struct MfmaTraits<float> {
 template <typename T>
 using RType = typename std::conditional<sizeof(T) == 8, double, float>::type;
 
-template <int L, typename T, int LD, int LAYOUT, int NP>
+// RPG rows per group from Hermite row ROFF on (the whole group by default; the split-BF16 list's L = 4 head / tail
+// chunks take rows [0, 32) / [32, 35) of each group, see chunk_rpg)
+template <int L, typename T, int LD, int LAYOUT, int NP, int RPG = ntuv(L), int ROFF = 0>
 __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, double py, double pz, bool valid, int q,
                                         int nq, const ChunkDev &ch, const double4 *__restrict__ pP,
                                         const RType<T> *__restrict__ ctab, T *Rl) {
     constexpr int nt = ntuv(L);
+    static_assert(ROFF + RPG <= nt, "chunk rows beyond the group's Hermite block");
     using C = RType<T>;
     const int np = ch.pe - ch.pb;
     const int glo = np * q / nq, ghi = np * (q + 1) / nq;
@@ -560,15 +569,15 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
             for (int t = 0; t < nt; ++t) R[t] = C(0.0);
         }
 #pragma unroll
-        for (int t = 0; t < nt; ++t) {
+        for (int t = 0; t < RPG; ++t) {
             if constexpr (LAYOUT == 0)
-                Rl[(g * nt + t) * LD + lane] = (T)R[t];
+                Rl[(g * RPG + t) * LD + lane] = (T)R[ROFF + t];
             else
-                Rl[lane * LD + g * nt + t] = (T)R[t];
+                Rl[lane * LD + g * RPG + t] = (T)R[ROFF + t];
         }
     }
     if (q == 0)
-        for (int r = np * nt + rw; r < ch.kr; r += nrw) {
+        for (int r = np * RPG + rw; r < ch.kr; r += nrw) {
             if constexpr (LAYOUT == 0)
                 Rl[r * LD + lane] = (T)0;
             else
@@ -585,7 +594,15 @@ __device__ __forceinline__ void produce_R(int rw, int nrw, int lane, double px, 
         case 1: chunk_R<1, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
         case 2: chunk_R<2, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
         case 3: chunk_R<3, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
-        default: chunk_R<4, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        case 4: chunk_R<4, T, LD, LAYOUT, NP>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP, tab, Rl); break;
+        case 5:
+            chunk_R<4, T, LD, LAYOUT, NP, chunk_rpg(5), chunk_roff(5)>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP,
+                                                                      tab, Rl);
+            break;
+        default:
+            chunk_R<4, T, LD, LAYOUT, NP, chunk_rpg(6), chunk_roff(6)>(rw, nrw, lane, px, py, pz, valid, q, nq, ch, pP,
+                                                                      tab, Rl);
+            break;
     }
 }
 
@@ -778,11 +795,14 @@ template <int NP>
 __global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__restrict__ H, int ldh, int nF,
                           int64_t nsteps, __bf16 *__restrict__ Hs) {
     const ChunkDev ch = chunks[blockIdx.x];
-    const int nreal = (ch.pe - ch.pb) * ntuv(ch.L);
+    const int rpg = chunk_rpg(ch.L), nt = ntuv(chunk_herm(ch.L)), roff = chunk_roff(ch.L);
+    const int nreal = (ch.pe - ch.pb) * rpg;
     const int nct = ldh / 16;
     for (int e = threadIdx.x; e < ldh * ch.kr; e += blockDim.x) {
         const int f = e / ch.kr, r = e - f * ch.kr;
-        const double v = (r < nreal && f < nF) ? H[(int64_t)(ch.hrow0 + r) * ldh + f] : 0.0;
+        // chunk row r: row roff + r % rpg of group r / rpg, whose Hermite block starts nt rows after the previous one
+        const int64_t hr = ch.hrow0 + (int64_t)(r / rpg) * nt + roff + r % rpg;
+        const double v = (r < nreal && f < nF) ? H[hr * ldh + f] : 0.0;
         const int64_t R = ch.prow0 + r;
         const int64_t idx = ((R / 32) * nct + f / 16) * 512 + (16 * ((R % 32) / 8) + f % 16) * 8 + R % 8;
         split_bf16<NP>(v, Hs + idx, nsteps * nct * 512);
