@@ -93,11 +93,11 @@ int32_t flosic_evaluate(flosic_ctx *ctx, double *zsic);
 int32_t flosic_apotnl_sic(flosic_ctx *ctx, int32_t ispn, double *results, double *zsic);
 
 /* The DFT-side potential of the last evaluation's densities (APOTNL_DFT with NSPN = 2: getvlxc.F -> SUBVLXC,
- * subvlxc.F:138-323): rho_s = sum of the FLO densities of spin s (an orthonormal rotation of the occupied
- * orbitals, occupations 1), V_H = the exact electronic Hartree potential of rho_up + rho_dn (the sum of the FLO
- * Hartree potentials); the terms APOTNL_DFT adds beyond it (nuclear and external potentials, EXTPOT and the field
- * term, apotnl_dft.F) are not included: the context holds no nuclear charges.  LSDA: exchange per spin at 2 rho_s (PW91EX, s = u = v = 0) and PW92
- * correlation (PW91LC) at the point's zeta.  PBE: exchange per spin at 2 rho_s (PBEEX with the s, u, v of 2 rho_s)
+ * subvlxc_libxc.F90:197-470, the SUBVLXC the program links): rho_s = sum of the FLO densities of spin s (an
+ * orthonormal rotation of the occupied orbitals, occupations 1), V_H = the exact electronic Hartree potential of
+ * rho_up + rho_dn (the sum of the FLO Hartree potentials); the terms APOTNL_DFT adds beyond it (nuclear and external
+ * potentials, EXTPOT and the field term, apotnl_dft.F) are not included: the context holds no nuclear charges.
+ * LSDA: exchange per spin at 2 rho_s (PW91EX, s = u = v = 0) and PW92 correlation (PW91LC) at the point's zeta.  PBE: exchange per spin at 2 rho_s (PBEEX with the s, u, v of 2 rho_s)
  * and PBECOR at the point's zeta with t, uu, vv of the total density and the zeta-gradient term ww.  pot
  * (nmsh x 2, may be NULL): V_H + V_xc of spin up, then spin down (the POTOUT layout); energies (may be NULL):
  * {E_x, E_c, E_H = 1/2 int rho V_H, N_up, N_dn} on the mesh.  NSPN = 2 only (otherwise FLOSIC_EINVAL); in

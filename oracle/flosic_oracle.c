@@ -20,7 +20,7 @@
  *   FOD force formula .......... frmorb2.F:589-717 (including the undoubled overlap derivative in its first term)
  *   exact FOD gradient ......... back-propagation of dE/dTMAT = 2 (SIC^T TMAT) through the Lowdin step
  *   LAGMULT_SLV, APATCH ........ lagmult_slv.F:2-187; sicham.F:920-945, sicmat.F:146-262
- *   DFT-side potential ......... subvlxc.F:138-323 with NSPN = 2 (LSDA and PBE), plus the electronic Hartree part
+ *   DFT-side potential ......... subvlxc_libxc.F90:197-470 (= subvlxc.F:138-323 without libxc) with NSPN = 2 (LSDA and PBE), plus the electronic Hartree part
  *   SICLAG_DER (DERSIC) ........ siclag_der.F:186-640 without point-group symmetry and libxc/meta-GGA terms
  *
  * Deliberate, documented differences from the reference text:
@@ -1479,7 +1479,7 @@ void oracle_dft_lsda(int n, const double *rup, const double *rdn, const double *
     }
 }
 
-/* The DFT-side PBE potential of the total spin densities: SUBVLXC with NSPN = 2 and IGGA = 1 (subvlxc.F:138-323).
+/* The DFT-side PBE potential of the total spin densities: SUBVLXC with NSPN = 2 and IGGA = 1 (subvlxc_libxc.F90:197-470, the routine the program links; subvlxc.F:138-323 is its non-libxc branch as a separate file, the one driven for the golden vectors).
  * rgu, rgd: 10 x n (value, gradient, Hessian xx,yy,zz,xy,xz,yz) of rho_up and rho_dn.  As there, RHT(:,1) holds the
  * total density and RHT(:,2) the down density; their values are clipped to [0, total]; |grad|^2, grad.H.grad and the
  * Laplacian are formed for the total density and for each spin; exchange per spin at D = 2 rho_s with PBEEX(s,u,v) of

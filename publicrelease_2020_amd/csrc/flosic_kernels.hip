@@ -1662,7 +1662,7 @@ __device__ __forceinline__ void gga_invariants(const double *r, double &dg2, dou
     dlp = r[4] + r[5] + r[6];
 }
 
-// DFT-side PBE potential of the total spin densities (APOTNL_DFT with NSPN = 2 and IGGA = 1, subvlxc.F:138-323): per
+// DFT-side PBE potential of the total spin densities (APOTNL_DFT with NSPN = 2 and IGGA = 1: SUBVLXC as the program links it, subvlxc_libxc.F90:197-470; subvlxc.F:138-323 has the same non-libxc branch): per
 // point the value, gradient and Hessian of rho_up and rho_dn from the FLO planes (rho = sum phi^2,
 // grad rho = 2 sum phi grad phi, H = 2 sum (grad phi grad phi^T + phi H phi)); the total and down densities clipped
 // as there; exchange per spin at D = 2 rho_s (PBEEX with s, u, v of 2 rho_s), PBE correlation at the point's zeta with
