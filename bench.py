@@ -433,8 +433,8 @@ def bench_batch(args) -> int:
         "peak_device_bytes": int(peak),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[form][1], "unit": "TFLOP/s",
                      "frac": achieved / PEAK[form][1], "traffic": None,
-                     "kernel": ("k_coulomb_ws_batch<float, *>" if form == "mixed" else "k_coulomb_ws_bf16_batch<*>")
-                     + " (all classes)", "kernel_ms": t_eager[2],
+                     "kernel": ("k_coulomb_ws_batch_all<float>" if form == "mixed" else "k_coulomb_ws_bf16_batch_all<*>"),
+                     "kernel_ms": t_eager[2],
                      "flops_per_launch": flops, "one_at_a_time_kernel_ms": seq_coul},
     }
     print(json.dumps(out), flush=True)

@@ -199,8 +199,8 @@ int32_t flosic_set_mesh_block(flosic_ctx *ctx, int64_t block_points);
 
 /* Batches of independent systems (one context each, all on one device; no counterpart in the reference, whose
  * GROUP mode spreads one system's orbitals over ranks, pasapotnl_grp.F90).  flosic_batch_evaluate is
- * flosic_evaluate of every context, with the Coulomb contractions of all systems as one launch per class of
- * equal precision and column-block width; zsic[i] (may be null) gets ZSIC of ctxs[i], and every per-context
+ * flosic_evaluate of every context, with the Coulomb contractions of all systems of one precision in one launch
+ * (each system at its own column-block width); zsic[i] (may be null) gets ZSIC of ctxs[i], and every per-context
  * getter then reads that context's results.  With use_graph the device part of the evaluation (Hermite
  * coefficients, batched Coulomb, fields, SIC matrix, V^i_ab of every system) is captured into one hipGraph on
  * the second evaluation and replayed on later ones while no context's settings or buffers change (moving FODs
@@ -212,7 +212,8 @@ const char *flosic_batch_last_error(const flosic_batch *batch);
 /* [0] host wall time of the last batch evaluation [1] its device part (batch stream events) [2] the batched
  * Coulomb launches alone (0 when the device part was a graph) [3] 0 eager, 1 captured, 2 replayed; ms */
 int32_t flosic_batch_get_timings(flosic_batch *batch, int32_t n, double *ms);
-/* [0] evaluations [1] graph captures [2] graph replays [3] Coulomb launch classes [4] Coulomb work items */
+/* [0] evaluations [1] graph captures [2] graph replays [3] batched Coulomb launches (one per precision form)
+ * [4] Coulomb work items */
 int32_t flosic_batch_get_counts(flosic_batch *batch, int32_t n, int64_t *counts);
 int32_t flosic_batch_destroy(flosic_batch *batch);
 

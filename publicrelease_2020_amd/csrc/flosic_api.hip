@@ -170,8 +170,8 @@ struct flosic_ctx {
 };
 
 // Many independent systems evaluated together: the Coulomb contractions of all of them run as one launch per
-// (Coulomb form, column block width) class, optionally with the whole device part captured once in a hipGraph
-// and replayed while no context's buffers or settings change.
+// Coulomb form (the systems keep their own column-block widths), optionally with the whole device part captured
+// once in a hipGraph and replayed while no context's buffers or settings change.
 struct flosic_batch {
     MemTally mem;
     int device = 0;
@@ -180,14 +180,17 @@ struct flosic_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_coul = nullptr, ev_end = nullptr, ev_t0 = nullptr, ev_c0 = nullptr,
                ev_c1 = nullptr;
-    // launch plan: per class of equal (form, pieces, column block width) the job tables of its launches
+    // launch plan: per class of equal (form, pieces) one launch over the jobs [job0, job0 + njobs) of d_jobs, whose
+    // item offsets are d_off[off0 .. off0 + njobs] (each class's own, starting at 0); jobs keep their column widths
     struct Class {
-        int form = 0, npieces = 0, nfb = 0;
+        int form = 0, npieces = 0;
         std::vector<int> members;
-        std::vector<CoulJobTable<double>> tables;  // the job layout does not depend on the operand type
-        int64_t nitems = 0;
+        int64_t job0 = 0, off0 = 0, nitems = 0;
+        int njobs = 0;
     };
     std::vector<Class> classes;
+    DevBuf<CoulJob<double>> d_jobs;  // the job layout does not depend on the operand type
+    DevBuf<int64_t> d_off;
     std::vector<uint64_t> plan_sig;  // context signatures the plan was built for
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1446,55 +1449,60 @@ CoulJob<double> coulomb_job(const flosic_ctx *c) {
     J.ldv = c->M;
     J.nsteps = form == COUL_BF16 ? c->rows_padded_bf16 / 32 : c->rows_padded / 4;
     J.ncb = c->ldh / nfb;
+    J.nfb = nfb;
     return J;
 }
 
-// Classes of equal (form, pieces, column block width); members heaviest first (work per item ~ K steps), so
-// that the longest items are dispatched first; kBatchJobs members per launch
+// Classes of equal (form, pieces), each one launch whatever the members' column-block widths; jobs heaviest first
+// (work per item ~ K steps x columns), so that the longest items are dispatched first and the hardware dispatcher
+// balances the tail.  The job records and item offsets go to device memory once per plan.
 int build_plan(flosic_batch *b) {
-    static_assert(sizeof(CoulJobTable<double>) == sizeof(CoulJobTable<float>) &&
-                      sizeof(CoulJobTable<double>) == sizeof(CoulJobTable<__bf16>),
+    static_assert(sizeof(CoulJob<double>) == sizeof(CoulJob<float>) && sizeof(CoulJob<double>) == sizeof(CoulJob<__bf16>),
                   "job layout is independent of the operand type");
-    static_assert(sizeof(CoulJobTable<double>) <= 4096, "job table must fit the kernel argument space");
     b->classes.clear();
     for (int i = 0; i < (int)b->ctxs.size(); ++i) {
         const flosic_ctx *c = b->ctxs[i];
         const CoulForm form = coulomb_form(c);
         if (form == COUL_VALU || c->nchunks <= 0) continue;
         const int np = form == COUL_BF16 ? (c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2) : 0;
-        const int nfb = coulomb_nfb(c);
         auto it = std::find_if(b->classes.begin(), b->classes.end(), [&](const flosic_batch::Class &k) {
-            return k.form == form && k.npieces == np && k.nfb == nfb;
+            return k.form == form && k.npieces == np;
         });
         if (it == b->classes.end()) {
             b->classes.push_back({});
             it = b->classes.end() - 1;
             it->form = form;
             it->npieces = np;
-            it->nfb = nfb;
         }
         it->members.push_back(i);
     }
+    std::vector<CoulJob<double>> all;
+    std::vector<int64_t> offs;
     for (auto &k : b->classes) {
         std::vector<CoulJob<double>> jobs;
         for (int i : k.members) jobs.push_back(coulomb_job(b->ctxs[i]));
-        std::stable_sort(jobs.begin(), jobs.end(),
-                         [](const CoulJob<double> &x, const CoulJob<double> &y) { return x.nsteps > y.nsteps; });
-        k.nitems = 0;
-        for (size_t j0 = 0; j0 < jobs.size(); j0 += kBatchJobs) {
-            CoulJobTable<double> t{};
-            int64_t items = 0;
-            for (size_t j = j0; j < std::min(jobs.size(), j0 + kBatchJobs); ++j) {
-                t.job[t.njobs] = jobs[j];
-                t.off[t.njobs++] = items;
-                items += (jobs[j].M + 63) / 64 * jobs[j].ncb;
-            }
-            t.off[t.njobs] = items;
-            if (items >= (int64_t)INT32_MAX)
-                return bfail(b, FLOSIC_EINVAL, "batch too large: %lld Coulomb items in one launch", (long long)items);
-            k.tables.push_back(t);
-            k.nitems += items;
+        std::stable_sort(jobs.begin(), jobs.end(), [](const CoulJob<double> &x, const CoulJob<double> &y) {
+            return x.nsteps * x.nfb > y.nsteps * y.nfb;
+        });
+        k.job0 = (int64_t)all.size();
+        k.off0 = (int64_t)offs.size();
+        k.njobs = (int)jobs.size();
+        int64_t items = 0;
+        for (const auto &J : jobs) {
+            all.push_back(J);
+            offs.push_back(items);
+            items += (J.M + 63) / 64 * J.ncb;
         }
+        offs.push_back(items);
+        if (items >= (int64_t)INT32_MAX)
+            return bfail(b, FLOSIC_EINVAL, "batch too large: %lld Coulomb items in one launch", (long long)items);
+        k.nitems = items;
+    }
+    if (!all.empty()) {
+        if (!b->d_jobs.ensure(all.size(), &b->mem) || !b->d_off.ensure(offs.size(), &b->mem))
+            return bfail(b, FLOSIC_ENOMEM, "device allocation of the batch job table failed");
+        BHIPCHK(b, hipMemcpy(b->d_jobs.p, all.data(), sizeof(CoulJob<double>) * all.size(), hipMemcpyHostToDevice));
+        BHIPCHK(b, hipMemcpy(b->d_off.p, offs.data(), sizeof(int64_t) * offs.size(), hipMemcpyHostToDevice));
     }
     b->plan_sig = batch_sig(b);
     return 0;
@@ -1503,16 +1511,17 @@ int build_plan(flosic_batch *b) {
 int launch_classes(flosic_batch *b) {
     for (const auto &k : b->classes) {
         const double *btab = b->ctxs[k.members[0]]->d_btab.p;
-        for (const auto &t : k.tables) {
-            if (k.form == COUL_FP64)
-                launch_coulomb_batch<double>(t, k.nfb, btab, b->stream);
-            else if (k.form == COUL_FP32)
-                launch_coulomb_batch<float>(reinterpret_cast<const CoulJobTable<float> &>(t), k.nfb, btab, b->stream);
-            else
-                launch_coulomb_bf16_batch(k.npieces, reinterpret_cast<const CoulJobTable<__bf16> &>(t), k.nfb, btab,
-                                          b->stream);
-            BHIPCHK(b, hipGetLastError());
-        }
+        const CoulJob<double> *jobs = b->d_jobs.p + k.job0;
+        const int64_t *off = b->d_off.p + k.off0;
+        if (k.form == COUL_FP64)
+            launch_coulomb_batch_all<double>(jobs, off, k.njobs, k.nitems, btab, b->stream);
+        else if (k.form == COUL_FP32)
+            launch_coulomb_batch_all<float>(reinterpret_cast<const CoulJob<float> *>(jobs), off, k.njobs, k.nitems,
+                                            btab, b->stream);
+        else
+            launch_coulomb_bf16_batch_all(k.npieces, reinterpret_cast<const CoulJob<__bf16> *>(jobs), off, k.njobs,
+                                          k.nitems, btab, b->stream);
+        BHIPCHK(b, hipGetLastError());
     }
     return 0;
 }

@@ -68,6 +68,7 @@ struct CoulJob {
     int64_t ldv;
     int64_t nsteps;
     int ncb;
+    int nfb;  // FLO columns per block (the single-launch batch kernels switch on it)
 };
 
 // Column-block widths the Coulomb kernels are built for: nfb must divide ldh; the split-BF16 form takes at most 192
@@ -116,20 +117,14 @@ void launch_coulomb_mfma(const ChunkDev *chunks, int nchunks, int64_t nsteps, co
                          int ldh, const double *x, const double *y, const double *z, int64_t M, const double *btab,
                          int nF, double *V, int64_t ldv, hipStream_t s, int nfb = 0);
 
-// Batched form: one launch over the (system, point tile, column block) items of up to kBatchJobs jobs of one
-// Coulomb form and one column block width nfb.  off[j] is the first item of job j (a job has ceil(M/64) * ncb
-// items, tile-major), off[njobs] the total (< 2^31).  The table is passed by value as the kernel argument.
-constexpr int kBatchJobs = 24;
+// Batched form: every job of one Coulomb form in one launch over the (system, point tile, column block) items; the
+// job records and the item offsets in device memory (off[j] the first item of job j, which has ceil(M/64) * ncb items,
+// tile-major; off[njobs] the total, < 2^31), each job with its own column-block width (jobs[j].nfb).
 template <typename T>
-struct CoulJobTable {
-    CoulJob<T> job[kBatchJobs];
-    int64_t off[kBatchJobs + 1];
-    int njobs;
-};
-template <typename T>
-void launch_coulomb_batch(const CoulJobTable<T> &jt, int nfb, const double *btab, hipStream_t s);
-void launch_coulomb_bf16_batch(int npieces, const CoulJobTable<__bf16> &jt, int nfb, const double *btab,
-                               hipStream_t s);
+void launch_coulomb_batch_all(const CoulJob<T> *jobs, const int64_t *off, int njobs, int64_t items,
+                              const double *btab, hipStream_t s);
+void launch_coulomb_bf16_batch_all(int npieces, const CoulJob<__bf16> *jobs, const int64_t *off, int njobs,
+                                   int64_t items, const double *btab, hipStream_t s);
 int coulomb_cols_split(int nF);
 
 // split-BF16 form: R and H split into NP = 2 or 3 bf16 pieces, 3 or 6 cross products per 16x16x32 MFMA step,

@@ -495,12 +495,26 @@ __device__ __forceinline__ int lds_copy_async(const char *src, char *dst, int un
     return n;
 }
 
+// chunk record c with every field read into scalar registers.  The tile code loops and branches on these fields; read
+// through a pointer that the compiler cannot prove unwritten during the kernel (the batch kernels take it from a job
+// record in device memory) they would otherwise be per-lane values, and those loops divergent ones.
+__device__ __forceinline__ ChunkDev chunk_at(const ChunkDev *__restrict__ chunks, int c) {
+    static_assert(sizeof(ChunkDev) % 4 == 0, "chunk record in 32-bit words");
+    constexpr int NW = sizeof(ChunkDev) / 4;
+    const unsigned *src = reinterpret_cast<const unsigned *>(chunks + c);
+    unsigned w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    ChunkDev ch;
+    __builtin_memcpy(&ch, w, sizeof(ch));
+    return ch;
+}
 // position in the chunk / sub-step sequence
 struct SubStep {
     int c, q;
 };
 __device__ __forceinline__ void advance(SubStep &p, const ChunkDev *chunks, int sr) {
-    if (++p.q * sr >= chunks[p.c].kr) {
+    if (++p.q * sr >= chunk_at(chunks, p.c).kr) {
         p.q = 0;
         ++p.c;
     }
@@ -612,7 +626,7 @@ template <typename T, int NFB, int SR>
 __device__ __forceinline__ int stage_h_fp(const ChunkDev *__restrict__ chunks, SubStep p, const T *__restrict__ Hf,
                                           int nct, int fb, T *slot, int cw, int ncw, int lane) {
     constexpr int UPS = NFB * 4 * (int)sizeof(T) / 1024;  // KiB per K step
-    const ChunkDev ch = chunks[p.c];
+    const ChunkDev ch = chunk_at(chunks, p.c);
     const int64_t r0 = ch.prow0 + (int64_t)p.q * SR;
     const int nstep = min(SR, ch.kr - p.q * SR) / 4;
     int n = 0;
@@ -628,20 +642,28 @@ __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | 
 
 // One (point tile, column block) of one system's Coulomb contraction, executed by a whole block.  All waves
 // return from here together; the persistent batch kernel calls it once per work item.
+// LDS of one FP Coulomb tile: the two R tiles and the ring of staged H sub-steps
 template <typename T, int NFB>
-__device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chunks, int nchunks,
+struct FpTileLds {
+    static constexpr int SR = sizeof(T) == 4 ? 32 : (NFB <= 192 ? 16 : 8);  // Hermite rows per sub-step
+    static constexpr int LDR = 64 + (sizeof(T) == 8 ? 8 : 16);
+    T Rl[2][kChunkRows * LDR];
+    __attribute__((aligned(16))) T Hl[kSlots][SR * NFB];
+};
+
+template <typename T, int NFB>
+__device__ __forceinline__ void coulomb_tile_fp(FpTileLds<T, NFB> &L, const ChunkDev *__restrict__ chunks, int nchunks,
                                                 const double4 *__restrict__ pP, const T *__restrict__ Hf, int nct,
                                                 const double *__restrict__ X, const double *__restrict__ Y,
                                                 const double *__restrict__ Z, int64_t M, int nF,
                                                 double *__restrict__ V, int64_t ldv, const double *__restrict__ tab,
                                                 int64_t tile, int cb) {
     constexpr int TP = 64, NRT = 2, NCW = NFB / 64;  // consumer wave: 32 points x NFB/4 columns (TP = wave size)
-    constexpr int SR = sizeof(T) == 4 ? 32 : (NFB <= 192 ? 16 : 8);  // Hermite rows per sub-step
-    constexpr int NRW = 7, NCPW = 1;                                 // R waves, copy waves
-    constexpr int PAD = sizeof(T) == 8 ? 8 : 16;
-    constexpr int LDR = TP + PAD;
-    __shared__ T Rl[2][kChunkRows * LDR];
-    __shared__ __attribute__((aligned(16))) T Hl[kSlots][SR * NFB];
+    constexpr int SR = FpTileLds<T, NFB>::SR;        // Hermite rows per sub-step
+    constexpr int NRW = 7, NCPW = 1;                 // R waves, copy waves
+    constexpr int LDR = FpTileLds<T, NFB>::LDR;
+    T(&Rl)[2][kChunkRows * LDR] = L.Rl;
+    T(&Hl)[kSlots][SR * NFB] = L.Hl;
     using Tr = MfmaTraits<T>;
     using acc_t = typename Tr::acc;
     const int64_t k0 = tile * TP;
@@ -652,14 +674,14 @@ __device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chu
         const bool valid = k < M;
         const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
         const RType<T> *ctab = sizeof(T) == 8 ? (const RType<T> *)tab : (const RType<T> *)(tab + kBoysRows * kBoysCols);
-        produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, ctab, Rl[0]);
+        produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunk_at(chunks, 0), pP, ctab, Rl[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
-            const int nsub = (chunks[c].kr + SR - 1) / SR;
+            const int nsub = (chunk_at(chunks, c).kr + SR - 1) / SR;
             for (int q = 0; q < nsub; ++q) {
                 if (c + 1 < nchunks)
-                    produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, ctab,
-                                            Rl[(c + 1) & 1]);
+                    produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunk_at(chunks, c + 1), pP,
+                                            ctab, Rl[(c + 1) & 1]);
                 __syncthreads();
             }
         }
@@ -681,7 +703,7 @@ __device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chu
         // raw barrier: a __syncthreads() fence would also wait for the copies still in flight
         __builtin_amdgcn_s_barrier();
         for (int c = 0; c < nchunks; ++c) {
-            const int nsub = (chunks[c].kr + SR - 1) / SR;
+            const int nsub = (chunk_at(chunks, c).kr + SR - 1) / SR;
             for (int q = 0; q < nsub; ++q) {
                 int issued = 0;
                 if (nxt.c < nchunks) {
@@ -712,7 +734,7 @@ __device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chu
     __syncthreads();
     int gs = 0;
     for (int c = 0; c < nchunks; ++c) {
-        const int kr = chunks[c].kr;
+        const int kr = chunk_at(chunks, c).kr;
         const int nsub = (kr + SR - 1) / SR;
         const T *R = Rl[c & 1];
         for (int q = 0; q < nsub; ++q, ++gs) {
@@ -762,7 +784,8 @@ __device__ __forceinline__ void coulomb_tile_fp(const ChunkDev *__restrict__ chu
 
 template <typename T, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws(CoulJob<T> J, const double *__restrict__ tab) {
-    coulomb_tile_fp<T, NFB>(J.chunks, J.nchunks, J.pP, (const T *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab,
+    __shared__ FpTileLds<T, NFB> lds;
+    coulomb_tile_fp<T, NFB>(lds, J.chunks, J.nchunks, J.pP, (const T *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab,
                             blockIdx.x, blockIdx.y);
 }
 
@@ -838,7 +861,7 @@ struct HUnits {
     v4u r[UPW];
     __device__ __forceinline__ void load(const ChunkDev *__restrict__ chunks, SubStep p, const __bf16 *__restrict__ Hs,
                                          int64_t nsteps, int nct, int fb, int cw, int lane) {
-        const int64_t g = chunks[p.c].prow0 / 32 + p.q;
+        const int64_t g = chunk_at(chunks, p.c).prow0 / 32 + p.q;
 #pragma unroll
         for (int k = 0; k < UPW; ++k) {
             const int u = cw + k * NCPW;
@@ -858,8 +881,19 @@ struct HUnits {
     }
 };
 
+// LDS of one split-BF16 Coulomb tile: the two FP32 R tiles, the ring of staged H sub-steps, the FP32 Boys table
 template <int NP, int NFB>
-__device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ chunks, int nchunks, int64_t nsteps,
+struct Bf16TileLds {
+    static constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
+    static constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
+    __attribute__((aligned(16))) float Rf[2][64 * LDK];
+    __attribute__((aligned(16))) __bf16 Hl[kSlotsReg][SLOT];
+    __attribute__((aligned(16))) float Tb[kBoysRows * kBoysCols];
+};
+
+template <int NP, int NFB>
+__device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const ChunkDev *__restrict__ chunks,
+                                                  int nchunks, int64_t nsteps,
                                                   const double4 *__restrict__ pP, const __bf16 *__restrict__ Hs,
                                                   int nct, const double *__restrict__ X, const double *__restrict__ Y,
                                                   const double *__restrict__ Z, int64_t M, int nF,
@@ -867,11 +901,11 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
                                                   const double *__restrict__ tab, int64_t tile, int cb) {
     constexpr int TP = 64, NCW = NFB / 32, SR = 32;  // consumer wave: 16 points x NFB/2 columns
     constexpr int NCPW = NP == 3 ? 2 : 1, NRW = kConsumerWave0 - NCPW;
-    constexpr int LDK = kChunkRowsBf16 + 4;  // FP32 point rows of 272 bytes: conflict-free 16-byte reads
-    constexpr int SLOT = NP * NFB * 32;      // bf16 elements of one staged K step
-    __shared__ __attribute__((aligned(16))) float Rf[2][TP * LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 Hl[kSlotsReg][SLOT];
-    __shared__ __attribute__((aligned(16))) float Tb[kBoysRows * kBoysCols];  // FP32 Boys table for the R waves
+    constexpr int LDK = Bf16TileLds<NP, NFB>::LDK;
+    constexpr int SLOT = Bf16TileLds<NP, NFB>::SLOT;
+    float(&Rf)[2][TP * LDK] = L.Rf;
+    __bf16(&Hl)[kSlotsReg][SLOT] = L.Hl;
+    float(&Tb)[kBoysRows * kBoysCols] = L.Tb;  // FP32 Boys table for the R waves
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -884,14 +918,14 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
         const int64_t k = k0 + lane;
         const bool valid = k < M;
         const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
-        produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunks[0], pP, Tb, Rf[0]);
+        produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunk_at(chunks, 0), pP, Tb, Rf[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
-            const int nsub = chunks[c].kr / SR;
+            const int nsub = chunk_at(chunks, c).kr / SR;
             for (int q = 0; q < nsub; ++q) {
                 if (c + 1 < nchunks)
-                    produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunks[c + 1], pP, Tb,
-                                                Rf[(c + 1) & 1]);
+                    produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, q, nsub, chunk_at(chunks, c + 1),
+                                                pP, Tb, Rf[(c + 1) & 1]);
                 __syncthreads();
             }
         }
@@ -911,7 +945,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
         __builtin_amdgcn_s_barrier();
         int gs = 0;
         for (int c = 0; c < nchunks; ++c) {
-            const int nsub = chunks[c].kr / SR;
+            const int nsub = chunk_at(chunks, c).kr / SR;
             for (int q = 0; q < nsub; ++q, ++gs) {
                 if (held) h.store(Hl[(gs + 1) % kSlotsReg], cw, lane);
                 held = nxt.c < nchunks;
@@ -941,7 +975,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
     __syncthreads();
     int gs = 0;
     for (int c = 0; c < nchunks; ++c) {
-        const int nsub = chunks[c].kr / SR;
+        const int nsub = chunk_at(chunks, c).kr / SR;
         const float *R = Rf[c & 1];
         for (int q = 0; q < nsub; ++q, ++gs) {
             const __bf16 *Hs_ = Hl[gs % kSlotsReg];
@@ -990,41 +1024,108 @@ __device__ __forceinline__ void coulomb_tile_bf16(const ChunkDev *__restrict__ c
 
 template <int NP, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(CoulJob<__bf16> J, const double *__restrict__ tab) {
-    coulomb_tile_bf16<NP, NFB>(J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
+    __shared__ Bf16TileLds<NP, NFB> lds;
+    coulomb_tile_bf16<NP, NFB>(lds, J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
                                J.nF, J.V, J.ldv, tab, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Batched Coulomb: one launch over the (system, point tile, column block) items of up to kBatchJobs systems.
-// Block b runs item b with the single-system tile code; the host orders the systems (largest first) so that the
-// dispatcher, which hands blocks to CUs in index order as CUs free up, balances them like a work queue.  One
-// block fills a CU (16 waves at 128 VGPRs), so the grid is the queue and no block loops.  The job table travels
-// in the kernel arguments: loaded from there, every job field stays a scalar (uniform) value in the tile code,
-// exactly as in the single-system kernel (loaded from global memory, the compiler treats the tile's loop
-// bounds as divergent and the kernel runs 20% slower).
+// Batched Coulomb: one launch over the (system, point tile, column block) items of every system of one form.
+// Block b runs item b with the single-system tile code; the host orders the systems (heaviest items first) so
+// that the dispatcher, which hands blocks to CUs in index order as CUs free up, balances them like a work queue.
+// One block fills a CU (16 waves at 128 VGPRs), so the grid is the queue and no block loops.
 // ---------------------------------------------------------------------------------------------------
-template <typename T, int NFB>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_batch(const CoulJobTable<T> jt,
-                                                                   const double *__restrict__ tab) {
-    const unsigned it = blockIdx.x;
-    int j = 0;
-    while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
-    const unsigned local = it - (unsigned)jt.off[j];  // items of one launch: < 2^31
-    const CoulJob<T> &J = jt.job[j];
-    coulomb_tile_fp<T, NFB>(J.chunks, J.nchunks, J.pP, (const T *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab,
-                            local / (unsigned)J.ncb, (int)(local % (unsigned)J.ncb));
+// Every job of a batch form in one launch.  The job records sit in device memory (a batch of 48 systems does not fit
+// the kernel argument space); each block copies its job through readfirstlane word by word, so every field is a
+// scalar (uniform) value in the tile code, as when it comes from the kernel arguments, and the tile's loop bounds
+// stay scalar branches.  Jobs of different column-block widths share the launch: the block switches on its job's
+// width, and the LDS of the widest tile is shared by all widths.
+template <typename T>
+__device__ __forceinline__ CoulJob<T> uniform_job(const CoulJob<T> *__restrict__ jobs, int j) {
+    static_assert(sizeof(CoulJob<T>) % 4 == 0, "job record in 32-bit words");
+    constexpr int NW = sizeof(CoulJob<T>) / 4;
+    const unsigned *src = reinterpret_cast<const unsigned *>(jobs + j);
+    unsigned w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    CoulJob<T> J;
+    __builtin_memcpy(&J, w, sizeof(J));
+    return J;
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(uint64_t)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+// block -> (job, local item): off[] ascending, off[0] = 0
+__device__ __forceinline__ int job_of_block(const int64_t *__restrict__ off, int njobs, int64_t it, int64_t &local) {
+    int lo = 0, hi = njobs - 1;  // largest j with off[j] <= it
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uniform_i64(off[mid]) <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    const int j = __builtin_amdgcn_readfirstlane(lo);
+    local = it - uniform_i64(off[j]);
+    return j;
 }
 
-template <int NP, int NFB>
-__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16_batch(const CoulJobTable<__bf16> jt,
-                                                                        const double *__restrict__ tab) {
-    const unsigned it = blockIdx.x;
-    int j = 0;
-    while (j + 1 < jt.njobs && (int64_t)it >= jt.off[j + 1]) ++j;
-    const unsigned local = it - (unsigned)jt.off[j];
-    const CoulJob<__bf16> &J = jt.job[j];
-    coulomb_tile_bf16<NP, NFB>(J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
-                               J.nF, J.V, J.ldv, tab, local / (unsigned)J.ncb, (int)(local % (unsigned)J.ncb));
+template <typename T>
+constexpr size_t fp_lds_bytes() {
+    size_t b = sizeof(FpTileLds<T, 64>);
+    b = b > sizeof(FpTileLds<T, 128>) ? b : sizeof(FpTileLds<T, 128>);
+    b = b > sizeof(FpTileLds<T, 192>) ? b : sizeof(FpTileLds<T, 192>);
+    b = b > sizeof(FpTileLds<T, 256>) ? b : sizeof(FpTileLds<T, 256>);
+    return b;
+}
+template <typename T>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_batch_all(const CoulJob<T> *__restrict__ jobs,
+                                                                       const int64_t *__restrict__ off, int njobs,
+                                                                       const double *__restrict__ tab) {
+    __shared__ __attribute__((aligned(16))) unsigned char raw[fp_lds_bytes<T>()];
+    int64_t local = 0;
+    const int j = job_of_block(off, njobs, (int64_t)blockIdx.x, local);
+    const CoulJob<T> J = uniform_job(jobs, j);
+    const int64_t tile = local / J.ncb;
+    const int cb = (int)(local - tile * J.ncb);
+#define FLOSIC_TILE_FP(NFB)                                                                                          \
+    coulomb_tile_fp<T, NFB>(*reinterpret_cast<FpTileLds<T, NFB> *>(raw), J.chunks, J.nchunks, J.pP, (const T *)J.H,   \
+                            J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab, tile, cb)
+    switch (J.nfb) {
+        case 64: FLOSIC_TILE_FP(64); break;
+        case 128: FLOSIC_TILE_FP(128); break;
+        case 192: FLOSIC_TILE_FP(192); break;
+        default: FLOSIC_TILE_FP(256); break;
+    }
+#undef FLOSIC_TILE_FP
+}
+
+template <int NP>
+constexpr size_t bf16_lds_bytes() {
+    size_t b = sizeof(Bf16TileLds<NP, 64>);
+    b = b > sizeof(Bf16TileLds<NP, 128>) ? b : sizeof(Bf16TileLds<NP, 128>);
+    b = b > sizeof(Bf16TileLds<NP, 192>) ? b : sizeof(Bf16TileLds<NP, 192>);
+    return b;
+}
+template <int NP>
+__global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16_batch_all(const CoulJob<__bf16> *__restrict__ jobs,
+                                                                            const int64_t *__restrict__ off,
+                                                                            int njobs, const double *__restrict__ tab) {
+    __shared__ __attribute__((aligned(16))) unsigned char raw[bf16_lds_bytes<NP>()];
+    int64_t local = 0;
+    const int j = job_of_block(off, njobs, (int64_t)blockIdx.x, local);
+    const CoulJob<__bf16> J = uniform_job(jobs, j);
+    const int64_t tile = local / J.ncb;
+    const int cb = (int)(local - tile * J.ncb);
+#define FLOSIC_TILE_BF(NFB)                                                                                         \
+    coulomb_tile_bf16<NP, NFB>(*reinterpret_cast<Bf16TileLds<NP, NFB> *>(raw), J.chunks, J.nchunks, J.nsteps, J.pP, \
+                               (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab, tile, cb)
+    switch (J.nfb) {
+        case 64: FLOSIC_TILE_BF(64); break;
+        case 128: FLOSIC_TILE_BF(128); break;
+        default: FLOSIC_TILE_BF(192); break;
+    }
+#undef FLOSIC_TILE_BF
 }
 
 // FLO columns per Coulomb block: all of them up to 256 (ldh is nF rounded up to a multiple of this); the
@@ -1099,39 +1200,29 @@ void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64
 #undef FLOSIC_WSB
 }
 
-template <typename T>
-void launch_coulomb_batch(const CoulJobTable<T> &jt, int nfb, const double *btab, hipStream_t s) {
-    if (jt.njobs <= 0 || jt.off[jt.njobs] <= 0) return;
-    const dim3 grid((unsigned)jt.off[jt.njobs]), blk(kCoulThreads);
-    switch (nfb) {
-        case 64: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 64>), grid, blk, 0, s, jt, btab); break;
-        case 128: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 128>), grid, blk, 0, s, jt, btab); break;
-        case 192: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 192>), grid, blk, 0, s, jt, btab); break;
-        default: hipLaunchKernelGGL((k_coulomb_ws_batch<T, 256>), grid, blk, 0, s, jt, btab); break;
-    }
-}
-template void launch_coulomb_batch<double>(const CoulJobTable<double> &, int, const double *, hipStream_t);
-template void launch_coulomb_batch<float>(const CoulJobTable<float> &, int, const double *, hipStream_t);
 
-void launch_coulomb_bf16_batch(int npieces, const CoulJobTable<__bf16> &jt, int nfb, const double *btab,
-                               hipStream_t s) {
-    if (jt.njobs <= 0 || jt.off[jt.njobs] <= 0) return;
-    const dim3 grid((unsigned)jt.off[jt.njobs]), blk(kCoulThreads);
-#define FLOSIC_WSBB(NP, NFB) hipLaunchKernelGGL((k_coulomb_ws_bf16_batch<NP, NFB>), grid, blk, 0, s, jt, btab)
-#define FLOSIC_WSBB_NFB(NP)                        \
-    switch (nfb) {                                 \
-        case 64: FLOSIC_WSBB(NP, 64); break;       \
-        case 128: FLOSIC_WSBB(NP, 128); break;     \
-        default: FLOSIC_WSBB(NP, 192); break;      \
-    }
-    if (npieces == 2) {
-        FLOSIC_WSBB_NFB(2)
-    } else {
-        FLOSIC_WSBB_NFB(3)
-    }
-#undef FLOSIC_WSBB_NFB
-#undef FLOSIC_WSBB
+template <typename T>
+void launch_coulomb_batch_all(const CoulJob<T> *jobs, const int64_t *off, int njobs, int64_t items,
+                              const double *btab, hipStream_t s) {
+    if (njobs <= 0 || items <= 0) return;
+    hipLaunchKernelGGL((k_coulomb_ws_batch_all<T>), dim3((unsigned)items), dim3(kCoulThreads), 0, s, jobs, off, njobs,
+                       btab);
 }
+template void launch_coulomb_batch_all<double>(const CoulJob<double> *, const int64_t *, int, int64_t,
+                                               const double *, hipStream_t);
+template void launch_coulomb_batch_all<float>(const CoulJob<float> *, const int64_t *, int, int64_t, const double *,
+                                              hipStream_t);
+void launch_coulomb_bf16_batch_all(int npieces, const CoulJob<__bf16> *jobs, const int64_t *off, int njobs,
+                                   int64_t items, const double *btab, hipStream_t s) {
+    if (njobs <= 0 || items <= 0) return;
+    if (npieces == 2)
+        hipLaunchKernelGGL((k_coulomb_ws_bf16_batch_all<2>), dim3((unsigned)items), dim3(kCoulThreads), 0, s, jobs, off,
+                           njobs, btab);
+    else
+        hipLaunchKernelGGL((k_coulomb_ws_bf16_batch_all<3>), dim3((unsigned)items), dim3(kCoulThreads), 0, s, jobs, off,
+                           njobs, btab);
+}
+
 
 // ---------------------------------------------------------------------------------------------------
 // exchange-correlation for rho_up = rho_i, rho_dn = 0, NSPN = 2 (subvlxc_libxc.F90:197-470)

@@ -319,8 +319,8 @@ class Flosic:
 
 
 class FlosicBatch:
-    """Independent systems (Flosic contexts on one device) evaluated together: one batched Coulomb launch per
-    class, optionally the whole device part as a replayed hipGraph."""
+    """Independent systems (Flosic contexts on one device) evaluated together: the Coulomb contractions of all
+    systems of one precision in one launch, optionally the whole device part as a replayed hipGraph."""
 
     def __init__(self, engines, use_graph: bool = False):
         self._L = library()

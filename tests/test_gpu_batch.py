@@ -119,7 +119,7 @@ def test_batch_matches_single_systems():
     for e, r in zip(engines, r_single):
         _compare(_all(e), r, 1e-13)
     c = batch.counts()
-    assert c[0] == 1 and c[3] >= 3  # FP64, FP32 (64-column) and split-bf16 classes
+    assert c[0] == 1 and c[3] == 3  # one Coulomb launch per form: FP64, FP32 and split-bf16
     # a second evaluation reuses the plan and gives the same numbers
     _close(batch.evaluate(), z_single, 1e-13)
 
@@ -202,6 +202,40 @@ def test_coulomb_column_blocks_and_autotuner(prec):
     with pytest.raises(F.FlosicError):
         e.set_coulomb_cols(192)  # does not divide the padded FLO count (128)
         e.evaluate()
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "mixed", "bf16x3"])
+def test_batch_one_launch_mixed_tile_widths(prec):
+    """Systems of one form at different Coulomb tile widths share one batched launch (each job switches to its own
+    width inside the kernel) and give the numbers of their single-system evaluations."""
+    F = _F()
+    big = S.make_synthetic(8, 40, 21, S.XC_LSDA, nrad=24, nang=50, box=5.0)  # 80 FLOs: widths 64 and 128
+    small = S.make_synthetic(3, 3, 7, S.XC_PBE, nrad=24, nang=50, box=5.0)
+    setup = [(big, 128), (small, 64), (big, 64)]
+
+    def engines():
+        out = []
+        for sysm, cols in setup:
+            e = F.Flosic(0)
+            e.load_system(sysm, prec)
+            e.set_coulomb_cols(cols)
+            out.append(e)
+        return out
+
+    single = engines()
+    z_single = np.array([e.evaluate() for e in single])
+    r_single = [_all(e) for e in single]
+    v_single = [e.potential(1, 2)[1] for e in single]
+    assert [e.coulomb_cols() for e in single] == [128, 64, 64]
+    batch = F.FlosicBatch(engines())
+    z = batch.evaluate()
+    c = batch.counts()
+    assert c[3] == 1  # every system in one Coulomb launch
+    _close(z, z_single, 1e-13)
+    for e, r, v in zip(batch.engines, r_single, v_single):
+        _compare(_all(e), r, 1e-13)
+        _close(e.potential(1, 2)[1], v, 1e-13)
+    batch.close()
 
 
 @pytest.mark.parametrize("prec", [2, 3], ids=["bf16x3", "bf16x2"])
