@@ -157,6 +157,11 @@ module flosic_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ncols
     end function
+    integer(c_int32_t) function flosic_set_far_threshold(ctx, tfar) bind(C, name="flosic_set_far_threshold")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tfar
+    end function
     integer(c_int32_t) function flosic_set_mesh_block(ctx, block_points) bind(C, name="flosic_set_mesh_block")
       import :: c_int32_t, c_ptr, c_int64_t
       type(c_ptr), value :: ctx

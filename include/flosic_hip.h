@@ -175,8 +175,9 @@ int32_t flosic_get_timings(flosic_ctx *ctx, int32_t n, double *ms);
  * rows (shell pairs that share both primitive exponents and centres are one group with one row block)
  * [2] nmsh [3] nbas [4] nflo (both spins) [5] exponent-pair groups [6] Hermite rows as the Coulomb kernel of
  * the current precision streams them (chunks zero-extended to its MFMA K step) [7] peak device bytes the
- * context has held since its creation [8] FLO columns per Coulomb block in use (0 for the VALU form).  n <= 9
- * values are written. */
+ * context has held since its creation [8] FLO columns per Coulomb block in use (0 for the VALU form) [9] (64-point
+ * tile, Hermite row) pairs of the split-BF16 stream contracted with two pieces (far chunks, flosic_set_far_threshold;
+ * 0 for the other forms).  n <= 10 values are written. */
 int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
 
 /* Tile shape of the Coulomb kernels: FLO columns per block (each block contracts 64 mesh points with this many
@@ -186,6 +187,15 @@ int32_t flosic_get_counts(flosic_ctx *ctx, int32_t n, int64_t *counts);
  * evaluation, every legal width timed on a strided sample of <= 32768 mesh points and the fastest kept (cached
  * per process by precision, FLO count, Hermite rows and mesh size).  No reference counterpart. */
 int32_t flosic_set_coulomb_cols(flosic_ctx *ctx, int32_t ncols);
+
+/* Far chunks of the three-piece split-BF16 form (FLOSIC_PREC_BF16X3).  A (64-point tile, chunk of exponent-pair
+ * groups) pair whose every (point, group) pair has T = p|r - P|^2 > tfar (p the summed exponent and P the centre of
+ * the pair's Gaussian product, bounded from below through the tile's centroid and radius) is contracted with two
+ * bf16 pieces of R and H (3 products) instead of three (6 products); every other pair keeps three.  tfar <= 0 turns
+ * this off; the default is 30, POISSON1's THIRTYSOMETHING (poisson1.f:101), whose RCUT = sqrt(30/DELTA_MIN) + DMAX
+ * (poisson1.f:528) is where the reference's default mode hands a pair batch to its multipole far field.  The bits
+ * are built once per mesh, basis and threshold.  No reference counterpart. */
+int32_t flosic_set_far_threshold(flosic_ctx *ctx, double tfar);
 
 /* Mesh-block recomputation.  block_points = 0 (default): the FOD-independent basis planes (values, and for PBE
  * gradients and Hessians: 10 planes of nbas x nmsh) stay resident between evaluations and the FLO fields of the

@@ -34,6 +34,10 @@ constexpr int kCpow[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0,
                               {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
 constexpr int kLoff[3] = {0, 1, 4};
 constexpr double kZed = 1.0e-30;  // coupot_sic.F:61
+// three-piece split-BF16 Coulomb: (tile, chunk) pairs whose every T = p|r - P|^2 exceeds this run with two pieces.
+// 30 is POISSON1's THIRTYSOMETHING (poisson1.f:101): beyond RCUT = sqrt(30/DELTA_MIN) + DMAX (poisson1.f:528) the
+// reference replaces the exact pair integrals by its multipole far field.
+constexpr double kFarTDefault = 30.0;
 
 struct Status {
     int code = 0;
@@ -136,6 +140,13 @@ struct flosic_ctx {
     DevBuf<float> d_H32;   // H in FP32 MFMA operand order
     DevBuf<double> d_H64;  // H in FP64 MFMA operand order
     DevBuf<__bf16> d_Hs;
+    // far-chunk bits of the three-piece split-BF16 form (64-point tiles x far_words), FOD-independent: built once per
+    // mesh, basis and threshold; far_rows = streamed (tile, row) pairs of the far chunks
+    DevBuf<unsigned> d_far;
+    DevBuf<double> d_farrows;
+    int far_words = 0;
+    double far_t = kFarTDefault;
+    bool far_ok = false;
     DevBuf<double4> d_pairP;
     DevBuf<double> d_btab;
     // mesh
@@ -166,7 +177,7 @@ struct flosic_ctx {
     bool evaluated = false, fo_done = false;
     bool vab_on_host = false;  // V^i_ab of the last evaluation copied to sp[].vab (on the first request)
     double timings[10] = {};
-    int64_t counts[9] = {};
+    int64_t counts[10] = {};
 };
 
 // Many independent systems evaluated together: the Coulomb contractions of all of them run as one launch per
@@ -1017,11 +1028,14 @@ void coulomb_launch(flosic_ctx *c, const double *x, const double *y, const doubl
             launch_coulomb(c->segL, c->d_pairP.p, c->d_pair_row.p, c->d_H.p, c->ldh, x, y, z, M, c->d_btab.p, nF, V,
                            ldv, c->stream);
             break;
-        case COUL_BF16:
+        case COUL_BF16: {
+            // the far bits describe the tiles of the whole mesh (not the autotuner's point sample)
+            const bool mesh = x == c->d_x.p && M == c->M && c->far_ok && c->far_words > 0;
             launch_coulomb_bf16(c->prec == FLOSIC_PREC_BF16X3 ? 3 : 2, c->d_chunks_bf16.p, c->nchunks_bf16,
                                 c->rows_padded_bf16 / 32, c->d_pairP.p, c->d_Hs.p, c->ldh, x, y, z, M, c->d_btab.p, nF,
-                                V, ldv, c->stream, nfb);
+                                V, ldv, c->stream, nfb, mesh ? c->d_far.p : nullptr, mesh ? c->far_words : 0);
             break;
+        }
         case COUL_FP32:
             launch_coulomb_mfma<float>(c->d_chunks.p, c->nchunks, c->rows_padded / 4, c->d_pairP.p, c->d_H32.p, c->ldh,
                                        x, y, z, M, c->d_btab.p, nF, V, ldv, c->stream, nfb);
@@ -1140,6 +1154,25 @@ int eval_fo(flosic_ctx *c) {
     return 0;
 }
 
+// Far-chunk bits of the three-piece split-BF16 form for the whole mesh: built once per mesh, basis and threshold
+// (they depend on neither the FODs nor the orbitals).  Threshold <= 0, or more chunks than the kernel's LDS row
+// holds, leaves the form at three pieces everywhere.
+int ensure_far(flosic_ctx *c) {
+    if (c->far_ok) return 0;
+    c->far_words = 0;
+    if (c->far_t > 0.0 && c->nchunks_bf16 <= 32 * kFarWords && c->M > 0) {
+        const int w = (c->nchunks_bf16 + 31) / 32;
+        ALLOC(c, c->d_far, (size_t)((c->M + 63) / 64) * w);
+        ALLOC(c, c->d_farrows, 1);
+        launch_far_bits(c->d_x.p, c->d_y.p, c->d_z.p, c->M, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_pairP.p,
+                        c->far_t, w, c->d_far.p, c->d_farrows.p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        c->far_words = w;
+    }
+    c->far_ok = true;
+    return 0;
+}
+
 // Phase A2 (device work only): Hermite pair coefficients H[row][f] of all FLOs and the Coulomb operand of the
 // context's form
 int eval_hermite(flosic_ctx *c) {
@@ -1159,6 +1192,10 @@ int eval_hermite(flosic_ctx *c) {
             ALLOC(c, c->d_Hs, (size_t)np * c->ldh * c->rows_padded_bf16);
             launch_split_h(np, c->d_chunks_bf16.p, c->nchunks_bf16, c->d_H.p, c->ldh, nF, c->rows_padded_bf16 / 32,
                            c->d_Hs.p, c->stream);
+            if (np == 3) {
+                const int r = ensure_far(c);
+                if (r) return r;
+            }
             break;
         }
         case COUL_FP32:
@@ -1384,6 +1421,7 @@ int eval_finish(flosic_ctx *c) {
     c->counts[6] = coulomb_form(c) == COUL_BF16 ? c->rows_padded_bf16 : c->rows_padded;
     c->counts[7] = (int64_t)c->mem.peak;
     c->counts[8] = coulomb_form(c) == COUL_VALU ? 0 : coulomb_nfb(c);
+    c->counts[9] = -1;  // read from the device on request
     c->evaluated = true;
     return 0;
 }
@@ -1450,6 +1488,9 @@ CoulJob<double> coulomb_job(const flosic_ctx *c) {
     J.nsteps = form == COUL_BF16 ? c->rows_padded_bf16 / 32 : c->rows_padded / 4;
     J.ncb = c->ldh / nfb;
     J.nfb = nfb;
+    const bool far = form == COUL_BF16 && c->prec == FLOSIC_PREC_BF16X3 && c->far_ok && c->far_words > 0;
+    J.far = far ? c->d_far.p : nullptr;
+    J.farw = far ? c->far_words : 0;
     return J;
 }
 
@@ -1733,6 +1774,7 @@ int32_t flosic_set_basis(flosic_ctx *c, int32_t natom, const double *xyz, const 
     int r = 0;
     if ((r = build_basis_device(c)) || (r = build_pairs(c))) return r;
     c->planes_ok = false;
+    c->far_ok = false;
     c->lapplane_ok = false;
     c->evaluated = false;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_basis sync");
@@ -1764,6 +1806,7 @@ int32_t flosic_set_mesh(flosic_ctx *c, int64_t nmsh, const double *rmsh, const d
         (r = upload(c, c->d_w, w)))
         return r;
     c->planes_ok = false;
+    c->far_ok = false;
     c->lapplane_ok = false;
     c->evaluated = false;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, FLOSIC_EDEVICE, "set_mesh sync");
@@ -1826,6 +1869,16 @@ int32_t flosic_set_coulomb_cols(flosic_ctx *c, int32_t ncols) {
     c->coul_req = ncols;
     c->coul_nfb = 0;
     c->tuned_key.clear();
+    c->evaluated = false;
+    ++c->cfg_gen;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_set_far_threshold(flosic_ctx *c, double tfar) {
+    if (!c) return FLOSIC_EINVAL;
+    if (!(tfar == tfar)) return fail(c, FLOSIC_EINVAL, "far threshold is NaN");
+    c->far_t = tfar;
+    c->far_ok = false;
     c->evaluated = false;
     ++c->cfg_gen;
     return FLOSIC_OK;
@@ -2257,7 +2310,15 @@ int32_t flosic_get_timings(flosic_ctx *c, int32_t n, double *ms) {
 
 int32_t flosic_get_counts(flosic_ctx *c, int32_t n, int64_t *counts) {
     if (!c || !counts || n < 0) return FLOSIC_EINVAL;
-    for (int i = 0; i < n && i < 9; ++i) counts[i] = c->counts[i];
+    if (n > 9) {
+        double rows = 0.0;
+        const bool far = coulomb_form(c) == COUL_BF16 && c->prec == FLOSIC_PREC_BF16X3 && c->far_ok &&
+                         c->far_words > 0;
+        if (far && hipMemcpy(&rows, c->d_farrows.p, sizeof rows, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(c, FLOSIC_EDEVICE, "flosic_get_counts: far-row count");
+        c->counts[9] = (int64_t)rows;
+    }
+    for (int i = 0; i < n && i < 10; ++i) counts[i] = c->counts[i];
     return FLOSIC_OK;
 }
 

@@ -44,6 +44,7 @@ struct PairDev {
 // zero-extended to kr rows (a multiple of the MFMA K step of the kernel that uses the chunk list)
 constexpr int kChunkRows = 64;      // FP64 / FP32 MFMA kernels: K step 4
 constexpr int kChunkRowsBf16 = 64;  // split-BF16 kernel: K step 32
+constexpr int kFarWords = 1000;     // far-chunk bits per tile the split-BF16 kernel holds in LDS (32000 chunks)
 struct ChunkDev {
     // groups [pb, pe); L: their Hermite order, or in the split-BF16 list 5 / 6 for the first 32 / last 3 rows of
     // L = 4 groups; kr: rows after zero extension; hrow0: first Hermite row of group pb
@@ -69,6 +70,9 @@ struct CoulJob {
     int64_t nsteps;
     int ncb;
     int nfb;  // FLO columns per block (the single-launch batch kernels switch on it)
+    // three-piece split-BF16 form: far-chunk bits of every 64-point tile (farw words per tile), or null
+    const unsigned *far;
+    int farw;
 };
 
 // Column-block widths the Coulomb kernels are built for: nfb must divide ldh; the split-BF16 form takes at most 192
@@ -134,7 +138,14 @@ void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const doub
                     int64_t nsteps, __bf16 *Hs, hipStream_t s);
 void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
                          const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
-                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb = 0);
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb = 0,
+                         const unsigned *far = nullptr, int farw = 0);
+// Far-chunk bits for the three-piece split-BF16 form: bit c of word far[t * farw + c / 32] is set when every (point,
+// group) pair of 64-point tile t and chunk c of the split-BF16 chunk list has T = p|r - P|^2 > tfar (a lower bound
+// from the tile's centroid and radius); farrows[0] (zeroed here) accumulates the rows of the far (tile, chunk) pairs
+void launch_far_bits(const double *x, const double *y, const double *z, int64_t M, const ChunkDev *chunks,
+                     int nchunks, const double4 *pairP, double tfar, int farw, unsigned *far, double *farrows,
+                     hipStream_t s);
 
 // XC + total potential + energy partials over M points: phi nd planes of [nF][M] (plane stride nF*M), Vtot and U
 // [nF][M], VH[f * ldvh + k], w[k]; 256-point block b writes partial[(f * nblk + blk0 + b) * 9 + q].  For PBE,

@@ -495,6 +495,27 @@ __device__ __forceinline__ int lds_copy_async(const char *src, char *dst, int un
     return n;
 }
 
+// Loads and stores through a pointer typed as global memory.  The batch kernels read their arrays' addresses from a
+// job table in device memory, and to the compiler such an address is generic: accesses through it become flat
+// instructions, which also count against the LDS counter, so a copy wave's wait for its LDS stores would drain its
+// global loads in flight as well.  The Coulomb tiles load and store through these (with scalar or ext_vector
+// types: a class type such as double4 would be copied through its generic-address copy constructor).
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+// Load of read-only data at a wave-uniform address through the constant address space: a scalar load (lgkm
+// counter), which leaves the wave's vector loads in flight.  A vector load there would need a vmcnt wait before
+// its value could be made uniform, draining the copy waves' H loads with it.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 // chunk record c with every field read into scalar registers.  The tile code loops and branches on these fields; read
 // through a pointer that the compiler cannot prove unwritten during the kernel (the batch kernels take it from a job
 // record in device memory) they would otherwise be per-lane values, and those loops divergent ones.
@@ -504,7 +525,7 @@ __device__ __forceinline__ ChunkDev chunk_at(const ChunkDev *__restrict__ chunks
     const unsigned *src = reinterpret_cast<const unsigned *>(chunks + c);
     unsigned w[NW];
 #pragma unroll
-    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(cload(src + i));
     ChunkDev ch;
     __builtin_memcpy(&ch, w, sizeof(ch));
     return ch;
@@ -574,7 +595,7 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
     const int np = ch.pe - ch.pb;
     const int glo = np * q / nq, ghi = np * (q + 1) / nq;
     for (int g = glo + rw; g < ghi; g += nrw) {
-        const double4 P = pP[ch.pb + g];
+        const v4d P = gload(reinterpret_cast<const v4d *>(pP + ch.pb + g));  // x, y, z, p
         C R[nt];
         if (valid) {
             hermite_R<L, C>((C)P.w, (C)(P.x - px), (C)(P.y - py), (C)(P.z - pz), ctab, R);
@@ -672,7 +693,7 @@ __device__ __forceinline__ void coulomb_tile_fp(FpTileLds<T, NFB> &L, const Chun
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
         const bool valid = k < M;
-        const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
+        const double px = valid ? gload(X + k) : 0.0, py = valid ? gload(Y + k) : 0.0, pz = valid ? gload(Z + k) : 0.0;
         const RType<T> *ctab = sizeof(T) == 8 ? (const RType<T> *)tab : (const RType<T> *)(tab + kBoysRows * kBoysCols);
         produce_R<T, LDR, 0, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunk_at(chunks, 0), pP, ctab, Rl[0]);
         __syncthreads();
@@ -776,7 +797,7 @@ __device__ __forceinline__ void coulomb_tile_fp(FpTileLds<T, NFB> &L, const Chun
             for (int r = 0; r < 4; ++r) {
                 const int64_t k = k0 + pw + i * 16 + Tr::row(lr, r);
                 const double v = sizeof(T) == 4 ? sum[i][j][r] : (double)acc[i][j][r];
-                if (f < nF && k < M) V[(int64_t)f * ldv + k] = v;
+                if (f < nF && k < M) gstore(V + (int64_t)f * ldv + k, v);
             }
         }
     }
@@ -832,6 +853,75 @@ __global__ void k_split_h(const ChunkDev *__restrict__ chunks, const double *__r
     }
 }
 
+// Far-chunk bits, one block of 256 threads per 64-point tile.  The tile's centroid o and radius rho (largest distance
+// of its points from o) bound every point's distance to a group centre P from below by |P - o| - rho, so
+// p (|P - o| - rho)^2 bounds T = p|r - P|^2 of every (point, group) pair; a chunk is far when the bound of each of its
+// groups exceeds tfar.  Each wave assembles 64 chunk bits with one ballot.
+__global__ void __launch_bounds__(256) k_far_bits(const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int64_t M,
+                                                  const ChunkDev *__restrict__ chunks, int nchunks,
+                                                  const double4 *__restrict__ pP, double tfar, int farw,
+                                                  unsigned *__restrict__ far, double *__restrict__ farrows) {
+    __shared__ double geo[4];
+    const int64_t t = blockIdx.x, k0 = t * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave == 0) {
+        const int64_t k = k0 + lane;
+        const bool valid = k < M;
+        const double x = valid ? X[k] : 0.0, y = valid ? Y[k] : 0.0, z = valid ? Z[k] : 0.0;
+        double sx = x, sy = y, sz = z, n = valid ? 1.0 : 0.0;
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o);
+            sy += __shfl_xor(sy, o);
+            sz += __shfl_xor(sz, o);
+            n += __shfl_xor(n, o);
+        }
+        const double cx = sx / n, cy = sy / n, cz = sz / n;
+        double d = valid ? sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz)) : 0.0;
+        for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o));
+        if (lane == 0) {
+            geo[0] = cx;
+            geo[1] = cy;
+            geo[2] = cz;
+            geo[3] = d;
+        }
+    }
+    __syncthreads();
+    const double cx = geo[0], cy = geo[1], cz = geo[2], rho = geo[3];
+    double rows = 0.0;
+    for (int base = wave * 64; base < farw * 32; base += 256) {
+        const int c = base + lane;
+        bool is_far = false;
+        if (c < nchunks) {
+            const ChunkDev ch = chunks[c];
+            double tmin = INFINITY;
+            for (int g = ch.pb; g < ch.pe; ++g) {
+                const double4 q = pP[g];
+                const double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+                const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz) - rho, 0.0);
+                tmin = fmin(tmin, q.w * d * d);
+            }
+            is_far = tmin > tfar;
+            if (is_far) rows += ch.kr;
+        }
+        const unsigned long long m = __ballot(is_far);
+        const int w = base / 32;
+        if (lane == 0) far[t * farw + w] = (unsigned)m;
+        if (lane == 1 && w + 1 < farw) far[t * farw + w + 1] = (unsigned)(m >> 32);
+    }
+    for (int o = 32; o > 0; o >>= 1) rows += __shfl_xor(rows, o);
+    if (lane == 0 && rows > 0.0) atomicAdd(farrows, rows);
+}
+
+void launch_far_bits(const double *x, const double *y, const double *z, int64_t M, const ChunkDev *chunks,
+                     int nchunks, const double4 *pairP, double tfar, int farw, unsigned *far, double *farrows,
+                     hipStream_t s) {
+    launch_fill_zero(farrows, 1, s);
+    if (M <= 0 || farw <= 0) return;
+    hipLaunchKernelGGL(k_far_bits, dim3((unsigned)((M + 63) / 64)), dim3(256), 0, s, x, y, z, M, chunks, nchunks,
+                       pairP, tfar, farw, far, farrows);
+}
+
 // A operand of one K step: the 8 FP32 rows of this lane's point split into NP bf16 pieces, each the rounded
 // remainder of the previous ones, so that the pieces shrink by 2^-8 each and the last remainder is below 2^-8 of
 // the last piece.  (An exact split by truncation, keeping the upper 16 bits of each remainder, is 1% faster here
@@ -859,16 +949,19 @@ struct HUnits {
     static constexpr int UPP = NFB / 16, NU = NP * UPP, UPW = (NU + NCPW - 1) / NCPW;
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     v4u r[UPW];
+    // npe: pieces of this sub-step read from H (NP, or 2 for a far chunk, whose third piece is staged as zeros)
     __device__ __forceinline__ void load(const ChunkDev *__restrict__ chunks, SubStep p, const __bf16 *__restrict__ Hs,
-                                         int64_t nsteps, int nct, int fb, int cw, int lane) {
+                                         int64_t nsteps, int nct, int fb, int cw, int lane, int npe) {
         const int64_t g = chunk_at(chunks, p.c).prow0 / 32 + p.q;
 #pragma unroll
         for (int k = 0; k < UPW; ++k) {
             const int u = cw + k * NCPW;
-            if (u < NU) {
+            if (u < npe * UPP) {
                 const int pc = u / UPP, v = u - pc * UPP;
-                r[k] = *(const v4u *)((const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512) + v * 1024 +
-                                      lane * 16);
+                r[k] = gload((const v4u *)((const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512) +
+                                           v * 1024 + lane * 16));
+            } else if (u < NU) {
+                r[k] = v4u{0u, 0u, 0u, 0u};
             }
         }
     }
@@ -889,8 +982,38 @@ struct Bf16TileLds {
     __attribute__((aligned(16))) float Rf[2][64 * LDK];
     __attribute__((aligned(16))) __bf16 Hl[kSlotsReg][SLOT];
     __attribute__((aligned(16))) float Tb[kBoysRows * kBoysCols];
+    unsigned Fl[NP == 3 ? kFarWords : 1];  // far-chunk bits of this tile (three-piece form)
 };
 
+// One 32-row K step of a consumer wave (16 points x NFB/2 columns): the A operand split into NPE pieces and the
+// products of the pieces down to order 2^-8(NPE-1): 3 (a0b0, a0b1, a1b0) for NPE = 2, plus a0b2, a1b1, a2b0 for
+// NPE = 3 (zero for a0b2 in a far chunk, whose staged third H piece is zero)
+template <int NPE, int NFB>
+__device__ __forceinline__ void bf16_substep(const float *Rrow, const __bf16 *Hs_, int cl0, int lane,
+                                             v4f (&acc)[NFB / 32]) {
+    v8bf a[NPE];
+    split_a<NPE>(Rrow, a);
+#pragma unroll
+    for (int j = 0; j < NFB / 32; ++j) {
+        v8bf b[NPE];
+#pragma unroll
+        for (int p = 0; p < NPE; ++p) b[p] = *(const v8bf *)(Hs_ + ((p * (NFB / 16) + cl0 + j) * 64 + lane) * 8);
+        v4f cc = acc[j];
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], cc, 0, 0, 0);
+        if constexpr (NPE == 3) {
+            cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], cc, 0, 0, 0);
+            cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], cc, 0, 0, 0);
+            cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], cc, 0, 0, 0);
+        }
+        acc[j] = cc;
+    }
+}
+
+// far: null, or the far-chunk bits of every tile (farw 32-bit words per tile, bit c of a tile's row set where every
+// (point, group) pair of tile and chunk c has T = p|r - P|^2 above the far threshold): the three-piece form reads
+// two H pieces of those chunks from memory and stages the third as zeros (H to ~2^-17 there, R still in three)
 template <int NP, int NFB>
 __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const ChunkDev *__restrict__ chunks,
                                                   int nchunks, int64_t nsteps,
@@ -898,7 +1021,8 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
                                                   int nct, const double *__restrict__ X, const double *__restrict__ Y,
                                                   const double *__restrict__ Z, int64_t M, int nF,
                                                   double *__restrict__ V, int64_t ldv,
-                                                  const double *__restrict__ tab, int64_t tile, int cb) {
+                                                  const double *__restrict__ tab, int64_t tile, int cb,
+                                                  const unsigned *__restrict__ far, int farw) {
     constexpr int TP = 64, NCW = NFB / 32, SR = 32;  // consumer wave: 16 points x NFB/2 columns
     constexpr int NCPW = NP == 3 ? 2 : 1, NRW = kConsumerWave0 - NCPW;
     constexpr int LDK = Bf16TileLds<NP, NFB>::LDK;
@@ -909,15 +1033,24 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool use_far = NP == 3 && far != nullptr;
     {  // the table in LDS keeps the R waves' per-lane table lookups off the load path of the H copies
         const v4f *src = (const v4f *)(tab + kBoysRows * kBoysCols);
         for (int i = threadIdx.x; i < kBoysRows * kBoysCols / 4; i += kCoulThreads) ((v4f *)Tb)[i] = src[i];
+        if (use_far)
+            for (int i = threadIdx.x; i < farw; i += kCoulThreads) L.Fl[i] = gload(far + tile * farw + i);
         __syncthreads();
     }
+    // H pieces read from memory for chunk c (uniform)
+    auto pieces = [&](int c) -> int {
+        if (!use_far) return NP;
+        const unsigned w = __builtin_amdgcn_readfirstlane(L.Fl[c >> 5]);
+        return (w >> (c & 31)) & 1u ? 2 : NP;
+    };
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
         const bool valid = k < M;
-        const double px = valid ? X[k] : 0.0, py = valid ? Y[k] : 0.0, pz = valid ? Z[k] : 0.0;
+        const double px = valid ? gload(X + k) : 0.0, py = valid ? gload(Y + k) : 0.0, pz = valid ? gload(Z + k) : 0.0;
         produce_R<float, LDK, 2, 1>(wave, NRW, lane, px, py, pz, valid, 0, 1, chunk_at(chunks, 0), pP, Tb, Rf[0]);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
@@ -933,12 +1066,12 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
         const int cw = wave - NRW;
         HUnits<NP, NFB, NCPW> h;
         SubStep nxt{0, 0};
-        h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
+        h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(0));
         h.store(Hl[0], cw, lane);
         advance(nxt, chunks, SR);
         bool held = nxt.c < nchunks;  // h holds the next sub-step
         if (held) {
-            h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
+            h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(nxt.c));
             advance(nxt, chunks, SR);
         }
         wait_lgkm0();
@@ -950,7 +1083,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
                 if (held) h.store(Hl[(gs + 1) % kSlotsReg], cw, lane);
                 held = nxt.c < nchunks;
                 if (held) {
-                    h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane);
+                    h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(nxt.c));
                     advance(nxt, chunks, SR);
                 }
                 // the stores must have landed; the loads just issued stay in flight across the barrier
@@ -978,26 +1111,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
         const int nsub = chunk_at(chunks, c).kr / SR;
         const float *R = Rf[c & 1];
         for (int q = 0; q < nsub; ++q, ++gs) {
-            const __bf16 *Hs_ = Hl[gs % kSlotsReg];
-            v8bf a[NP];
-            split_a<NP>(R + (pw + lc) * LDK + SR * q + 8 * lr, a);
-#pragma unroll
-            for (int j = 0; j < NCW; ++j) {
-                v8bf b[NP];
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    b[p] = *(const v8bf *)(Hs_ + ((p * (NFB / 16) + cl0 + j) * 64 + lane) * 8);
-                v4f cc = acc[j];
-                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], cc, 0, 0, 0);
-                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], cc, 0, 0, 0);
-                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], cc, 0, 0, 0);
-                if constexpr (NP == 3) {
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], cc, 0, 0, 0);
-                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], cc, 0, 0, 0);
-                }
-                acc[j] = cc;
-            }
+            bf16_substep<NP, NFB>(R + (pw + lc) * LDK + SR * q + 8 * lr, Hl[gs % kSlotsReg], cl0, lane, acc);
             if (q + 1 == nsub) {
 #pragma unroll
                 for (int j = 0; j < NCW; ++j)
@@ -1016,7 +1130,7 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t k = k0 + pw + 4 * lr + r;
-            if (f < nF && k < M) V[(int64_t)f * ldv + k] = sum[j][r];
+            if (f < nF && k < M) gstore(V + (int64_t)f * ldv + k, sum[j][r]);
         }
     }
     }
@@ -1026,7 +1140,7 @@ template <int NP, int NFB>
 __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16(CoulJob<__bf16> J, const double *__restrict__ tab) {
     __shared__ Bf16TileLds<NP, NFB> lds;
     coulomb_tile_bf16<NP, NFB>(lds, J.chunks, J.nchunks, J.nsteps, J.pP, (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M,
-                               J.nF, J.V, J.ldv, tab, blockIdx.x, blockIdx.y);
+                               J.nF, J.V, J.ldv, tab, blockIdx.x, blockIdx.y, J.far, J.farw);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -1047,7 +1161,7 @@ __device__ __forceinline__ CoulJob<T> uniform_job(const CoulJob<T> *__restrict__
     const unsigned *src = reinterpret_cast<const unsigned *>(jobs + j);
     unsigned w[NW];
 #pragma unroll
-    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(cload(src + i));
     CoulJob<T> J;
     __builtin_memcpy(&J, w, sizeof(J));
     return J;
@@ -1062,11 +1176,11 @@ __device__ __forceinline__ int job_of_block(const int64_t *__restrict__ off, int
     int lo = 0, hi = njobs - 1;  // largest j with off[j] <= it
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (uniform_i64(off[mid]) <= it) lo = mid;
+        if (uniform_i64(cload(off + mid)) <= it) lo = mid;
         else hi = mid - 1;
     }
     const int j = __builtin_amdgcn_readfirstlane(lo);
-    local = it - uniform_i64(off[j]);
+    local = it - uniform_i64(cload(off + j));
     return j;
 }
 
@@ -1119,7 +1233,8 @@ __global__ void __launch_bounds__(kCoulThreads) k_coulomb_ws_bf16_batch_all(cons
     const int cb = (int)(local - tile * J.ncb);
 #define FLOSIC_TILE_BF(NFB)                                                                                         \
     coulomb_tile_bf16<NP, NFB>(*reinterpret_cast<Bf16TileLds<NP, NFB> *>(raw), J.chunks, J.nchunks, J.nsteps, J.pP, \
-                               (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab, tile, cb)
+                               (const __bf16 *)J.H, J.nct, J.X, J.Y, J.Z, J.M, J.nF, J.V, J.ldv, tab, tile, cb,  \
+                               J.far, J.farw)
     switch (J.nfb) {
         case 64: FLOSIC_TILE_BF(64); break;
         case 128: FLOSIC_TILE_BF(128); break;
@@ -1179,10 +1294,12 @@ void launch_split_h(int npieces, const ChunkDev *chunks, int nchunks, const doub
 
 void launch_coulomb_bf16(int npieces, const ChunkDev *chunks, int nchunks, int64_t nsteps, const double4 *pairP,
                          const __bf16 *Hs, int ldh, const double *x, const double *y, const double *z, int64_t M,
-                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb) {
+                         const double *btab, int nF, double *V, int64_t ldv, hipStream_t s, int nfb,
+                         const unsigned *far, int farw) {
     if (nchunks <= 0 || M <= 0) return;
     if (nfb <= 0) nfb = coulomb_cols_split(nF);
-    const CoulJob<__bf16> J{chunks, nchunks, pairP, Hs, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb};
+    const CoulJob<__bf16> J{chunks, nchunks, pairP, Hs, ldh / 16, x, y, z, M, nF, V, ldv, nsteps, ldh / nfb, nfb,
+                            npieces == 3 ? far : nullptr, npieces == 3 ? farw : 0};
     const dim3 grid((unsigned)((M + 63) / 64), (unsigned)(ldh / nfb)), blk(kCoulThreads);
 #define FLOSIC_WSB(NP, NFB) hipLaunchKernelGGL((k_coulomb_ws_bf16<NP, NFB>), grid, blk, 0, s, J, btab)
 #define FLOSIC_WSB_NFB(NP)                        \

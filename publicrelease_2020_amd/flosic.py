@@ -74,6 +74,7 @@ def library():
             "flosic_get_counts": [vp, C.c_int32, _lp],
             "flosic_set_mesh_block": [vp, C.c_int64],
             "flosic_set_coulomb_cols": [vp, C.c_int32],
+            "flosic_set_far_threshold": [vp, C.c_double],
             "flosic_batch_create": [C.c_int32, C.POINTER(vp), C.c_int32, C.POINTER(vp)],
             "flosic_batch_evaluate": [vp, _dp],
             "flosic_batch_get_timings": [vp, C.c_int32, _dp],
@@ -98,6 +99,7 @@ def exported_symbols() -> list:
             "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_apotnl_dft_noalloc", "flosic_siclag_der", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
             "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
+            "flosic_set_far_threshold",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
             "flosic_batch_get_counts", "flosic_batch_destroy"]
 
@@ -291,9 +293,14 @@ class Flosic:
         return t
 
     def counts(self) -> np.ndarray:
-        c = np.zeros(9, np.int64)
-        self._chk(self._L.flosic_get_counts(self._h, 9, c.ctypes.data_as(_lp)))
+        c = np.zeros(10, np.int64)
+        self._chk(self._L.flosic_get_counts(self._h, 10, c.ctypes.data_as(_lp)))
         return c
+
+    def set_far_threshold(self, tfar: float):
+        """Three-piece split-BF16 form: (tile, chunk) pairs with every T = p|r-P|^2 > tfar run with two pieces;
+        tfar <= 0 keeps three pieces everywhere."""
+        self._chk(self._L.flosic_set_far_threshold(self._h, float(tfar)))
 
     def set_coulomb_cols(self, ncols: int):
         """FLO columns per Coulomb block: 0 widest, -1 autotune, or 64/128/192/256."""

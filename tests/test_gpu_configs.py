@@ -222,3 +222,61 @@ def test_batch_members_graph_and_gate():
         batch.close()
         for e in engines + single + fp64:
             e.close()
+
+
+def test_batch_workload_all_members_coarse_mesh():
+    """All 48 systems of the batch workload (their atoms, basis, FLOs and FODs; Becke meshes of 16 x 50 instead of
+    60 x 194 points per atom, 1.7 M points in all) as one mixed-form batch in mesh-block mode: one Coulomb launch for
+    all of them, the captured and replayed graph equal to the eager batch and to the systems evaluated one at a time
+    (1e-13), and every system within the gate of its plain FP64 evaluation."""
+    import time
+    F = _F()
+    t0 = time.perf_counter()
+    systems = []
+    for i in range(48):  # host-side generation, about a minute
+        systems.append(S.make_config(5, index=i, nrad=16, nang=50))
+        if i % 8 == 7:
+            print(f"{i + 1} members generated ({time.perf_counter() - t0:.0f} s)", flush=True)
+    print(f"48 batch members: K {[s.meta['K'] for s in systems]}, {sum(s.mesh.npts for s in systems)} points, "
+          f"{sum(sum(s.nocc) for s in systems)} FLOs", flush=True)
+
+    def ctx(sy, prec):
+        e = F.Flosic(0)
+        e.load_system(sy, prec)
+        e.set_mesh_block(8192)
+        return e
+
+    def forces(e):
+        return np.concatenate([e.fod_forces(1), e.fod_forces(2)])
+
+    engines = [ctx(sy, F.PREC_MIXED) for sy in systems]
+    batch = F.FlosicBatch(engines, use_graph=True)
+    try:
+        z_eager = batch.evaluate()
+        z_capt = batch.evaluate()
+        z_rep = batch.evaluate()
+        c = batch.counts()
+        print(f"batch counts {c}, timings {batch.timings()}", flush=True)
+        assert c[1] == 1 and c[2] == 1 and c[3] == 1  # one capture, one replay, one Coulomb launch
+        f_batch = [forces(e) for e in engines]
+        z1, z64, dF = [], [], 0.0
+        for sy, fb in zip(systems, f_batch):
+            e = ctx(sy, F.PREC_MIXED)
+            z1.append(e.evaluate())
+            assert np.abs(forces(e) - fb).max() <= 1e-13 * np.abs(fb).max() + 1e-15
+            e.close()
+            e = ctx(sy, F.PREC_FP64)
+            z64.append(e.evaluate())
+            dF = max(dF, float(np.abs(forces(e) - fb).max()))
+            e.close()
+        _close(z_eager, z1, 1e-13, 0.0, "eager batch vs one at a time")
+        _close(z_capt, z1, 1e-13, 0.0, "captured batch vs one at a time")
+        _close(z_rep, z1, 1e-13, 0.0, "replayed batch vs one at a time")
+        dE = np.abs(np.asarray(z_rep) - np.asarray(z64))
+        print(f"48 members vs FP64: max |dE| {dE.max():.3e} Ha, max |dF| {dF:.3e} Ha/bohr", flush=True)
+        assert dE.max() < 1e-6
+        assert dF < 1e-5
+    finally:
+        batch.close()
+        for e in engines:
+            e.close()
