@@ -157,6 +157,20 @@ module flosic_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ncols
     end function
+    integer(c_int32_t) function flosic_diagv_fo(m, n, ham, ldh, phires, sweeps) bind(C, name="flosic_diagv_fo")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), value :: m, n, ldh
+      real(c_double) :: ham(*), phires(*)
+      integer(c_int32_t) :: sweeps
+    end function
+    integer(c_int32_t) function flosic_scf_sic_orbitals(ctx, ispn, ham_ao, ldham, coef, eval, sweeps) &
+        bind(C, name="flosic_scf_sic_orbitals")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ispn, ldham
+      real(c_double) :: ham_ao(*), coef(*), eval(*)
+      integer(c_int32_t) :: sweeps
+    end function
     integer(c_int32_t) function flosic_set_far_threshold(ctx, tfar) bind(C, name="flosic_set_far_threshold")
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -159,6 +159,30 @@ int32_t flosic_lagmult_slv(flosic_ctx *ctx, int32_t ispn, const double *pot, int
  * nbas x nbas column-major (full, symmetric). */
 int32_t flosic_sic_hamiltonian(flosic_ctx *ctx, int32_t ispn, double *hao);
 
+/* DIAGV_FO (diagv_fo.F:5-156; caller SCFSICW, scfsicw.f:548-552): the orbital rotation of the FLO-SIC SCF.  ham is
+ * the non-symmetric m x m orbital Hamiltonian, ham[J + ldh*I] = HAM(J,I), of m orbitals whose first n are occupied
+ * (the FLOs).  Its symmetric part is diagonalised (the eigenvectors, by ascending eigenvalue, start the orbitals;
+ * diagv_fo.F:48-71), then sweeps of occupied-virtual Jacobi rotations, t = atan(2 h_ij / (h_ii - h_jj)) / 2,
+ * continue until max |phi_occ^T HAM phi_virt| < 1e-6 (diagv_fo.F:74-140), at most 600.  phires[inew + m*iold] =
+ * PHIRES(INEW,IOLD) on return (the new orbitals in the old ones, after the reference's final transpose); *sweeps
+ * (may be NULL) = the converged sweep, or 0 when 600 sweeps did not converge (the reference warns and keeps the
+ * last orbitals, diagv_fo.F:148-151; FLOSIC_OK either way).  The eigenvectors' signs follow this library's
+ * eigensolver (largest component positive), so orbitals may differ from the reference's in sign only.  A host
+ * computation; no context. */
+int32_t flosic_diagv_fo(int32_t m, int32_t n, const double *ham, int32_t ldh, double *phires, int32_t *sweeps);
+
+/* SCFSICW's FLO-SIC orbital update of spin ispn (scfsicw.f:330-614, NITER > 1 without the unified Hamiltonian),
+ * from the last evaluation: C = [C_occ TMAT^T, C_virt] (the FLOs, then the virtual orbitals; scfsicw.f:330-349),
+ * HAM = C^T ham_ao C (scfsicw.f:353-371), HAM(J,I) += SIC(I,J) for every FLO J (scfsicw.f:389-395, SIC as
+ * flosic_siclagm returns it), DIAGV_FO (flosic_diagv_fo), then EVAL(i) = sum_kl PHIRES(i,k) PHIRES(i,l) HAM(l,k)
+ * (scfsicw.f:584-597) and the new orbitals C_new(:,i) = sum_k PHIRES(i,k) C(:,k) (scfsicw.f:601-614).  ham_ao: the
+ * spin's Kohn-Sham Hamiltonian in the basis (nbas x nbas, column-major, ldham >= nbas).  coef receives C_new
+ * (nbas x nwf, leading dimension nbas), eval (may be NULL) nwf values, *sweeps (may be NULL) as flosic_diagv_fo.
+ * The reference uses the full basis as orbital set (nwf = nbas); with nwf < nbas the update stays in the orbitals'
+ * span.  The transforms run on rocBLAS, DIAGV_FO on the host. */
+int32_t flosic_scf_sic_orbitals(flosic_ctx *ctx, int32_t ispn, const double *ham_ao, int32_t ldham, double *coef,
+                                double *eval, int32_t *sweeps);
+
 /* Primitive pair list built by flosic_set_basis (coupot_sic.F:228-310): npair rows of
  * (atomA, lA, bareA, atomB, lB, bareB), 0-based, in canonical order.  Pass list = NULL to query npair. */
 int32_t flosic_get_pairs(flosic_ctx *ctx, int32_t *npair, int32_t *list);

@@ -320,3 +320,47 @@ def flo_fields(sysm, ispn: int):
     cf = flo_coefs(sysm, ispn)
     chi = basis_eval(sysm.basis, sysm.mesh.xyz, 0)[0]
     return (chi @ cf).T.copy()
+
+
+def diagv_fo(ham, n: int, maxiter: int = 600, convg: float = 1.0e-6):
+    """DIAGV_FO (diagv_fo.F:5-156) restated with numpy, loop for loop: the symmetric part of the m x m orbital
+    Hamiltonian ham[J, I] = HAM(J,I) is diagonalised (DIAGGE with a unit overlap, diagv_fo.F:48-71; here this
+    oracle's eigensolver, so eigenvector signs may differ from LAPACK's), its eigenvectors by ascending eigenvalue
+    start the orbitals; each sweep rotates every (occupied i < n, virtual j >= n) pair by
+    t = atan(2 h_ij / (h_ii - h_jj)) / 2 with h_ii, h_ij, h_jj from two matrix-vector products per pair
+    (diagv_fo.F:74-112), and stops once max |phi_occ^T HAM phi_virt| < convg (diagv_fo.F:117-140).
+    Returns (phires, sweep): phires[inew, iold] = PHIRES(INEW,IOLD) after the final transpose, sweep the converged
+    sweep or 0 after maxiter sweeps."""
+    H = np.asarray(ham, np.float64)
+    m = H.shape[0]
+    _, P = eigh(0.5 * (H + H.T))
+    P = P.copy()
+    with np.errstate(divide="ignore"):
+        for sweep in range(1, maxiter + 1):
+            for i in range(n):
+                for j in range(n, m):
+                    f = H @ P[:, i]
+                    hii = P[:, i] @ f
+                    f = H @ P[:, j]
+                    hij = P[:, i] @ f
+                    hjj = P[:, j] @ f
+                    t = 0.5 * np.arctan(2.0 * hij / (hii - hjj))
+                    pi, pj = P[:, i].copy(), P[:, j].copy()
+                    P[:, i] = np.cos(t) * pi + np.sin(t) * pj
+                    P[:, j] = -np.sin(t) * pi + np.cos(t) * pj
+            if n == 0 or n == m or np.abs(P[:, :n].T @ H @ P[:, n:]).max() < convg:
+                return P.T.copy(), sweep
+    return P.T.copy(), 0
+
+
+def scf_sic_orbitals(ham_ao, coef_flo_virt, sic, n: int):
+    """SCFSICW's FLO-SIC orbital update (scfsicw.f:353-614, NITER > 1, no unified Hamiltonian) restated:
+    coef_flo_virt = C (nbas x m: the FLOs C_occ TMAT^T, then the virtual orbitals), sic[I, J] = SIC(I,J) for the
+    FLOs J < n.  HAM = C^T ham_ao C; HAM(J,I) += SIC(I,J); DIAGV_FO; EVAL(i) = phi_i^T HAM phi_i; C_new = C PHIRES^T.
+    Returns (C_new, EVAL, sweep)."""
+    Cm = np.asarray(coef_flo_virt, np.float64)
+    ham = Cm.T @ np.asarray(ham_ao, np.float64) @ Cm
+    ham[:n, :] += np.asarray(sic, np.float64)[:, :n].T
+    phi, sweep = diagv_fo(ham, n)
+    ev = np.einsum("ik,lk,il->i", phi, ham, phi)
+    return Cm @ phi.T, ev, sweep

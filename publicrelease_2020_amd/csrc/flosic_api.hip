@@ -136,6 +136,7 @@ struct flosic_ctx {
     int nchunks = 0, nchunks_bf16 = 0;
     int64_t rows_padded = 0, rows_padded_bf16 = 0;
     DevBuf<double> d_lapplane, d_lap, d_pot, d_wop, d_small, d_B, d_hao;  // LAGMULT_SLV / AO SIC Hamiltonian
+    DevBuf<double> d_scf;  // SCFSICW orbital update: ham_ao, ham_ao C, C^T ham_ao C
     bool lapplane_ok = false;
     DevBuf<float> d_H32;   // H in FP32 MFMA operand order
     DevBuf<double> d_H64;  // H in FP64 MFMA operand order
@@ -2292,6 +2293,137 @@ int32_t flosic_sic_hamiltonian(flosic_ctx *c, int32_t ispn, double *hao) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int b = 0; b < N; ++b)
         for (int a = b + 1; a < N; ++a) hao[a + (size_t)N * b] = hao[b + (size_t)N * a];
+    return FLOSIC_OK;
+}
+
+namespace {
+
+// DIAGV_FO (diagv_fo.F:5-156) on the host.  h: HAM(J,I) at h[J + ldh*I] (m x m, not symmetric); on return phi holds
+// PHIRES after its final transpose, phi[inew + m*iold].  The symmetric part is diagonalised first (DIAGGE with a unit
+// overlap, diagv_fo.F:48-71: eigenvectors by ascending eigenvalue are the starting orbitals); then sweeps over the
+// (occupied i <= n, virtual j > n) pairs rotate orbitals i and j by t = atan(2 h_ij / (h_ii - h_jj)) / 2 with
+// h_ab = phi_a^T H phi_b (diagv_fo.F:74-112), until max |phi_occ^T H phi_virt| < 1e-6 after a sweep
+// (diagv_fo.F:117-140), at most 600 sweeps.  The reference forms H phi_i and H phi_j anew for every pair; here
+// G = H Phi is formed at the start of each sweep and its columns i, j rotate with those of Phi (H is linear, so
+// this is the same arithmetic up to rounding at O(m) instead of O(m^2) per pair).  Returns the converged sweep, or
+// 0 when 600 sweeps did not converge (the reference then warns and keeps the last orbitals, diagv_fo.F:148-151).
+int diagv_fo_host(int m, int n, const double *h, int ldh, double *phi) {
+    constexpr int kMaxSweeps = 600;
+    constexpr double kConvg = 1.0e-6;
+    std::vector<double> avg((size_t)m * m), q(m), P((size_t)m * m), G((size_t)m * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) avg[i + (size_t)m * j] = 0.5 * (h[i + (size_t)ldh * j] + h[j + (size_t)ldh * i]);
+    eigh_sym(m, avg.data(), q.data(), P.data());  // P column k: orbital k in the old basis
+    auto form_g = [&]() {  // G = H P
+        std::fill(G.begin(), G.end(), 0.0);
+        for (int k = 0; k < m; ++k)
+            for (int l = 0; l < m; ++l) {
+                const double plk = P[l + (size_t)m * k];
+                const double *hl = h + (size_t)ldh * l;
+                double *gk = &G[(size_t)m * k];
+                for (int r = 0; r < m; ++r) gk[r] += hl[r] * plk;
+            }
+    };
+    auto dot = [&](const double *a, const double *b) {
+        double acc = 0.0;
+        for (int r = 0; r < m; ++r) acc += a[r] * b[r];
+        return acc;
+    };
+    int converged = 0;
+    for (int sweep = 1; sweep <= kMaxSweeps && n > 0 && n < m; ++sweep) {
+        form_g();
+        for (int i = 0; i < n; ++i)
+            for (int j = n; j < m; ++j) {
+                double *pi = &P[(size_t)m * i], *pj = &P[(size_t)m * j];
+                double *gi = &G[(size_t)m * i], *gj = &G[(size_t)m * j];
+                const double hii = dot(pi, gi), hij = dot(pi, gj), hjj = dot(pj, gj);
+                const double t = 0.5 * std::atan(2.0 * hij / (hii - hjj));
+                const double ct = std::cos(t), st = std::sin(t);
+                for (int r = 0; r < m; ++r) {
+                    const double a = pi[r], b = pj[r];
+                    pi[r] = ct * a + st * b;
+                    pj[r] = -st * a + ct * b;
+                    const double ga = gi[r], gb = gj[r];
+                    gi[r] = ct * ga + st * gb;
+                    gj[r] = -st * ga + ct * gb;
+                }
+            }
+        form_g();
+        double worst = 0.0;  // max |phi_i^T H phi_j|, i occupied, j virtual
+        for (int j = n; j < m; ++j)
+            for (int i = 0; i < n; ++i)
+                worst = std::max(worst, std::fabs(dot(&P[(size_t)m * i], &G[(size_t)m * j])));
+        if (worst < kConvg) {
+            converged = sweep;
+            break;
+        }
+    }
+    if (n == 0 || n == m) converged = 1;  // nothing to rotate
+    for (int k = 0; k < m; ++k)
+        for (int r = 0; r < m; ++r) phi[k + (size_t)m * r] = P[r + (size_t)m * k];
+    return converged;
+}
+
+}  // namespace
+
+int32_t flosic_diagv_fo(int32_t m, int32_t n, const double *ham, int32_t ldh, double *phires, int32_t *sweeps) {
+    if (m <= 0 || n < 0 || n > m || ldh < m || !ham || !phires) return FLOSIC_EINVAL;
+    const int k = diagv_fo_host(m, n, ham, ldh, phires);
+    if (sweeps) *sweeps = k;
+    return FLOSIC_OK;
+}
+
+int32_t flosic_scf_sic_orbitals(flosic_ctx *c, int32_t ispn, const double *ham_ao, int32_t ldham, double *coef,
+                                double *eval, int32_t *sweeps) {
+    if (!c || !ham_ao || !coef) return FLOSIC_EINVAL;
+    const int s = spin_index(c, ispn);
+    if (s < 0) return FLOSIC_EINVAL;
+    int r = need_eval(c);
+    if (r) return r;
+    const Spin &sp = c->sp[s];
+    const int N = c->nbas, m = sp.nwf, n = sp.nocc;
+    if (ldham < N) return fail(c, FLOSIC_EINVAL, "flosic_scf_sic_orbitals: ldham %d < nbas %d", ldham, N);
+    if (m <= 0) return fail(c, FLOSIC_EINVAL, "flosic_scf_sic_orbitals: spin %d has no orbitals", ispn);
+    // HAM = C^T ham_ao C over the orbitals of the evaluation (FLOs, then virtual orbitals: d_ctil)
+    ALLOC(c, c->d_scf, (size_t)N * N + (size_t)N * m + (size_t)m * m);
+    double *d_h = c->d_scf.p, *d_t = d_h + (size_t)N * N, *d_ham = d_t + (size_t)N * m;
+    HIPCHK(c, hipMemcpy2DAsync(d_h, sizeof(double) * N, ham_ao, sizeof(double) * ldham, sizeof(double) * N, N,
+                               hipMemcpyHostToDevice, c->stream));
+    const double one = 1.0, zero = 0.0;
+    BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, N, m, N, &one, d_h, N,
+                             sp.d_ctil.p, N, &zero, d_t, N));
+    BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, m, m, N, &one,
+                             sp.d_ctil.p, N, d_t, N, &zero, d_ham, m));
+    std::vector<double> ham((size_t)m * m), cold((size_t)N * m), phi((size_t)m * m);
+    HIPCHK(c, hipMemcpyAsync(ham.data(), d_ham, sizeof(double) * ham.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cold.data(), sp.d_ctil.p, sizeof(double) * cold.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // HAM(J,I) += SIC(I,J), J a FLO (scfsicw.f:389-395; SIC(I,J) = <psi_I|V_J|phi_J> from SICLAGM)
+    for (int J = 0; J < n; ++J)
+        for (int I = 0; I < m; ++I) ham[J + (size_t)m * I] += sp.sic[I + (size_t)m * J];
+    const int k = diagv_fo_host(m, n, ham.data(), m, phi.data());
+    if (sweeps) *sweeps = k;
+    // EVAL(i) = sum_kl PHIRES(i,k) PHIRES(i,l) HAM(l,k) (scfsicw.f:584-597); C_new(:,i) = sum_k PHIRES(i,k) C(:,k)
+    // (scfsicw.f:601-614)
+    for (int i = 0; i < m; ++i) {
+        if (eval) {
+            double e = 0.0;
+            for (int kk = 0; kk < m; ++kk) {
+                double t = 0.0;
+                for (int l = 0; l < m; ++l) t += phi[i + (size_t)m * l] * ham[l + (size_t)m * kk];
+                e += phi[i + (size_t)m * kk] * t;
+            }
+            eval[i] = e;
+        }
+        double *ci = coef + (size_t)N * i;
+        std::fill(ci, ci + N, 0.0);
+        for (int kk = 0; kk < m; ++kk) {
+            const double w = phi[i + (size_t)m * kk];
+            const double *ck = &cold[(size_t)N * kk];
+            for (int b = 0; b < N; ++b) ci[b] += w * ck[b];
+        }
+    }
     return FLOSIC_OK;
 }
 

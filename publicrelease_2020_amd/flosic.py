@@ -75,6 +75,8 @@ def library():
             "flosic_set_mesh_block": [vp, C.c_int64],
             "flosic_set_coulomb_cols": [vp, C.c_int32],
             "flosic_set_far_threshold": [vp, C.c_double],
+            "flosic_diagv_fo": [C.c_int32, C.c_int32, _dp, C.c_int32, _dp, _ip],
+            "flosic_scf_sic_orbitals": [vp, C.c_int32, _dp, C.c_int32, _dp, _dp, _ip],
             "flosic_batch_create": [C.c_int32, C.POINTER(vp), C.c_int32, C.POINTER(vp)],
             "flosic_batch_evaluate": [vp, _dp],
             "flosic_batch_get_timings": [vp, C.c_int32, _dp],
@@ -99,13 +101,27 @@ def exported_symbols() -> list:
             "flosic_evaluate", "flosic_apotnl_sic", "flosic_apotnl_dft", "flosic_apotnl_dft_noalloc", "flosic_siclag_der", "flosic_siclagm", "flosic_sic_vab", "flosic_fod_forces",
             "flosic_fod_gradient", "flosic_get_potential", "flosic_lagmult_slv", "flosic_sic_hamiltonian",
             "flosic_get_pairs", "flosic_get_timings", "flosic_get_counts", "flosic_set_mesh_block", "flosic_set_coulomb_cols",
-            "flosic_set_far_threshold",
+            "flosic_set_far_threshold", "flosic_diagv_fo", "flosic_scf_sic_orbitals",
             "flosic_batch_create", "flosic_batch_evaluate", "flosic_batch_last_error", "flosic_batch_get_timings",
             "flosic_batch_get_counts", "flosic_batch_destroy"]
 
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+def diagv_fo(ham, n: int):
+    """DIAGV_FO (flosic_diagv_fo) of the m x m orbital Hamiltonian ham[J, I] = HAM(J,I) with n occupied orbitals:
+    (PHIRES with phires[inew, iold], converged sweep or 0)."""
+    L = library()
+    h = np.asfortranarray(np.asarray(ham, np.float64))
+    m = h.shape[0]
+    phi = np.zeros((m, m), order="F")
+    sw = C.c_int32()
+    st = L.flosic_diagv_fo(m, int(n), _d(h), m, _d(phi), C.byref(sw))
+    if st:
+        raise FlosicError(st, "flosic_diagv_fo")
+    return np.ascontiguousarray(phi), sw.value
 
 
 class Flosic:
@@ -279,6 +295,18 @@ class Flosic:
         h = np.zeros(self.nbas * self.nbas)
         self._chk(self._L.flosic_sic_hamiltonian(self._h, ispn, _d(h)))
         return h.reshape(self.nbas, self.nbas).T.copy()
+
+    def scf_sic_orbitals(self, ispn: int, ham_ao):
+        """SCFSICW's FLO-SIC orbital update (flosic_scf_sic_orbitals): new orbitals (nbas x nwf), EVAL (nwf) and the
+        DIAGV_FO sweep count from the spin's Kohn-Sham Hamiltonian in the basis (nbas x nbas)."""
+        nwf = self._nwf[ispn - 1]
+        h = np.asfortranarray(np.asarray(ham_ao, np.float64))
+        assert h.shape == (self.nbas, self.nbas)
+        coef = np.zeros((self.nbas, nwf), order="F")
+        ev = np.zeros(nwf)
+        sw = C.c_int32()
+        self._chk(self._L.flosic_scf_sic_orbitals(self._h, ispn, _d(h), self.nbas, _d(coef), _d(ev), C.byref(sw)))
+        return np.ascontiguousarray(coef), ev, sw.value
 
     def pairs(self) -> np.ndarray:
         n = C.c_int32()

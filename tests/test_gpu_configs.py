@@ -280,3 +280,37 @@ def test_batch_workload_all_members_coarse_mesh():
         batch.close()
         for e in engines:
             e.close()
+
+
+def test_batch_fp64_against_oracle_on_mesh_samples():
+    """Three members of the batch workload (K = 15 LSDA, 4 PBE, 12 PBE) on mesh samples of a few hundred points,
+    evaluated together as one FP64 batch (one Coulomb launch for all three), against the oracle system by system:
+    E_SIC to 1e-10 Ha, RESULTS, SIC and V^i_ab to 1e-10 relative, DEBDAX to 1e-9 Ha/bohr."""
+    from oracle import oracle as O
+    F = _F()
+    systems = [S.subsample_mesh(S.make_config(5, index=i), 400) for i in (1, 4, 11)]
+    refs = [O.evaluate(sy, nthreads=16) for sy in systems]
+    assert all(r["status"] == 0 for r in refs)
+    engines = []
+    for sy in systems:
+        e = F.Flosic(0)
+        e.load_system(sy, F.PREC_FP64)
+        engines.append(e)
+    batch = F.FlosicBatch(engines)
+    try:
+        z = batch.evaluate()
+        assert batch.counts()[3] == 1
+        for sy, e, r, zz in zip(systems, engines, refs, z):
+            got = e.results()
+            print(f"{sy.name}: M={sy.mesh.npts} E_SIC {zz:.12f} (oracle {r['esic']:.12f})", flush=True)
+            assert abs(zz - r["esic"]) < 1e-10
+            for s in range(2):
+                g, o = got["spins"][s], r["spins"][s]
+                _close(g["results"], o["results"], 1e-10, 1e-12, "RESULTS")
+                _close(g["sic"], o["sic"], 1e-10, 1e-12, "SIC")
+                _close(g["vab"], o["vab"], 1e-10, 1e-12, "V^i_ab")
+                _close(g["debdax"], o["debdax"], 0.0, 1e-9, "DEBDAX")
+    finally:
+        batch.close()
+        for e in engines:
+            e.close()
