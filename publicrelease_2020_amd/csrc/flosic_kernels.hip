@@ -311,7 +311,13 @@ void launch_hermite(int ngroup, const PairDev *pairs, const double *dps, int nF,
 template <int TOP, typename C>
 __device__ __forceinline__ void boys(C T, C eT, const C *__restrict__ tab, C *F) {
     if (T > C(16.0)) {
-        const C rect1 = C(0.5) / T;
+        // FP32: the hardware reciprocal (1 ulp) instead of a correctly rounded division, well inside the FP32 R's
+        // own rounding; FP64 keeps the division of the reference scheme
+        C rect1;
+        if constexpr (sizeof(C) == 4)
+            rect1 = 0.5f * __builtin_amdgcn_rcpf(T);
+        else
+            rect1 = C(0.5) / T;
         F[0] = rect1 * (sqrt(T) * C(1.77245385090551602728) - eT);
 #pragma unroll
         for (int m = 1; m <= TOP; ++m) F[m] = rect1 * ((2 * m - 1) * F[m - 1] - eT);
@@ -319,7 +325,10 @@ __device__ __forceinline__ void boys(C T, C eT, const C *__restrict__ tab, C *F)
         const int it = (int)(C(100.0) * T + C(1.5));
         const C dt = C(1.0e-2) * (it - 1) - T;
         const C *f = tab + (it - 1) * kBoysCols;
-        F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] / C(2.0) + dt * (f[TOP + 3] / C(6.0))));
+        if constexpr (sizeof(C) == 4)  // FP32: times the rounded 1/6 (a correctly rounded division costs 10 VALU)
+            F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] * 0.5f + dt * (f[TOP + 3] * (1.0f / 6.0f))));
+        else
+            F[TOP] = f[TOP] + dt * (f[TOP + 1] + dt * (f[TOP + 2] / C(2.0) + dt * (f[TOP + 3] / C(6.0))));
         const C rect1 = C(2.0) * T;
 #pragma unroll
         for (int m = TOP - 1; m >= 0; --m) {
@@ -462,6 +471,12 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kCoulThreads = 1024;  // 16 wavefronts: R waves, copy waves, then 8 consumer waves
 constexpr int kConsumerWave0 = 8;
 constexpr int kSlots = 3;     // FP operand stream: ring of LDS slots filled by asynchronous global->LDS loads
+// split-BF16: the consumers' FP32 accumulators go into their FP64 sums once at least this many Hermite rows have
+// been accumulated (at chunk ends, and after the last chunk).  On the K=64 workload (same box): every 64 rows
+// 1109 ms, 128 rows 1083 ms, 256 rows 1060-1070 ms, 512 rows 1053 ms, with |dE| against FP64 of 4.9e-7, 5.3e-7,
+// 4.4e-7 and 6.2e-7 Ha; the flush (24 FP64 conversions and adds per consumer lane) is VALU work that the kernel's
+// sub-steps wait for.  (The FP32 MFMA form keeps its flush per chunk: every 256 rows saved it under 1% there.)
+constexpr int kFlushRowsBf16 = 256;
 constexpr int kSlotsReg = 2;  // split-BF16: H staged through the copy waves' registers, the sub-step in use and the next
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -596,13 +611,10 @@ __device__ __forceinline__ void chunk_R(int rw, int nrw, int lane, double px, do
     const int glo = np * q / nq, ghi = np * (q + 1) / nq;
     for (int g = glo + rw; g < ghi; g += nrw) {
         const v4d P = gload(reinterpret_cast<const v4d *>(pP + ch.pb + g));  // x, y, z, p
+        // lanes past the mesh end (valid false, coordinates 0) compute finite values too: an A row feeds only its own
+        // point's output row, which is never stored, so no lane needs its R zeroed (that took nt moves per group)
         C R[nt];
-        if (valid) {
-            hermite_R<L, C>((C)P.w, (C)(P.x - px), (C)(P.y - py), (C)(P.z - pz), ctab, R);
-        } else {
-#pragma unroll
-            for (int t = 0; t < nt; ++t) R[t] = C(0.0);
-        }
+        hermite_R<L, C>((C)P.w, (C)(P.x - px), (C)(P.y - py), (C)(P.z - pz), ctab, R);
 #pragma unroll
         for (int t = 0; t < RPG; ++t) {
             if constexpr (LAYOUT == 0)
@@ -926,19 +938,29 @@ void launch_far_bits(const double *x, const double *y, const double *z, int64_t 
 // remainder of the previous ones, so that the pieces shrink by 2^-8 each and the last remainder is below 2^-8 of
 // the last piece.  (An exact split by truncation, keeping the upper 16 bits of each remainder, is 1% faster here
 // but its pieces shrink only by 2^-7, which triples the worst dropped cross product.)
+// Rows go two at a time through one packed conversion (v_cvt_pk_bf16_f32, round to nearest even like the scalar
+// cast); a piece's bf16 value is back in FP32 by a shift (low half) or a mask (high half) of the packed word, so a
+// piece of two rows costs 5 VALU operations and the split needs no repacking into the operand vector.
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef unsigned v4w __attribute__((ext_vector_type(4)));
 template <int NP>
 __device__ __forceinline__ void split_a(const float *src, v8bf (&a)[NP]) {
     const v4f x0 = *(const v4f *)src, x1 = *(const v4f *)(src + 4);
-    const float xs[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+    float r[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        float r = xs[e];
+    for (int p = 0; p < NP; ++p) {
+        v4w w;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const __bf16 hb = (__bf16)r;
-            a[p][e] = hb;
-            r -= (float)hb;
+        for (int e = 0; e < 4; ++e) {
+            const unsigned u = __builtin_bit_cast(unsigned, __builtin_convertvector((v2f){r[2 * e], r[2 * e + 1]}, v2bf));
+            w[e] = u;
+            if (p + 1 < NP) {
+                r[2 * e] -= __builtin_bit_cast(float, u << 16);
+                r[2 * e + 1] -= __builtin_bit_cast(float, u & 0xffff0000u);
+            }
         }
+        a[p] = __builtin_bit_cast(v8bf, w);
     }
 }
 
@@ -1106,13 +1128,16 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
             sum[j][r] = 0.0;
         }
     __syncthreads();
-    int gs = 0;
+    int gs = 0, since = 0;  // since: rows accumulated in FP32 since the last flush into the FP64 sums
     for (int c = 0; c < nchunks; ++c) {
-        const int nsub = chunk_at(chunks, c).kr / SR;
+        const int kr = chunk_at(chunks, c).kr, nsub = kr / SR;
         const float *R = Rf[c & 1];
+        since += kr;
+        const bool flush = since >= kFlushRowsBf16 || c + 1 == nchunks;
+        if (flush) since = 0;
         for (int q = 0; q < nsub; ++q, ++gs) {
             bf16_substep<NP, NFB>(R + (pw + lc) * LDK + SR * q + 8 * lr, Hl[gs % kSlotsReg], cl0, lane, acc);
-            if (q + 1 == nsub) {
+            if (flush && q + 1 == nsub) {
 #pragma unroll
                 for (int j = 0; j < NCW; ++j)
 #pragma unroll
