@@ -181,7 +181,7 @@ def run_form(sysm, prec_name: str, steps: int, warmup: int, coulomb_cols: int = 
 
 
 def host_threads() -> int:
-    """Host threads this process may use (its CPU affinity), at most 16 (the GPU box's CPU share per GPU)."""
+    """Host threads this process may use (its CPU affinity), at most 16 (the host cores set aside per GPU)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
