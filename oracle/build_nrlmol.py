@@ -9,8 +9,8 @@ the rest in parallel.
 
     python oracle/build_nrlmol.py [-j N] [--dropin]
 
---dropin also links oracle/_ref/nrlmol/nrlmol_flosic_hip: the same objects with apotnl_sic.o, frmorb2.o, frmiorb.o
-and frmorb_mpi.o replaced by the library's Fortran drop-ins (publicrelease_2020_amd/fortran/flosic_dropin.F90,
+--dropin also links oracle/_ref/nrlmol/nrlmol_flosic_hip: the same objects with apotnl_sic.o, frmorb2.o, frmiorb.o,
+frmorb_mpi.o and diagv_fo.o replaced by the library's Fortran drop-ins (publicrelease_2020_amd/fortran/flosic_dropin.F90,
 compiled against the reference's module files) and libflosic_hip.so: the reference program with its per-FLO SIC
 block on the GPU.
 """
@@ -31,7 +31,7 @@ EXE = os.path.join(OUT, "nrlmol_exe")
 DROPIN_EXE = os.path.join(OUT, "nrlmol_flosic_hip")
 REPO = os.path.dirname(HERE)
 PKG = os.path.join(REPO, "publicrelease_2020_amd")
-REPLACED = ["apotnl_sic.o", "frmorb2.o", "frmiorb.o", "frmorb_mpi.o"]
+REPLACED = ["apotnl_sic.o", "frmorb2.o", "frmiorb.o", "frmorb_mpi.o", "diagv_fo.o"]
 FC = "/opt/rocm/bin/amdflang"
 MKL = "/opt/conda/lib"
 LISTS = ["FTNBASICS", "FBASICS", "FBASICSMPI", "LOCALSIC", "METAGGA", "SIC", "SIC2", "SIC3", "EXTRAFIL",

@@ -9,6 +9,8 @@
 !   SICLAGM(LSPN)               frmorb_mpi.F:2  SIC(1:NWFS(LSPN),1:NFRM(LSPN),LSPN) into SICMAT
 !   SICLMSLV(...)               frmorb_mpi.F:184 the MPI worker half of SICLAGM; never reached (SICLAGM above does
 !                               the whole product), stops if called
+!   DIAGV_FO(IPREONLY)          diagv_fo.F:5    SCFSICW's occupied-virtual rotation (PHIRES of DIAGV1 from HAM of
+!                                               FOR_DIAG1); DIAGV_FO_UNI, diagv_fo.F:157, its unified-Hamiltonian form
 !
 ! They read what the replaced routines read from the reference's modules -- COMMON2 (RIDT, IFUIDT, NIDENT, NCNT,
 ! BFALP, BFCON, N_BARE, N_CON, NSPN, IGGA), MESH1 (NMSH, RMSH, WMSH), COMMON5 (PSI_COEF, NWFS), COMMON8 (N_REP,
@@ -18,8 +20,8 @@
 ! LDA-PW91 or GGA-PBE.  The HIP device is $FLOSIC_DEVICE (default 0), the precision $FLOSIC_PRECISION (0 FP64,
 ! 1 FP32-MFMA, 2 split-BF16x3; default 0).
 !
-! Compiled against the reference's module files and linked in place of apotnl_sic.o, frmorb2.o, frmiorb.o and
-! frmorb_mpi.o (oracle/build_nrlmol.py --dropin).
+! Compiled against the reference's module files and linked in place of apotnl_sic.o, frmorb2.o, frmiorb.o,
+! frmorb_mpi.o and diagv_fo.o (oracle/build_nrlmol.py --dropin).
 module flosic_dropin
   use iso_c_binding
   use flosic_hip
@@ -333,3 +335,54 @@ subroutine SICLMSLV(LSPN, LFX, MPTS, WMSA, SICP, RPTS)
   write(6, *) 'FLOSIC_HIP: SICLMSLV is not used with the library (SICLAGM computes the whole SIC matrix)'
   call stopit
 end subroutine SICLMSLV
+
+
+! DIAGV_FO(IPREONLY) (diagv_fo.F:5-156), SCFSICW's orbital rotation of the FLO-SIC SCF (scfsicw.f:556): HAM of
+! FOR_DIAG1 (leading dimension its own first extent, NDH), m = INBAS orbitals of which NORB are occupied, PHIRES of
+! DIAGV1 on return.  IPREONLY = 1 returns the pre-diagonalisation's eigenvectors as columns, as the reference does
+! before its transpose (diagv_fo.F:64-71).  The eigenvectors' signs follow the library's eigensolver, so orbitals can
+! differ from the reference's in sign only.
+subroutine DIAGV_FO(IPREONLY)
+  use flosic_dropin, only : dropin_check
+  use flosic_hip, only : flosic_diagv_fo
+  use iso_c_binding
+  use for_diag1, only : HAM
+  use diagv1, only : NORB, PHIRES
+  use global_inputs, only : INBAS
+  implicit none
+  integer, intent(in) :: IPREONLY
+  integer(c_int32_t) :: sweeps
+  integer :: m, n
+  m = INBAS
+  n = NORB
+  if (IPREONLY == 1) n = 0
+  call dropin_check(flosic_diagv_fo(int(m, c_int32_t), int(n, c_int32_t), HAM, int(size(HAM, 1), c_int32_t), &
+                                    PHIRES, sweeps), 'flosic_diagv_fo')
+  if (IPREONLY == 1) then
+    PHIRES(1:m, 1:m) = transpose(PHIRES(1:m, 1:m))
+  else if (sweeps > 0) then
+    write(6, *) 'FLOSIC_HIP: DIAGV_FO converged in sweep', sweeps
+  else
+    write(6, *) 'WARNING: DIAGV not converged!'
+  end if
+end subroutine DIAGV_FO
+
+
+! DIAGV_FO_UNI(matlen,N,M,FOHAM,PHIRES,IPREONLY) (diagv_fo.F:157-220): the unified Hamiltonian's eigenvectors, PHIRES(i,j)
+! = component j of the eigenvector of the i-th eigenvalue (ascending); the same call with no occupied orbitals
+subroutine DIAGV_FO_UNI(matlen, N, M, FOHAM, PHIRES, IPREONLY)
+  use flosic_dropin, only : dropin_check
+  use flosic_hip, only : flosic_diagv_fo
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: matlen, N, M, IPREONLY
+  real(8) :: FOHAM(matlen, matlen), PHIRES(matlen, matlen)
+  real(c_double), allocatable :: p(:, :)
+  integer(c_int32_t) :: sweeps
+  allocate(p(M, M))
+  call dropin_check(flosic_diagv_fo(int(M, c_int32_t), 0_c_int32_t, FOHAM, int(matlen, c_int32_t), p, sweeps), &
+                    'flosic_diagv_fo')
+  PHIRES = 0.0d0
+  PHIRES(1:M, 1:M) = p
+  deallocate(p)
+end subroutine DIAGV_FO_UNI

@@ -1,7 +1,7 @@
 """The reference program with its per-FLO SIC block on the GPU: nrlmol_flosic_hip is the reference's own objects
-(built from its sources by oracle/build_nrlmol.py --dropin) with APOTNL_SIC, FRMORB2, FRMIORB and SICLAGM replaced by
-the library's Fortran drop-ins (publicrelease_2020_amd/fortran/flosic_dropin.F90), which read the reference's module
-data and call the C ABI.
+(built from its sources by oracle/build_nrlmol.py --dropin) with APOTNL_SIC, FRMORB2, FRMIORB, SICLAGM and the SCF's
+DIAGV_FO replaced by the library's Fortran drop-ins (publicrelease_2020_amd/fortran/flosic_dropin.F90), which read
+the reference's module data and call the C ABI.
 
 On the config-1 H2O inputs (tests/golden/reference_runs/h2o_lsda/inputs): the reference's own SCF runs first
 (nrlmol_exe, CPU), then one FLO-SIC evaluation with forces (NONSCF) twice from the same SCF state, once by the
@@ -10,7 +10,8 @@ through the reference's own code; they agree to |dE| < 1e-10 Ha and |dF| < 1e-10
 the reference's SCF reproduces the golden fixture's SCF energy.
 
 Then the reference's FLO-SIC SCF itself (SCF-ONLY with the FODs: the SIC block every iteration, SCFSICW adding the SIC
-matrix to the Hamiltonian and DIAGGE_FO diagonalising it, scfsicw.f:290-404) runs with the drop-ins from the same
+matrix to the Hamiltonian, DIAGGE_FO on the first iteration and the library's DIAGV_FO rotation on the later ones,
+scfsicw.f:290-614) runs with the drop-ins from the same
 converged LDA state: every iteration's EDFT+SIC in SUMMARY follows the reference's own run
 (tests/golden/reference_runs/h2o_lsda_scf) to 1e-6 Ha, with the same iteration count, the final energy to 1e-7 Ha and
 the final FOD forces to 1e-6 Ha/bohr.
@@ -116,6 +117,7 @@ def test_reference_flosic_scf_with_gpu_dropins(lda_state, tmp_path):
           f"{rows[-1, 5]!r} (reference {want[-1, 5]!r}); max per-iteration |dE| "
           f"{np.abs(rows[:n, 5] - want[:n, 5]).max():.2e}", flush=True)
     assert text.count("FLOSIC_HIP: evaluation done") >= len(want)
+    assert "FLOSIC_HIP: DIAGV_FO converged" in text  # the SCF's orbital rotation ran in the library
     assert len(rows) == len(want)
     assert np.abs(rows[:, 5] - want[:, 5]).max() < 1e-6
     assert abs(rows[-1, 5] - want[-1, 5]) < 1e-7
