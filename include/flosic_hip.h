@@ -238,8 +238,9 @@ int32_t flosic_set_mesh_block(flosic_ctx *ctx, int64_t block_points);
  * getter then reads that context's results.  With use_graph the device part of the evaluation (Hermite
  * coefficients, batched Coulomb, fields, SIC matrix, V^i_ab of every system) is captured into one hipGraph on
  * the second evaluation and replayed on later ones while no context's settings or buffers change (moving FODs
- * keeps it valid); the FO/Lowdin algebra and the forces run on the host around it.  The contexts must not be
- * used elsewhere while the batch evaluates, and must outlive the batch. */
+ * keeps it valid); the FO/Lowdin algebra and the forces run on the host around it, the systems shared out over
+ * up to 8 host threads.  The contexts must not be used elsewhere while the batch evaluates, and must outlive the
+ * batch. */
 int32_t flosic_batch_create(int32_t n, flosic_ctx *const *ctxs, int32_t use_graph, flosic_batch **batch);
 int32_t flosic_batch_evaluate(flosic_batch *batch, double *zsic);
 const char *flosic_batch_last_error(const flosic_batch *batch);

@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/flosic_hip.h"
@@ -27,6 +28,38 @@ namespace {
 
 double wall_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// f(i0, i1) over [0, n) split into contiguous ranges on up to kHostThreads host threads (the calling thread takes the
+// first range).  Work started from inside a range runs on the calling thread alone.  Results do not depend on the
+// thread count: every range writes its own outputs and nothing is reduced across ranges.
+constexpr int kHostThreads = 8;
+thread_local bool t_in_range = false;
+template <typename F>
+void host_parallel(int n, F &&f) {
+    int nt = t_in_range ? 1 : std::min(n, kHostThreads);
+    const unsigned hw = std::thread::hardware_concurrency();
+    if (hw > 0) nt = std::min(nt, (int)hw);
+    if (nt <= 1) {
+        if (n > 0) f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    int t = 1;
+    try {
+        for (; t < nt; ++t)
+            th.emplace_back([&f, n, nt, t] {
+                t_in_range = true;
+                f((int)((int64_t)n * t / nt), (int)((int64_t)n * (t + 1) / nt));
+            });
+    } catch (...) {  // no thread to be had: the remaining ranges run here
+    }
+    const bool outer = t_in_range;
+    t_in_range = true;
+    f(0, (int)((int64_t)n / nt));
+    for (int u = t; u < nt; ++u) f((int)((int64_t)n * u / nt), (int)((int64_t)n * (u + 1) / nt));
+    t_in_range = outer;
+    for (auto &x : th) x.join();
 }
 
 constexpr int kNcart[3] = {1, 3, 6};
@@ -110,7 +143,7 @@ struct flosic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[11] = {};
     Status st;
     // options
     int nspn = 2, xc = FLOSIC_XC_LSDA, prec = FLOSIC_PREC_FP64;
@@ -177,6 +210,9 @@ struct flosic_ctx {
     double zsic = 0.0;
     bool evaluated = false, fo_done = false;
     bool vab_on_host = false;  // V^i_ab of the last evaluation copied to sp[].vab (on the first request)
+    // the host results and forces of a single evaluation run while the V^i_ab kernels do: their inputs are copied to the
+    // host before those kernels are queued, and ev[EV_HOST] marks the copies done
+    bool results_queued = false;
     double timings[10] = {};
     int64_t counts[10] = {};
 };
@@ -457,28 +493,12 @@ void eigh_sym(int n, const double *Ain, double *q, double *U) {
 // FRMORB2(0,0)+FRMIORB for one spin: FMAT, OVER, Lowdin, TMAT and the FLO AO coefficients.  The orbitals at
 // the FODs (psi_j(a_i) and gradients: 4 products of nocc x nbas x nocc) and the FLO coefficients
 // cflo = C_occ TMAT^T run as rocBLAS products on the context stream; the n x n algebra stays on the host.
-int frmorb_spin(flosic_ctx *c, Spin &s, double *d_cflo /* device, nbas x n col-major */) {
-    const int n = s.nocc, N = c->nbas;
+// FMAT, OVER, the Lowdin eigen-decomposition, S^-1/2 and TMAT of one spin on the host (frmorb2.F:140-290,
+// lowden_files.F) from Psi_K(j, i) = d_K psi_j at FOD i (K = 0 value, 1..3 gradient; psiK[K][i][j]); returns 0, or
+// the 1-based index of a Lowdin eigenvalue below 1e-8
+int lowdin_host(Spin &s, const std::vector<double> &psiK) {
+    const int n = s.nocc;
     const size_t nn = (size_t)n * n;
-    std::vector<double> xf(4 * (size_t)N * n), chi(4 * (size_t)N);
-    for (int i = 0; i < n; ++i) {
-        host_basis_at(c, &s.fod[3 * i], chi.data());
-        for (int K = 0; K < 4; ++K)
-            std::copy(chi.begin() + (size_t)K * N, chi.begin() + (size_t)(K + 1) * N,
-                      xf.begin() + ((size_t)K * n + i) * N);
-    }
-    int r = 0;
-    if ((r = upload(c, s.d_cocc, s.coef.data(), (size_t)N * n)) || (r = upload(c, s.d_xfod, xf))) return r;
-    ALLOC(c, s.d_psi, 4 * nn);
-    {   // Psi_K(j, i) = sum_mu C(mu, j) chi_K(mu, a_i)
-        const double one = 1.0, zero = 0.0;
-        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, N,
-                                                 &one, s.d_cocc.p, N, 0, s.d_xfod.p, N, (rocblas_stride)N * n, &zero,
-                                                 s.d_psi.p, n, (rocblas_stride)nn, 4));
-    }
-    std::vector<double> psiK(4 * nn);
-    HIPCHK(c, hipMemcpyAsync(psiK.data(), s.d_psi.p, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     s.fmat.assign(nn * 4, 0.0);
     for (int i = 0; i < n; ++i) {
         auto psi = [&](int K, int j) { return psiK[(size_t)K * nn + j + (size_t)n * i]; };
@@ -503,8 +523,7 @@ int frmorb_spin(flosic_ctx *c, Spin &s, double *d_cflo /* device, nbas x n col-m
     s.U.assign(nn, 0.0);
     eigh_sym(n, s.over.data(), s.q.data(), s.U.data());
     for (int i = 0; i < n; ++i)
-        if (s.q[i] < 1.0e-8)
-            return fail(c, FLOSIC_ELOWDIN, "Lowdin eigenvalue %d = %.3e < 1e-8 (coincident FODs?)", i + 1, s.q[i]);
+        if (s.q[i] < 1.0e-8) return i + 1;
     s.sinvh.assign(nn, 0.0);
     {
         std::vector<double> Uq(nn);  // U_ak / sqrt(q_k)
@@ -526,12 +545,6 @@ int frmorb_spin(flosic_ctx *c, Spin &s, double *d_cflo /* device, nbas x n col-m
             for (int k = 0; k < n; ++k) acc += s.sinvh[i + (size_t)n * k] * s.fmat[j + (size_t)n * k];
             s.tmat[i + (size_t)n * j] = acc;
         }
-    if ((r = upload(c, s.d_tmat, s.tmat))) return r;
-    {   // cflo(mu, i) = sum_j C(mu, j) TMAT(i, j)
-        const double one = 1.0, zero = 0.0;
-        BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, N, n, n, &one,
-                                 s.d_cocc.p, N, s.d_tmat.p, n, &zero, d_cflo, N));
-    }
     return 0;
 }
 
@@ -939,16 +952,65 @@ int check_ready(flosic_ctx *c) {
 }
 
 // Fermi orbitals of both spins; the FLO coefficients of all FLOs land in d_cflo (nbas x nF, spin 1 first)
+// FRMORB2(0,0) + FRMIORB(1,1) for both spins: the basis and its gradient at every FOD (host, FODs shared out over host
+// threads), Psi_K = C^T chi_K at the FODs (rocBLAS), the Lowdin step of the two spins on two host threads, then
+// cflo = C TMAT^T (rocBLAS)
 int run_frmorb(flosic_ctx *c) {
     const int N = c->nbas;
     ALLOC(c, c->d_cflo, (size_t)N * (c->sp[0].nocc + c->sp[1].nocc));
+    std::vector<double> xf[2];  // xf[s][(K * n + i) * N + mu]: chi_K(mu) at FOD i of spin s
+    for (int s = 0; s < 2; ++s) xf[s].assign(4 * (size_t)N * c->sp[s].nocc, 0.0);
+    const int n0 = c->sp[0].nocc, nfod = n0 + c->sp[1].nocc;
+    host_parallel(nfod, [&](int t0, int t1) {
+        std::vector<double> chi(4 * (size_t)N);
+        for (int t = t0; t < t1; ++t) {
+            const int s = t < n0 ? 0 : 1, i = t < n0 ? t : t - n0, n = c->sp[s].nocc;
+            host_basis_at(c, &c->sp[s].fod[3 * i], chi.data());
+            for (int K = 0; K < 4; ++K)
+                std::copy(chi.begin() + (size_t)K * N, chi.begin() + (size_t)(K + 1) * N,
+                          xf[s].begin() + ((size_t)K * n + i) * N);
+        }
+    });
+    int r = 0;
+    std::vector<double> psiK[2];
+    for (int s = 0; s < 2; ++s) {
+        Spin &sp = c->sp[s];
+        const int n = sp.nocc;
+        if (n == 0) continue;
+        const size_t nn = (size_t)n * n;
+        if ((r = upload(c, sp.d_cocc, sp.coef.data(), (size_t)N * n)) || (r = upload(c, sp.d_xfod, xf[s]))) return r;
+        ALLOC(c, sp.d_psi, 4 * nn);
+        // Psi_K(j, i) = sum_mu C(mu, j) chi_K(mu, a_i)
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, N,
+                                                 &one, sp.d_cocc.p, N, 0, sp.d_xfod.p, N, (rocblas_stride)N * n, &zero,
+                                                 sp.d_psi.p, n, (rocblas_stride)nn, 4));
+        psiK[s].resize(4 * nn);
+        HIPCHK(c, hipMemcpyAsync(psiK[s].data(), sp.d_psi.p, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int bad[2] = {0, 0};
+    host_parallel(2, [&](int s0, int s1) {
+        for (int s = s0; s < s1; ++s)
+            if (c->sp[s].nocc > 0) bad[s] = lowdin_host(c->sp[s], psiK[s]);
+    });
+    for (int s = 0; s < 2; ++s)
+        if (bad[s])
+            return fail(c, FLOSIC_ELOWDIN, "Lowdin eigenvalue %d = %.3e < 1e-8 (coincident FODs?)", bad[s],
+                        c->sp[s].q[bad[s] - 1]);
     int off = 0;
     for (int s = 0; s < 2; ++s) {
-        if (c->sp[s].nocc > 0) {
-            const int r = frmorb_spin(c, c->sp[s], c->d_cflo.p + (size_t)off * N);
-            if (r) return r;
+        Spin &sp = c->sp[s];
+        const int n = sp.nocc;
+        if (n > 0) {
+            if ((r = upload(c, sp.d_tmat, sp.tmat))) return r;
+            // cflo(mu, i) = sum_j C(mu, j) TMAT(i, j)
+            const double one = 1.0, zero = 0.0;
+            BLASCHK(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, N, n, n, &one,
+                                     sp.d_cocc.p, N, sp.d_tmat.p, n, &zero, c->d_cflo.p + (size_t)off * N, N));
         }
-        off += c->sp[s].nocc;
+        off += n;
     }
     c->fo_done = true;
     return 0;
@@ -995,7 +1057,7 @@ int orbitals_on_points(flosic_ctx *c, const double *planes, int64_t m, int nd, c
 }
 
 // Event slots of one evaluation
-enum { EV_START, EV_FO, EV_HERM, EV_COUL0, EV_COUL1, EV_PHI, EV_XC, EV_SIC, EV_VAB, EV_END };
+enum { EV_START, EV_FO, EV_HERM, EV_COUL0, EV_COUL1, EV_PHI, EV_XC, EV_SIC, EV_VAB, EV_END, EV_HOST };
 
 int mark(flosic_ctx *c, int slot) {
     if (c->timed) HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
@@ -1226,8 +1288,47 @@ int eval_coulomb(flosic_ctx *c) {
 // Phase C: FLOs on the mesh, XC, total potential, energies, SIC matrix and V^i_ab.  With resident planes the mesh
 // is one block; in mesh-block mode the planes, FLO fields, V_tot and U of each block are built, used and dropped,
 // and G, V^i_ab accumulate over the blocks in block order.
-int eval_fields(flosic_ctx *c) {
+// Results of the evaluation that the host needs (energy partial sums, V_H at the outermost point, SIC(j,i), and for
+// PBE SIC_ex) queued for copying to the host on the context stream
+int queue_results(flosic_ctx *c) {
+    const int nF = c->nF;
+    c->h_energy.resize((size_t)nF * 9);
+    c->h_asy.resize(nF);
+    HIPCHK(c, hipMemcpyAsync(c->h_energy.data(), c->d_energy.p, sizeof(double) * nF * 9, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_asy.data(), c->d_asy.p, sizeof(double) * nF, hipMemcpyDeviceToHost, c->stream));
+    {
+        size_t lo = 0;
+        for (int s = 0; s < 2; ++s) {
+            Spin &sp = c->sp[s];
+            const int n = sp.nocc;
+            sp.sic.resize((size_t)sp.nwf * n);
+            if (n > 0)
+                HIPCHK(c, hipMemcpyAsync(sp.sic.data(), c->d_lambda.p + lo, sizeof(double) * sp.nwf * n,
+                                         hipMemcpyDeviceToHost, c->stream));
+            lo += (size_t)sp.nwf * n;
+        }
+        c->vab_on_host = false;  // V^i_ab (n^3 per spin) stays on the device until it is asked for
+    }
+    if (c->xc == FLOSIC_XC_PBE) {
+        size_t lo = 0;
+        for (int s = 0; s < 2; ++s) {
+            Spin &sp = c->sp[s];
+            const size_t nn = (size_t)sp.nocc * sp.nocc;
+            sp.sic_ex.resize(nn);
+            if (nn) HIPCHK(c, hipMemcpyAsync(sp.sic_ex.data(), c->d_lamex.p + lo, sizeof(double) * nn,
+                                             hipMemcpyDeviceToHost, c->stream));
+            lo += nn;
+        }
+    }
+    return 0;
+}
+
+// overlap_host: with the fields resident (no mesh blocks), queue the host's results before the V^i_ab kernels so that
+// the host works on them while those kernels run (single evaluations; a batch keeps its captured region free of
+// copies to host memory)
+int eval_fields(flosic_ctx *c, bool overlap_host = false) {
     int r = 0;
+    c->results_queued = false;
     const int N = c->nbas, nF = c->nF;
     const int n0 = c->sp[0].nocc;
     const int64_t M = c->M;
@@ -1258,6 +1359,22 @@ int eval_fields(flosic_ctx *c) {
     }
     ALLOC(c, c->d_vab, vab_cnt);
     ALLOC(c, c->d_vabpart, vpart_cnt);
+    const bool defer_vab = overlap_host && !blocked;
+    auto vab_all = [&](int64_t k0, int64_t m) -> int {  // V^i_ab partials of points [k0, k0 + m), both spins
+        size_t vo = 0;
+        for (int s = 0; s < 2; ++s) {
+            const int n = c->sp[s].nocc;
+            if (n == 0) continue;
+            const int off = s == 0 ? 0 : n0;
+            const int nsl = vab_slices(m);
+            launch_vab(c->d_phi.p + (size_t)off * m, c->d_Vtot.p + (size_t)off * m, c->d_w.p + k0, m, n, nsl,
+                       c->d_vabpart.p, c->stream, c->prec != FLOSIC_PREC_FP64);
+            HIPCHK(c, hipGetLastError());
+            launch_reduce_partials(c->d_vabpart.p, 1, nsl, n * n * n, c->d_vab.p + vo, c->stream, k0 > 0);
+            vo += (size_t)n * n * n;
+        }
+        return 0;
+    };
     for (int64_t k0 = 0; k0 < M; k0 += mb) {
         const int64_t m = std::min(mb, M - k0);
         const double *planes = c->d_planes.p;
@@ -1284,18 +1401,7 @@ int eval_fields(flosic_ctx *c) {
         // G = Phi_0^T U over the points (split K, fixed-order partial sums)
         if ((r = gemm_tn_points(c, N, nF, planes, c->d_U.p, m, c->d_G.p, k0 > 0))) return r;
         if (!blocked) MARK(c, EV_SIC);
-        size_t vo = 0;
-        for (int s = 0; s < 2; ++s) {
-            const int n = c->sp[s].nocc;
-            if (n == 0) continue;
-            const int off = s == 0 ? 0 : n0;
-            const int nsl = vab_slices(m);
-            launch_vab(c->d_phi.p + (size_t)off * m, c->d_Vtot.p + (size_t)off * m, c->d_w.p + k0, m, n, nsl,
-                       c->d_vabpart.p, c->stream, c->prec != FLOSIC_PREC_FP64);
-            HIPCHK(c, hipGetLastError());
-            launch_reduce_partials(c->d_vabpart.p, 1, nsl, n * n * n, c->d_vab.p + vo, c->stream, k0 > 0);
-            vo += (size_t)n * n * n;
-        }
+        if (!defer_vab && (r = vab_all(k0, m))) return r;
     }
     if (blocked) MARK(c, EV_PHI);
     launch_reduce_partials(c->d_part.p, nF, nblk, 9, c->d_energy.p, c->stream);
@@ -1319,6 +1425,12 @@ int eval_fields(flosic_ctx *c) {
         }
     }
     if (blocked) MARK(c, EV_SIC);
+    if (defer_vab) {
+        if ((r = queue_results(c))) return r;
+        HIPCHK(c, hipEventRecord(c->ev[EV_HOST], c->stream));
+        c->results_queued = true;
+        if ((r = vab_all(0, M))) return r;
+    }
     MARK(c, EV_VAB);
     return 0;
 }
@@ -1326,35 +1438,14 @@ int eval_fields(flosic_ctx *c) {
 // Phase D: results to the host, RESULTS(1:13), forces, timings
 int eval_finish(flosic_ctx *c) {
     const int nF = c->nF;
-    c->h_energy.resize((size_t)nF * 9);
-    c->h_asy.resize(nF);
-    HIPCHK(c, hipMemcpyAsync(c->h_energy.data(), c->d_energy.p, sizeof(double) * nF * 9, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_asy.data(), c->d_asy.p, sizeof(double) * nF, hipMemcpyDeviceToHost, c->stream));
-    {
-        size_t lo = 0;
-        for (int s = 0; s < 2; ++s) {
-            Spin &sp = c->sp[s];
-            const int n = sp.nocc;
-            sp.sic.resize((size_t)sp.nwf * n);
-            if (n > 0)
-                HIPCHK(c, hipMemcpyAsync(sp.sic.data(), c->d_lambda.p + lo, sizeof(double) * sp.nwf * n,
-                                         hipMemcpyDeviceToHost, c->stream));
-            lo += (size_t)sp.nwf * n;
-        }
-        c->vab_on_host = false;  // V^i_ab (n^3 per spin) stays on the device until it is asked for
+    int r = 0;
+    if (c->results_queued) {  // copies queued ahead of the V^i_ab kernels, which may still be running
+        c->results_queued = false;
+        HIPCHK(c, hipEventSynchronize(c->ev[EV_HOST]));
+    } else {
+        if ((r = queue_results(c))) return r;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    if (c->xc == FLOSIC_XC_PBE) {
-        size_t lo = 0;
-        for (int s = 0; s < 2; ++s) {
-            Spin &sp = c->sp[s];
-            const size_t nn = (size_t)sp.nocc * sp.nocc;
-            sp.sic_ex.resize(nn);
-            if (nn) HIPCHK(c, hipMemcpyAsync(sp.sic_ex.data(), c->d_lamex.p + lo, sizeof(double) * nn,
-                                             hipMemcpyDeviceToHost, c->stream));
-            lo += nn;
-        }
-    }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < 2; ++s)
         for (double &x : c->sp[s].sic_ex) x = -x;
     // RESULTS(1:13) per FLO (apotnl_sic.F:564-623)
@@ -1386,16 +1477,18 @@ int eval_finish(flosic_ctx *c) {
             c->zsic += R[8];
         }
     }
-    for (int s = 0; s < 2; ++s) {
-        Spin &sp = c->sp[s];
-        sp.debdax.assign(3 * (size_t)sp.nocc, 0.0);
-        sp.grad.assign(3 * (size_t)sp.nocc, 0.0);
-        if (sp.nocc > 0) {
-            force_formula(sp, sp.debdax.data());
-            if (c->xc == FLOSIC_XC_PBE) force_adjoint(sp, sp.sic_ex.data(), sp.nocc, sp.grad.data());
-            else force_adjoint(sp, sp.sic.data(), sp.nwf, sp.grad.data());
+    host_parallel(2, [&](int s0, int s1) {  // the two spins' forces on two host threads
+        for (int s = s0; s < s1; ++s) {
+            Spin &sp = c->sp[s];
+            sp.debdax.assign(3 * (size_t)sp.nocc, 0.0);
+            sp.grad.assign(3 * (size_t)sp.nocc, 0.0);
+            if (sp.nocc > 0) {
+                force_formula(sp, sp.debdax.data());
+                if (c->xc == FLOSIC_XC_PBE) force_adjoint(sp, sp.sic_ex.data(), sp.nocc, sp.grad.data());
+                else force_adjoint(sp, sp.sic.data(), sp.nwf, sp.grad.data());
+            }
         }
-    }
+    });
     HIPCHK(c, hipEventRecord(c->ev[EV_END], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[EV_END]));
     // timing slots: [0] FO+Lowdin [1] orbitals on the mesh (mesh-block mode: the whole block loop) [2] Hermite
@@ -1429,7 +1522,7 @@ int eval_finish(flosic_ctx *c) {
 
 int evaluate(flosic_ctx *c) {
     int r = 0;
-    if ((r = eval_fo(c)) || (r = eval_hermite(c)) || (r = eval_coulomb(c)) || (r = eval_fields(c)) ||
+    if ((r = eval_fo(c)) || (r = eval_hermite(c)) || (r = eval_coulomb(c)) || (r = eval_fields(c, true)) ||
         (r = eval_finish(c)))
         return r;
     return 0;
@@ -1615,11 +1708,21 @@ int batch_evaluate(flosic_batch *b, double *zsic) {
     const int n = (int)b->ctxs.size();
     b->st = Status();
     BHIPCHK(b, hipSetDevice(b->device));
+    // host work of the systems (FO/Lowdin before the device region, results and forces after it) on host threads, one
+    // system per range step; a failure is reported for the first failing system in batch order
+    std::vector<int> rc(n, 0);
+    for (int i = 0; i < n; ++i) {
+        b->ctxs[i]->st = Status();
+        b->ctxs[i]->timed = true;
+    }
+    host_parallel(n, [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) rc[i] = eval_fo(b->ctxs[i]);
+    });
+    BHIPCHK(b, hipSetDevice(b->device));
+    for (int i = 0; i < n; ++i)
+        if (rc[i]) return bfail_ctx(b, i, rc[i]);
     for (int i = 0; i < n; ++i) {
         flosic_ctx *c = b->ctxs[i];
-        c->st = Status();
-        c->timed = true;
-        if ((r = eval_fo(c))) return bfail_ctx(b, i, r);
         BHIPCHK(b, hipEventRecord(c->ev[EV_FO], c->stream));
         BHIPCHK(b, hipStreamWaitEvent(b->stream, c->ev[EV_FO], 0));
     }
@@ -1652,12 +1755,18 @@ int batch_evaluate(flosic_batch *b, double *zsic) {
     }
     BHIPCHK(b, hipEventRecord(b->ev_end, b->stream));
     for (flosic_ctx *c : b->ctxs) BHIPCHK(b, hipStreamWaitEvent(c->stream, b->ev_end, 0));
+    host_parallel(n, [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            flosic_ctx *c = b->ctxs[i];
+            (void)hipSetDevice(c->device);
+            rc[i] = eval_finish(c);
+            c->timed = true;
+        }
+    });
+    BHIPCHK(b, hipSetDevice(b->device));
     for (int i = 0; i < n; ++i) {
-        flosic_ctx *c = b->ctxs[i];
-        r = eval_finish(c);
-        c->timed = true;
-        if (r) return bfail_ctx(b, i, r);
-        if (zsic) zsic[i] = c->zsic;
+        if (rc[i]) return bfail_ctx(b, i, rc[i]);
+        if (zsic) zsic[i] = b->ctxs[i]->zsic;
     }
     if (!replay && !capture) b->plan_sig = batch_sig(b);
     float ms = 0.f;
