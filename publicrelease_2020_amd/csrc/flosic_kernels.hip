@@ -704,7 +704,9 @@ __device__ __forceinline__ void coulomb_tile_fp(FpTileLds<T, NFB> &L, const Chun
     using acc_t = typename Tr::acc;
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index through readfirstlane: the compiler then knows it (and every role, group, unit and tile index
+    // derived from it) to be wave-uniform and keeps that arithmetic and the load addresses built on it scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wave < NRW) {  // R waves: lane l works on point k0 + l throughout
         const int64_t k = k0 + lane;
         const bool valid = k < M;
@@ -1057,7 +1059,9 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
     float(&Tb)[kBoysRows * kBoysCols] = L.Tb;  // FP32 Boys table for the R waves
     const int64_t k0 = tile * TP;
     const int fb = cb * NFB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index through readfirstlane: the compiler then knows it (and every role, group, unit and tile index
+    // derived from it) to be wave-uniform and keeps that arithmetic and the load addresses built on it scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool use_far = NP == 3 && far != nullptr;
     {  // the table in LDS keeps the R waves' per-lane table lookups off the load path of the H copies
         const v4f *src = (const v4f *)(tab + kBoysRows * kBoysCols);
