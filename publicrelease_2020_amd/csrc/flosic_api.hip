@@ -490,9 +490,6 @@ void eigh_sym(int n, const double *Ain, double *q, double *U) {
     }
 }
 
-// FRMORB2(0,0)+FRMIORB for one spin: FMAT, OVER, Lowdin, TMAT and the FLO AO coefficients.  The orbitals at
-// the FODs (psi_j(a_i) and gradients: 4 products of nocc x nbas x nocc) and the FLO coefficients
-// cflo = C_occ TMAT^T run as rocBLAS products on the context stream; the n x n algebra stays on the host.
 // FMAT, OVER, the Lowdin eigen-decomposition, S^-1/2 and TMAT of one spin on the host (frmorb2.F:140-290,
 // lowden_files.F) from Psi_K(j, i) = d_K psi_j at FOD i (K = 0 value, 1..3 gradient; psiK[K][i][j]); returns 0, or
 // the 1-based index of a Lowdin eigenvalue below 1e-8
@@ -951,10 +948,10 @@ int check_ready(flosic_ctx *c) {
     return 0;
 }
 
-// Fermi orbitals of both spins; the FLO coefficients of all FLOs land in d_cflo (nbas x nF, spin 1 first)
 // FRMORB2(0,0) + FRMIORB(1,1) for both spins: the basis and its gradient at every FOD (host, FODs shared out over host
-// threads), Psi_K = C^T chi_K at the FODs (rocBLAS), the Lowdin step of the two spins on two host threads, then
-// cflo = C TMAT^T (rocBLAS)
+// threads), Psi_K = C^T chi_K at the FODs (4 products of nocc x nbas x nocc per spin, rocBLAS), the n x n Lowdin
+// algebra of the two spins on two host threads, then cflo = C_occ TMAT^T (rocBLAS); the FLO coefficients of all FLOs
+// land in d_cflo (nbas x nF, spin 1 first)
 int run_frmorb(flosic_ctx *c) {
     const int N = c->nbas;
     ALLOC(c, c->d_cflo, (size_t)N * (c->sp[0].nocc + c->sp[1].nocc));
