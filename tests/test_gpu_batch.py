@@ -124,6 +124,43 @@ def test_batch_matches_single_systems():
     _close(batch.evaluate(), z_single, 1e-13)
 
 
+def test_batch_failure_names_first_failing_system():
+    """The systems' FO/Lowdin stages run on host threads: with coincident FODs in systems 1 and 3 of four, the batch
+    reports ELOWDIN for system 1 (the first in batch order), and once the FODs are set apart again it evaluates every
+    system as the single-system path does."""
+    F = _F()
+    systems = [S.make_synthetic(3, 3, 40 + i, S.XC_LSDA if i % 2 else S.XC_PBE, nrad=24, nang=50, box=5.0)
+               for i in range(4)]
+    engines = []
+    for i, sysm in enumerate(systems):
+        e = F.Flosic(0)
+        e.load_system(sysm)
+        if i in (1, 3):
+            bad = np.array(sysm.fod[0], dtype=np.float64)
+            bad[1] = bad[0]
+            e.set_fods(1, bad)
+        engines.append(e)
+    batch = F.FlosicBatch(engines)
+    with pytest.raises(F.FlosicError) as err:
+        batch.evaluate()
+    print("batch error:", err.value, flush=True)
+    assert err.value.status == F.ELOWDIN
+    assert "system 1:" in str(err.value)
+    for i in (1, 3):
+        engines[i].set_fods(1, systems[i].fod[0])
+    z = batch.evaluate()
+    single = []
+    for sysm in systems:
+        e = F.Flosic(0)
+        e.load_system(sysm)
+        single.append(e.evaluate())
+        e.close()
+    _close(z, single, 1e-13)
+    batch.close()
+    for e in engines:
+        e.close()
+
+
 def _move(systems, engines, step, rng):
     for sysm, e in zip(systems, engines):
         for s in (1, 2):
