@@ -976,17 +976,15 @@ struct HUnits {
     static constexpr int UPP = NFB / 16, NU = NP * UPP, UPW = (NU + NCPW - 1) / NCPW;
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     v4u r[UPW];
-    // npe: pieces of this sub-step read from H (NP, or 2 for a far chunk, whose third piece is staged as zeros)
-    __device__ __forceinline__ void load(const ChunkDev *__restrict__ chunks, SubStep p, const __bf16 *__restrict__ Hs,
-                                         int64_t nsteps, int nct, int fb, int cw, int lane, int npe) {
-        const int64_t g = chunk_at(chunks, p.c).prow0 / 32 + p.q;
+    // the units of the sub-step whose piece-0 run for the block's columns starts at sub_base, the pieces pstride bytes
+    // apart; npe: pieces read from H (NP, or 2 for a far chunk, whose third piece is staged as zeros)
+    __device__ __forceinline__ void load_at(const char *sub_base, int64_t pstride, int cw, int lane, int npe) {
 #pragma unroll
         for (int k = 0; k < UPW; ++k) {
             const int u = cw + k * NCPW;
             if (u < npe * UPP) {
                 const int pc = u / UPP, v = u - pc * UPP;
-                r[k] = gload((const v4u *)((const char *)(Hs + ((pc * nsteps + g) * nct + fb / 16) * 512) +
-                                           v * 1024 + lane * 16));
+                r[k] = gload((const v4u *)(sub_base + pc * pstride + v * 1024 + lane * 16));
             } else if (u < NU) {
                 r[k] = v4u{0u, 0u, 0u, 0u};
             }
@@ -1095,13 +1093,20 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
         const int cw = wave - NRW;
         HUnits<NP, NFB, NCPW> h;
         SubStep nxt{0, 0};
-        h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(0));
+        // H of sub-step g (chunk c, step q: g = prow0(c) / 32 + q, the sub-steps consecutive in the padded row space)
+        // for the block's columns starts g * nct KiB after hb; the pieces' runs are pstride bytes apart
+        const int64_t pstride = nsteps * nct * 1024;
+        const char *hb = (const char *)(Hs + (fb / 16) * 512);
+        int64_t gnext = 0;  // sub-step index of nxt
+        h.load_at(hb, pstride, cw, lane, pieces(0));
         h.store(Hl[0], cw, lane);
         advance(nxt, chunks, SR);
+        ++gnext;
         bool held = nxt.c < nchunks;  // h holds the next sub-step
         if (held) {
-            h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(nxt.c));
+            h.load_at(hb + gnext * nct * 1024, pstride, cw, lane, pieces(nxt.c));
             advance(nxt, chunks, SR);
+            ++gnext;
         }
         wait_lgkm0();
         __builtin_amdgcn_s_barrier();
@@ -1112,8 +1117,9 @@ __device__ __forceinline__ void coulomb_tile_bf16(Bf16TileLds<NP, NFB> &L, const
                 if (held) h.store(Hl[(gs + 1) % kSlotsReg], cw, lane);
                 held = nxt.c < nchunks;
                 if (held) {
-                    h.load(chunks, nxt, Hs, nsteps, nct, fb, cw, lane, pieces(nxt.c));
+                    h.load_at(hb + gnext * nct * 1024, pstride, cw, lane, pieces(nxt.c));
                     advance(nxt, chunks, SR);
+                    ++gnext;
                 }
                 // the stores must have landed; the loads just issued stay in flight across the barrier
                 wait_lgkm0();
