@@ -31,13 +31,16 @@ double wall_ms() {
 }
 
 // f(i0, i1) over [0, n) split into contiguous ranges on up to kHostThreads host threads (the calling thread takes the
-// first range).  Work started from inside a range runs on the calling thread alone.  Results do not depend on the
-// thread count: every range writes its own outputs and nothing is reduced across ranges.
+// first range); with worth false (work too small to pay for starting threads, some tens of microseconds each) it all
+// runs on the calling thread.  Work started from inside a range runs on the calling thread alone.  Results do not
+// depend on the thread count: every range writes its own outputs and nothing is reduced across ranges.
 constexpr int kHostThreads = 8;
+constexpr int64_t kThreadMinBasis = 16384;  // FODs x basis functions below which the FOD basis runs on one thread
+constexpr int kThreadMinOrbitals = 32;      // orbitals per spin below which both spins' n^3 algebra shares one thread
 thread_local bool t_in_range = false;
 template <typename F>
-void host_parallel(int n, F &&f) {
-    int nt = t_in_range ? 1 : std::min(n, kHostThreads);
+void host_parallel(int n, F &&f, bool worth = true) {
+    int nt = (t_in_range || !worth) ? 1 : std::min(n, kHostThreads);
     const unsigned hw = std::thread::hardware_concurrency();
     if (hw > 0) nt = std::min(nt, (int)hw);
     if (nt <= 1) {
@@ -958,6 +961,7 @@ int run_frmorb(flosic_ctx *c) {
     std::vector<double> xf[2];  // xf[s][(K * n + i) * N + mu]: chi_K(mu) at FOD i of spin s
     for (int s = 0; s < 2; ++s) xf[s].assign(4 * (size_t)N * c->sp[s].nocc, 0.0);
     const int n0 = c->sp[0].nocc, nfod = n0 + c->sp[1].nocc;
+    // threads only where the basis evaluations outweigh starting them (K=64: 192 FODs x 1832 functions)
     host_parallel(nfod, [&](int t0, int t1) {
         std::vector<double> chi(4 * (size_t)N);
         for (int t = t0; t < t1; ++t) {
@@ -967,7 +971,7 @@ int run_frmorb(flosic_ctx *c) {
                 std::copy(chi.begin() + (size_t)K * N, chi.begin() + (size_t)(K + 1) * N,
                           xf[s].begin() + ((size_t)K * n + i) * N);
         }
-    });
+    }, (int64_t)nfod * N >= kThreadMinBasis);
     int r = 0;
     std::vector<double> psiK[2];
     for (int s = 0; s < 2; ++s) {
@@ -991,7 +995,7 @@ int run_frmorb(flosic_ctx *c) {
     host_parallel(2, [&](int s0, int s1) {
         for (int s = s0; s < s1; ++s)
             if (c->sp[s].nocc > 0) bad[s] = lowdin_host(c->sp[s], psiK[s]);
-    });
+    }, std::min(n0, nfod - n0) >= kThreadMinOrbitals);
     for (int s = 0; s < 2; ++s)
         if (bad[s])
             return fail(c, FLOSIC_ELOWDIN, "Lowdin eigenvalue %d = %.3e < 1e-8 (coincident FODs?)", bad[s],
@@ -1474,7 +1478,7 @@ int eval_finish(flosic_ctx *c) {
             c->zsic += R[8];
         }
     }
-    host_parallel(2, [&](int s0, int s1) {  // the two spins' forces on two host threads
+    host_parallel(2, [&](int s0, int s1) {  // the two spins' forces on two host threads (n^3 work each)
         for (int s = s0; s < s1; ++s) {
             Spin &sp = c->sp[s];
             sp.debdax.assign(3 * (size_t)sp.nocc, 0.0);
@@ -1485,7 +1489,7 @@ int eval_finish(flosic_ctx *c) {
                 else force_adjoint(sp, sp.sic.data(), sp.nwf, sp.grad.data());
             }
         }
-    });
+    }, std::min(c->sp[0].nocc, c->sp[1].nocc) >= kThreadMinOrbitals);
     HIPCHK(c, hipEventRecord(c->ev[EV_END], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[EV_END]));
     // timing slots: [0] FO+Lowdin [1] orbitals on the mesh (mesh-block mode: the whole block loop) [2] Hermite
