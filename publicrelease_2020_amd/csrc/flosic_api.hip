@@ -35,8 +35,7 @@ double wall_ms() {
 // runs on the calling thread.  Work started from inside a range runs on the calling thread alone.  Results do not
 // depend on the thread count: every range writes its own outputs and nothing is reduced across ranges.
 constexpr int kHostThreads = 8;
-constexpr int64_t kThreadMinBasis = 16384;  // FODs x basis functions below which the FOD basis runs on one thread
-constexpr int kThreadMinOrbitals = 32;      // orbitals per spin below which both spins' n^3 algebra shares one thread
+constexpr int kThreadMinOrbitals = 32;  // orbitals per spin below which both spins' n^3 algebra shares one thread
 thread_local bool t_in_range = false;
 template <typename F>
 void host_parallel(int n, F &&f, bool worth = true) {
@@ -66,9 +65,6 @@ void host_parallel(int n, F &&f, bool worth = true) {
 }
 
 constexpr int kNcart[3] = {1, 3, 6};
-constexpr int kCpow[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
-                              {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
-constexpr int kLoff[3] = {0, 1, 4};
 constexpr double kZed = 1.0e-30;  // coupot_sic.F:61
 // three-piece split-BF16 Coulomb: (tile, chunk) pairs whose every T = p|r - P|^2 exceeds this run with two pieces.
 // 30 is POISSON1's THIRTYSOMETHING (poisson1.f:101): beyond RCUT = sqrt(30/DELTA_MIN) + DMAX (poisson1.f:528) the
@@ -136,7 +132,7 @@ struct Spin {
     std::vector<double> debdax, grad;  // 3 x n
     DevBuf<double> d_ctil;        // nbas x nwf: FLOs then virtual KS orbitals
     DevBuf<double> d_cocc;        // nbas x nocc: occupied KS orbitals
-    DevBuf<double> d_xfod, d_psi, d_tmat;  // basis at the FODs (4 x nbas x n), orbitals at the FODs (4 x n x n)
+    DevBuf<double> d_psi, d_tmat;  // orbitals at the FODs (4 x n x n), TMAT
 };
 
 }  // namespace
@@ -207,6 +203,7 @@ struct flosic_ctx {
     Spin sp[2];
     // evaluation buffers
     int nF = 0, ldh = 0;
+    DevBuf<double> d_fodxyz, d_fodchi;  // FOD coordinates of both spins (x, y, z arrays), basis planes at them
     DevBuf<double> d_cflo, d_dps, d_H, d_phi, d_VH, d_Vtot, d_U, d_part, d_energy, d_G, d_Gpart, d_vabpart, d_vab,
         d_lambda, d_asy, d_Ux, d_lamex, d_dft, d_dftpart, d_dplanes, d_dphi, d_dvt, d_du, d_dG;
     std::vector<double> h_energy, h_asy;
@@ -319,53 +316,6 @@ int upload(flosic_ctx *c, DevBuf<T> &b, const T *h, size_t cnt) {
 template <typename T>
 int upload(flosic_ctx *c, DevBuf<T> &b, const std::vector<T> &h) {
     return upload(c, b, h.data(), h.size());
-}
-
-// ------------------------------------------------------------------------------------------------
-// host basis at a few points (FODs): values and gradients, same screening as the mesh kernel
-// ------------------------------------------------------------------------------------------------
-void host_basis_at(const flosic_ctx *c, const double *pt, double *out /* 4 x nbas */) {
-    const double geps = std::exp(-kCutExp);
-    std::fill(out, out + 4 * c->nbas, 0.0);
-    for (int a = 0; a < c->natom; ++a) {
-        const double x = pt[0] - c->xyz[3 * a], y = pt[1] - c->xyz[3 * a + 1], z = pt[2] - c->xyz[3 * a + 2];
-        const double r2 = x * x + y * y + z * z;
-        const double r[3] = {x, y, z};
-        const int nb = c->bare_off[a + 1] - c->bare_off[a];
-        for (int l = 0; l < 3; ++l) {
-            const int nc = c->ncon[3 * a + l];
-            for (int cc = 0; cc < nc; ++cc) {
-                double S0 = 0, S1 = 0;
-                for (int i = 0; i < nb; ++i) {
-                    const double al = c->alpha[c->bare_off[a] + i];
-                    const double arg = al * r2;
-                    if (!(arg < kCutExp)) continue;
-                    const double g = std::exp(-arg);
-                    const double cf = c->con[c->con_off[3 * a + l] + cc * nb + i];
-                    if (!(std::fabs(cf) * g > geps)) continue;
-                    S0 += g * cf;
-                    S1 += g * cf * al;
-                }
-                for (int m = 0; m < kNcart[l]; ++m) {
-                    const int *pw = kCpow[kLoff[l] + m];
-                    auto ip = [](double v, int p) { return p == 0 ? 1.0 : (p == 1 ? v : v * v); };
-                    const double P = ip(x, pw[0]) * ip(y, pw[1]) * ip(z, pw[2]);
-                    const int mu = c->fstart[3 * a + l] + m * nc + cc;
-                    out[mu] = P * S0;
-                    for (int d = 0; d < 3; ++d) {
-                        int q[3] = {pw[0], pw[1], pw[2]};
-                        double dP = 0.0;
-                        if (q[d] > 0) {
-                            const int k = q[d];
-                            q[d] -= 1;
-                            dP = k * ip(x, q[0]) * ip(y, q[1]) * ip(z, q[2]);
-                        }
-                        out[(1 + d) * c->nbas + mu] = dP * S0 - 2.0 * r[d] * P * S1;
-                    }
-                }
-            }
-        }
-    }
 }
 
 // symmetric eigen-decomposition: Householder reduction to tridiagonal form, then implicit QL with Wilkinson
@@ -951,41 +901,42 @@ int check_ready(flosic_ctx *c) {
     return 0;
 }
 
-// FRMORB2(0,0) + FRMIORB(1,1) for both spins: the basis and its gradient at every FOD (host, FODs shared out over host
-// threads), Psi_K = C^T chi_K at the FODs (4 products of nocc x nbas x nocc per spin, rocBLAS), the n x n Lowdin
-// algebra of the two spins on two host threads, then cflo = C_occ TMAT^T (rocBLAS); the FLO coefficients of all FLOs
-// land in d_cflo (nbas x nF, spin 1 first)
+// FRMORB2(0,0) + FRMIORB(1,1) for both spins: the basis and its gradient at every FOD (the mesh basis kernel with the
+// FODs of both spins as its points), Psi_K = C^T chi_K at the FODs (4 products of nocc x nbas x nocc per spin,
+// rocBLAS), the n x n Lowdin algebra of the two spins on two host threads, then cflo = C_occ TMAT^T (rocBLAS); the FLO
+// coefficients of all FLOs land in d_cflo (nbas x nF, spin 1 first)
 int run_frmorb(flosic_ctx *c) {
     const int N = c->nbas;
-    ALLOC(c, c->d_cflo, (size_t)N * (c->sp[0].nocc + c->sp[1].nocc));
-    std::vector<double> xf[2];  // xf[s][(K * n + i) * N + mu]: chi_K(mu) at FOD i of spin s
-    for (int s = 0; s < 2; ++s) xf[s].assign(4 * (size_t)N * c->sp[s].nocc, 0.0);
     const int n0 = c->sp[0].nocc, nfod = n0 + c->sp[1].nocc;
-    // threads only where the basis evaluations outweigh starting them (K=64: 192 FODs x 1832 functions)
-    host_parallel(nfod, [&](int t0, int t1) {
-        std::vector<double> chi(4 * (size_t)N);
-        for (int t = t0; t < t1; ++t) {
-            const int s = t < n0 ? 0 : 1, i = t < n0 ? t : t - n0, n = c->sp[s].nocc;
-            host_basis_at(c, &c->sp[s].fod[3 * i], chi.data());
-            for (int K = 0; K < 4; ++K)
-                std::copy(chi.begin() + (size_t)K * N, chi.begin() + (size_t)(K + 1) * N,
-                          xf[s].begin() + ((size_t)K * n + i) * N);
-        }
-    }, (int64_t)nfod * N >= kThreadMinBasis);
+    ALLOC(c, c->d_cflo, (size_t)N * nfod);
     int r = 0;
+    {   // FOD coordinates of both spins as three point arrays; chi planes [K][mu][FOD], K = value, d/dx, d/dy, d/dz
+        std::vector<double> xyz(3 * (size_t)nfod);
+        for (int t = 0; t < nfod; ++t) {
+            const double *f = t < n0 ? &c->sp[0].fod[3 * t] : &c->sp[1].fod[3 * (t - n0)];
+            for (int d = 0; d < 3; ++d) xyz[(size_t)d * nfod + t] = f[d];
+        }
+        if ((r = upload(c, c->d_fodxyz, xyz))) return r;
+        ALLOC(c, c->d_fodchi, 4 * (size_t)N * nfod);
+        launch_fill_zero(c->d_fodchi.p, 4 * (size_t)N * nfod, c->stream);
+        if (nfod > 0)
+            launch_basis_planes<double>(basis_view(c), c->d_fodxyz.p, c->d_fodxyz.p + nfod, c->d_fodxyz.p + 2 * nfod,
+                                        nfod, 4, c->d_fodchi.p, nfod, c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
     std::vector<double> psiK[2];
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0, off = 0; s < 2; off += c->sp[s].nocc, ++s) {
         Spin &sp = c->sp[s];
         const int n = sp.nocc;
         if (n == 0) continue;
         const size_t nn = (size_t)n * n;
-        if ((r = upload(c, sp.d_cocc, sp.coef.data(), (size_t)N * n)) || (r = upload(c, sp.d_xfod, xf[s]))) return r;
+        if ((r = upload(c, sp.d_cocc, sp.coef.data(), (size_t)N * n))) return r;
         ALLOC(c, sp.d_psi, 4 * nn);
-        // Psi_K(j, i) = sum_mu C(mu, j) chi_K(mu, a_i)
+        // Psi_K(j, i) = sum_mu C(mu, j) chi_K(mu, a_i): C^T times the transpose of the [mu][FOD] plane of this spin's FODs
         const double one = 1.0, zero = 0.0;
-        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, N,
-                                                 &one, sp.d_cocc.p, N, 0, sp.d_xfod.p, N, (rocblas_stride)N * n, &zero,
-                                                 sp.d_psi.p, n, (rocblas_stride)nn, 4));
+        BLASCHK(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_transpose, n, n,
+                                                 N, &one, sp.d_cocc.p, N, 0, c->d_fodchi.p + off, nfod,
+                                                 (rocblas_stride)N * nfod, &zero, sp.d_psi.p, n, (rocblas_stride)nn, 4));
         psiK[s].resize(4 * nn);
         HIPCHK(c, hipMemcpyAsync(psiK[s].data(), sp.d_psi.p, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost,
                                  c->stream));
